@@ -1,0 +1,253 @@
+! ORACLE / TEST INFRASTRUCTURE ONLY.  Never linked into the product library.
+!
+! bind(C) driver over the UCLCHEM reference Fortran hot path, compiled in place
+! from the reference sources by oracle/Makefile into oracle/_ref/libucref.so.
+! It restates only the glue the reference keeps in files that do not compile
+! with amdflang (wrap.f90, hotcore.f90):
+!   * the solveAbundances time loop for points=1   (src/fortran_src/wrap.f90:598-690)
+!   * hotcore updateTargetTime / updatePhysics      (src/fortran_src/hotcore.f90:56-90)
+!   * a per-point reset of every module global the parameter parser can touch,
+!     so each grid point starts from the documented defaults
+!     (src/fortran_src/defaultparameters.f90:18-122; wrap.f90:699-983 never resets).
+! Everything numeric (rates, RHS, DVODE, chemistry driver, physics core, cloud
+! schedule) is the reference's own compiled code.
+MODULE ucref_driver
+    USE iso_c_binding
+    USE constants
+    USE defaultparameters
+    USE physicscore, only: points, dstep, cloudsize, h2crprate, &
+        & currentTime, targetTime, timeinyears, currentTimeOld, density, gastemp, dusttemp, av, coldens, &
+        & coreInitializePhysics, coreUpdatePhysics
+    USE network
+    USE f2py_constants, only: nspec, nreac
+    USE chemistry
+    USE rates, only: calculateReactionRates, lastTemp
+    USE surfacereactions, only: safeMantle, safeBulk, bulkLayersReciprocal, NUM_SITES_PER_GRAIN, &
+        & GAS_DUST_DENSITY_RATIO, bulkGainFromMantleBuildUp
+    USE postprocess_mod, only: lusecoldens, usepostprocess
+    USE cloud_mod, only: cloudInitializePhysics => initializePhysics, cloudUpdateTargetTime => updateTargetTime
+    USE DVODE_F90_M, only: GET_STATS
+    IMPLICIT NONE
+
+    ! hotcore.f90:20-22 tables (restated: hotcore.f90 itself does not compile with amdflang)
+    INTEGER, PARAMETER :: nMasses = 6
+    REAL(dp), PARAMETER :: tempa(nMasses)=(/1.927d-1,4.8560d-2,7.8470d-3,9.6966d-4,1.706d-4,4.74d-7/)
+    REAL(dp), PARAMETER :: tempb(nMasses)=(/0.5339,0.6255,0.8395,1.085,1.289,1.98/)
+    INTEGER :: modelKind = 0, tempIndx = 1
+    REAL(dp) :: maxTemp = 300.0d0
+
+    ! parameter vector layout; mirrored by oracle/oracle.py and uclchem_amd/params.py
+    INTEGER, PARAMETER :: P_DENS=1, P_TEMP=2, P_ZETA=3, P_RADFIELD=4, P_BASEAV=5, P_ROUT=6, P_RIN=7, &
+        & P_FINALTIME=8, P_FREEFALL=9, P_FINALDENS=10, P_RELTOL=11, P_ABSFAC=12, P_ABSMIN=13, &
+        & P_MODEL=14, P_TEMPINDX=15, P_MAXTEMP=16, P_FREEZEFAC=17, P_ENDATFINALDENS=18, P_FREEFALLFAC=19, &
+        & P_MXSTEP=20, P_NPARAM=24
+CONTAINS
+
+    SUBROUTINE reset_globals()
+        ! defaults exactly as declared in defaultparameters.f90:18-122
+        initialTemp=10.0; initialDens=1.00d2; finalDens=1.00d5; currentTime=0.0; finalTime=5.0d6
+        radfield=1.0; zeta=1.0; rout=0.05; rin=0.0; baseAv=2.0; points=1; bm0=1.0
+        freezeFactor=1.0; endAtFinalDensity=.False.; freefall=.False.; freefallFactor=1.0
+        desorb=.True.; h2desorb=.True.; crdesorb=.True.; uvdesorb=.True.; thermdesorb=.True.
+        instantSublimation=.False.; cosmicRayAttenuation=.False.; ionModel='L'
+        improvedH2CRPDissociation=.False.; enforceChargeConservation=.false.
+        metallicity=1.0; ion=2; fh=0.5; fhe=0.1; fc=1.77d-04; fo=3.34d-04; fn=6.18d-05; fs=3.51d-6
+        fmg=2.256d-06; fsi=1.78d-06; fcl=3.39d-08; fp=7.78d-08; ffe=2.01d-7; ff=3.6d-08; fd=0.0
+        fli=0.0; fna=0.0; fpah=0.0; f15n=0.0; f13c=0.0; f18O=0.0
+        reltol=1d-8; abstol_factor=1.0d-14; abstol_min=1.0d-25; MXSTEP=10000
+        ebmaxh2=1.21d3; ebmaxcr=1.21d3; ebmaxuvcr=1.0d4; epsilon=0.01; uv_yield=0.03; phi=1.0d5
+        uvcreff=1.0d-3; omega=0.5
+        usepostprocess=.true.; lusecoldens=.false.
+        modelKind=0; tempIndx=1; maxTemp=300.0d0
+    END SUBROUTINE reset_globals
+
+    SUBROUTINE apply_params(p)
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        initialDens=p(P_DENS); initialTemp=p(P_TEMP); zeta=p(P_ZETA); radfield=p(P_RADFIELD)
+        baseAv=p(P_BASEAV); rout=p(P_ROUT); rin=p(P_RIN); finalTime=p(P_FINALTIME)
+        freefall=(p(P_FREEFALL) .ne. 0.0d0); finalDens=p(P_FINALDENS)
+        reltol=p(P_RELTOL); abstol_factor=p(P_ABSFAC); abstol_min=p(P_ABSMIN)
+        modelKind=NINT(p(P_MODEL)); tempIndx=NINT(p(P_TEMPINDX)); maxTemp=p(P_MAXTEMP)
+        freezeFactor=p(P_FREEZEFAC); endAtFinalDensity=(p(P_ENDATFINALDENS) .ne. 0.0d0)
+        freefallFactor=p(P_FREEFALLFAC); MXSTEP=NINT(p(P_MXSTEP))
+    END SUBROUTINE apply_params
+
+    SUBROUTINE hotcoreUpdateTargetTime
+        ! hotcore.f90:56-72
+        IF (timeInYears .gt. 1.0d6) THEN
+            targetTime=(timeInYears+1.0d5)*SECONDS_PER_YEAR
+        ELSE  IF (timeInYears .gt. 1.0d5) THEN
+            targetTime=(timeInYears+1.0d4)*SECONDS_PER_YEAR
+        ELSE IF (timeInYears .gt. 1.0d4) THEN
+            targetTime=(timeInYears+1000.0)*SECONDS_PER_YEAR
+        ELSE IF (timeInYears .gt. 1000) THEN
+            targetTime=(timeInYears+100.0)*SECONDS_PER_YEAR
+        ELSE IF (timeInYears .gt. 100) THEN
+            targetTime=(timeInYears+10.0)*SECONDS_PER_YEAR
+        ELSE IF (timeInYears .gt. 0.0) THEN
+            targetTime=(timeInYears*10.0)*SECONDS_PER_YEAR
+        ELSE
+            targetTime=SECONDS_PER_YEAR*1.0d-7
+        ENDIF
+    END SUBROUTINE hotcoreUpdateTargetTime
+
+    SUBROUTINE hotcoreUpdatePhysics
+        ! hotcore.f90:78-90
+        IF (gasTemp(dstep) .lt. maxTemp) THEN
+            gasTemp(dstep)=(cloudSize/(rout*pc))*(real(dstep)/real(points))
+            gasTemp(dstep)=gasTemp(dstep)**(-0.5)
+            gasTemp(dstep)=initialTemp + ((tempa(tempindx)*(currentTime/SECONDS_PER_YEAR)**tempb(tempindx))*gasTemp(dstep))
+            if (gasTemp(dstep) .gt. maxTemp) gasTemp(dstep)=maxTemp
+        END IF
+        dustTemp=gasTemp
+    END SUBROUTINE hotcoreUpdatePhysics
+
+    SUBROUTINE setup_point(p, flag)
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        INTEGER, INTENT(OUT) :: flag
+        CALL reset_globals()
+        CALL apply_params(p)
+        dstep=1
+        currentTime=0.0
+        timeInYears=0.0
+        flag=0
+        CALL coreInitializePhysics(flag)
+        IF (flag .lt. 0) RETURN
+        CALL cloudInitializePhysics(flag)   ! hotcore's initializePhysics does the same for three-phase networks
+        IF (flag .lt. 0) RETURN
+        CALL initializeChemistry(.false.)
+    END SUBROUTINE setup_point
+
+    ! ---------------------------------------------------------------- C entry points
+
+    INTEGER(c_int) FUNCTION ucref_sizes(neq, nr) BIND(C, name="ucref_sizes")
+        INTEGER(c_int), INTENT(OUT) :: neq, nr
+        neq = nspec+1
+        nr = nreac
+        ucref_sizes = 0
+    END FUNCTION ucref_sizes
+
+    ! Set up one point (parameters, physics, elemental abundances); returns y0.
+    INTEGER(c_int) FUNCTION ucref_init(p, y0) BIND(C, name="ucref_init")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        REAL(c_double), INTENT(OUT) :: y0(nspec+1)
+        INTEGER :: flag
+        CALL setup_point(p, flag)
+        y0 = abund(:,1)
+        ucref_init = flag
+    END FUNCTION ucref_init
+
+    ! The pre-integration part of updateChemistry (chemistry.f90:169-203) on the given
+    ! state, then calculateReactionRates.  Module state (phi, lastTemp) persists
+    ! between calls exactly as in the reference.  y is updated with the BULK/SURFACE reset.
+    INTEGER(c_int) FUNCTION ucref_rates(y, rate_out, scal) BIND(C, name="ucref_rates")
+        REAL(c_double), INTENT(INOUT) :: y(nspec+1)
+        REAL(c_double), INTENT(OUT) :: rate_out(nreac), scal(8)
+        REAL(dp) :: sc
+        abund(:,1) = y
+        IF (.not. freefall) abund(nspec+1,dstep)=density(dstep)
+        h2ColToCell=0.0; coColToCell=0.0; cColToCell=0.0
+        h2Col=h2ColToCell+0.5*abund(nh2,dstep)*density(dstep)*(cloudSize/real(points))
+        coCol=coColToCell+0.5*abund(nco,dstep)*density(dstep)*(cloudSize/real(points))
+        cCol=cColToCell+0.5*abund(nc,dstep)*density(dstep)*(cloudSize/real(points))
+        abund(nBulk,dstep)=sum(abund(bulkList,dstep))
+        abund(nSurface,dstep)=sum(abund(surfaceList,dstep))
+        safeMantle=MAX(1d-30,abund(nSurface,dstep))
+        safeBulk=MAX(1d-30,abund(nBulk,dstep))
+        bulkLayersReciprocal=MIN(1.0,NUM_SITES_PER_GRAIN/(GAS_DUST_DENSITY_RATIO*safeBulk))
+        sc=bulkGainFromMantleBuildUp()
+        CALL calculateReactionRates(abund,safeMantle, h2col, cocol, ccol, rate)
+        rate_out = rate
+        y = abund(:,1)
+        scal(1)=h2col; scal(2)=cocol; scal(3)=ccol; scal(4)=safeMantle; scal(5)=safeBulk
+        scal(6)=bulkLayersReciprocal; scal(7)=phi; scal(8)=av(1)
+        ucref_rates = 0
+    END FUNCTION ucref_rates
+
+    ! RHS F(y) with the module's current rates (chemistry.f90:294-352).
+    INTEGER(c_int) FUNCTION ucref_rhs(y, ydot) BIND(C, name="ucref_rhs")
+        REAL(c_double), INTENT(IN) :: y(nspec+1)
+        REAL(c_double), INTENT(OUT) :: ydot(nspec+1)
+        REAL(dp) :: t
+        t = currentTime
+        CALL F(nspec+1, t, y, ydot)
+        ucref_rhs = 0
+    END FUNCTION ucref_rhs
+
+    ! Overwrite the module rate array (so RHS tests can use arbitrary rates).
+    INTEGER(c_int) FUNCTION ucref_set_rates(rate_in) BIND(C, name="ucref_set_rates")
+        REAL(c_double), INTENT(IN) :: rate_in(nreac)
+        rate = rate_in
+        ucref_set_rates = 0
+    END FUNCTION ucref_set_rates
+
+    INTEGER(c_int) FUNCTION ucref_set_phi(v) BIND(C, name="ucref_set_phi")
+        REAL(c_double), VALUE :: v
+        phi = v
+        ucref_set_phi = 0
+    END FUNCTION ucref_set_phi
+
+    ! Full single-point model (solveAbundances loop, wrap.f90:598-690, points=1).
+    ! rec(:, k) = (time_yr, density, gasTemp, av, y(1:nspec)) for each output row k,
+    ! row 1 being t=0.  stats = (nsteps_last, nfe_last, nje_last, n_intervals, n_dvode_calls_sum, ...)
+    INTEGER(c_int) FUNCTION ucref_model(p, maxrec, rec, nrec, stats) BIND(C, name="ucref_model")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec)
+        INTEGER(c_int), INTENT(OUT) :: nrec
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
+        INTEGER :: flag
+        REAL(dp) :: rstats(22)
+        INTEGER :: istats(31)
+        stats = 0
+        nrec = 0
+        CALL setup_point(p, flag)
+        IF (flag .lt. 0) THEN
+            ucref_model = flag
+            RETURN
+        END IF
+        CALL record_row(rec, maxrec, nrec)
+        DO WHILE ((flag .eq. 0) .and. (((endAtFinalDensity) .and. (density(1) < finalDens)) .or. &
+            &((.not. endAtFinalDensity) .and. (timeInYears < finalTime))))
+            currentTimeold=currentTime
+            timeInYears=currentTime/SECONDS_PER_YEAR
+            IF (modelKind .eq. 1) THEN
+                CALL hotcoreUpdateTargetTime
+            ELSE
+                CALL cloudUpdateTargetTime
+            END IF
+            dstep=1
+            currentTime=currentTimeold
+            CALL updateChemistry(flag)
+            IF (flag .lt. 0) EXIT
+            CALL GET_STATS(rstats, istats)
+            stats(1) = stats(1) + istats(11)
+            stats(2) = stats(2) + istats(12)
+            stats(3) = stats(3) + istats(13)
+            stats(4) = stats(4) + 1
+            timeInYears= targetTime/SECONDS_PER_YEAR
+            CALL coreUpdatePhysics
+            IF (modelKind .eq. 1) CALL hotcoreUpdatePhysics
+            CALL record_row(rec, maxrec, nrec)
+            IF (nrec .ge. maxrec) THEN
+                flag = NOT_ENOUGH_TIMEPOINTS_ERROR
+                EXIT
+            END IF
+        END DO
+        ucref_model = flag
+    END FUNCTION ucref_model
+
+    SUBROUTINE record_row(rec, maxrec, nrec)
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(INOUT) :: rec(4+nspec, maxrec)
+        INTEGER(c_int), INTENT(INOUT) :: nrec
+        IF (nrec .ge. maxrec) RETURN
+        nrec = nrec + 1
+        rec(1, nrec) = timeInYears
+        rec(2, nrec) = density(1)
+        rec(3, nrec) = gasTemp(1)
+        rec(4, nrec) = av(1)
+        rec(5:, nrec) = abund(1:nspec, 1)
+    END SUBROUTINE record_row
+
+END MODULE ucref_driver

@@ -116,6 +116,7 @@ CONTAINS
         CALL cloudInitializePhysics(flag)   ! hotcore's initializePhysics does the same for three-phase networks
         IF (flag .lt. 0) RETURN
         CALL initializeChemistry(.false.)
+        dstep=1   ! coreInitializePhysics leaves its loop counter at points+1 (wrap.f90:643 resets it too)
     END SUBROUTINE setup_point
 
     ! ---------------------------------------------------------------- C entry points

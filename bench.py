@@ -1,0 +1,195 @@
+"""Benchmark: grid-point solves/s to 1 Myr with p50/p99 solve latency (one MI355X).
+
+Workload (one "step"): the 4096-point static-cloud grid on the default UCLCHEM network
+(seeded: log n_H ~ U[2,7], T ~ U[10,100] K, log zeta ~ U[0,2], log G0 ~ U[0,3],
+Av ~ U[1,20]), every point integrated 0 -> 1e6 yr with UCLCHEM's interval semantics,
+all points in one persistent-kernel launch, inputs resident in HBM.  FP64 throughout
+(plain batched form); the FP32-factorisation form is reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--points 4096] [--lu 64]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12     # MI355X dense FP64 (vector and matrix), FLOP/s
+PEAK_HBM = 8.0e12       # bytes/s
+
+
+def algorithmic_flops(stats: np.ndarray, neq: int, rhs_flop: float) -> float:
+    """FLOPs of the dense Newton linear algebra and the RHS implied by the solver counters.
+
+    per Newton-matrix setup: (2/3) n^3 (LU); per Newton iteration: 2 n^2 (two triangular
+    solves); per RHS evaluation: the generated multiply/add count of the network."""
+    setups = stats[:, 2].sum()
+    iters = stats[:, 3].sum()
+    rhs = stats[:, 1].sum()
+    return setups * (2.0 / 3.0) * neq ** 3 + iters * 2.0 * neq ** 2 + rhs * rhs_flop
+
+
+def rhs_flop_count(tables) -> float:
+    a = tables.arrays
+    mults = int((a["nfac"]).sum())                 # one multiply per factor of every term
+    adds = len(a["loss_idx"]) + len(a["prod_idx"])  # one add per term occurrence
+    return float(mults + adds)
+
+
+def cpu_baseline(params: np.ndarray, cores: int, cap_s: float, n_sample: int) -> dict:
+    """The reference Fortran path (oracle/_ref) on the box's host cores, one process per
+    core, each solving one grid point with a wall-time cap."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libucref.so")
+    if not os.path.exists(ref_so):
+        return None
+    sample = params[:n_sample]
+    procs = []
+    t0 = time.time()
+    code = ("import sys,time,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
+            "p=np.array([float(x) for x in sys.argv[1].split(',')]); t=time.time(); "
+            "rc,rows,st=RefLib().model(p,maxrec=2000); print('%%d %%.6f'%%(rc,time.time()-t))") % os.path.join(ROOT, "oracle")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    running, times, capped = [], [], 0
+    queue = list(range(len(sample)))
+    while queue or running:
+        while queue and len(running) < cores:
+            i = queue.pop(0)
+            arg = ",".join(repr(float(x)) for x in sample[i])
+            pr = subprocess.Popen([sys.executable, "-c", code, arg], stdout=subprocess.PIPE,
+                                  stderr=subprocess.DEVNULL, text=True, env=env)
+            running.append((pr, time.time()))
+        still = []
+        for pr, ts in running:
+            if pr.poll() is not None:
+                out = pr.stdout.read().split()
+                times.append(float(out[1]) if len(out) == 2 else time.time() - ts)
+            elif time.time() - ts > cap_s:
+                pr.kill()
+                pr.wait()
+                times.append(cap_s)
+                capped += 1
+            else:
+                still.append((pr, ts))
+        running = still
+        time.sleep(0.05)
+    wall = time.time() - t0
+    per_point = float(np.mean(times))
+    return {"value": cores / per_point, "unit": "grid-point solves/s", "cores": cores, "kind": "reference",
+            "sample": (f"first {len(sample)} points of the same 4096-point grid through the reference Fortran "
+                       f"(oracle/_ref, amdflang -O2), one process per core; {capped} point(s) hit the "
+                       f"{cap_s:.0f} s cap and count as {cap_s:.0f} s (a lower bound); mean {per_point:.2f} s/point; "
+                       f"sample wall {wall:.1f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=4096)
+    ap.add_argument("--lu", type=int, default=64)
+    ap.add_argument("--also-fp32", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-cap", type=float, default=25.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; grid points are independent and would shard "
+                          "by contiguous point ranges with no collective (see DESIGN.md)", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import torch
+    from uclchem_amd import gpu, workloads
+    from uclchem_amd import makerates_hip as mh
+
+    tables = mh.load_network(mh.network_path("default"))
+    net = gpu.Network(tables)
+    params = workloads.static_cloud_grid(a.points)
+    dev = torch.device("cuda:0")
+    n, neq = len(params), tables.neq
+    d_params = torch.tensor(params, dtype=torch.float64, device=dev)
+    d_y = torch.zeros((n, neq), dtype=torch.float64, device=dev)
+    d_status = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_stats = torch.zeros((n, 8), dtype=torch.int32, device=dev)
+    d_t0 = torch.zeros(n, dtype=torch.float64, device=dev)
+    d_t1 = torch.zeros(n, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def run(lu_bits, npts):
+        net.solve_device(d_params[:npts], d_y[:npts], d_status[:npts], d_stats[:npts], t_start=d_t0[:npts],
+                         t_done=d_t1[:npts], lu_bits=lu_bits, stream=stream.cuda_stream)
+
+    results = {}
+    for lu_bits in ([a.lu] + ([32] if (a.also_fp32 and a.lu == 64) else [])):
+        for _ in range(a.warmup):
+            run(lu_bits, min(n, 256))
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_wall = time.time()
+        ev0.record(stream)
+        for _ in range(a.steps):
+            run(lu_bits, n)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.time() - t_wall
+        ms = ev0.elapsed_time(ev1) / a.steps
+        st = d_stats.cpu().numpy()
+        status = d_status.cpu().numpy()
+        t0 = d_t0.cpu().numpy()
+        t1 = d_t1.cpu().numpy()
+        lat = (t1 - t0.min()) * 1e-8          # launch to retirement, device wall clock at 100 MHz
+        solve = (t1 - t0) * 1e-8              # one point's own solve time on its workgroup
+        results[lu_bits] = dict(ms=ms, wall=wall, stats=st, status=status, lat=lat, solve=solve,
+                                y=d_y.cpu().numpy().copy())
+    main_r = results[a.lu]
+    value = n / (main_r["ms"] / 1e3)
+    flops = algorithmic_flops(main_r["stats"], neq, rhs_flop_count(tables))
+    achieved = flops / (main_r["ms"] / 1e3)
+    line = {
+        "metric": "grid-point solves/s to 1 Myr (p50/p99 solve latency, 1% abundance agreement)",
+        "value": round(value, 3), "unit": "grid-point solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(main_r["ms"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded 4096-point static-cloud grid)",
+        "config": {"workload": f"static cloud grid, default network ({tables.nspec} species, {tables.nreac} "
+                               f"reactions), {n} points to 1e6 yr, FP64 (plain batched)",
+                   "points": n, "network": "default", "lu": f"fp{a.lu}"},
+        "latency_s": {"p50": float(np.percentile(main_r["lat"], 50)), "p99": float(np.percentile(main_r["lat"], 99)),
+                      "solve_p50": float(np.percentile(main_r["solve"], 50)),
+                      "solve_p99": float(np.percentile(main_r["solve"], 99))},
+        "status_nonzero": int(np.count_nonzero(main_r["status"])),
+        "solver_counters": {"steps": int(main_r["stats"][:, 0].sum()), "rhs": int(main_r["stats"][:, 1].sum()),
+                            "newton_setups": int(main_r["stats"][:, 2].sum()),
+                            "newton_iters": int(main_r["stats"][:, 3].sum())},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "note": "solve_points_kernel; FP64 FLOPs of the dense LU (2/3 n^3 per setup), the two "
+                             "triangular solves (2 n^2 per Newton iteration) and the RHS, per launch"},
+    }
+    if 32 in results and a.lu == 64:
+        r32 = results[32]
+        line["fp32_lu"] = {"value": round(n / (r32["ms"] / 1e3), 3), "ms_per_step": round(r32["ms"], 3),
+                           "latency_p50": float(np.percentile(r32["lat"], 50)),
+                           "latency_p99": float(np.percentile(r32["lat"], 99)),
+                           "status_nonzero": int(np.count_nonzero(r32["status"])),
+                           "max_rel_vs_fp64_above_1e-14": float(np.max(np.where(
+                               main_r["y"][:, :tables.nspec] > 1e-14,
+                               np.abs(r32["y"][:, :tables.nspec] - main_r["y"][:, :tables.nspec]) /
+                               np.maximum(main_r["y"][:, :tables.nspec], 1e-300), 0.0)))}
+    if not a.no_cpu:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        line["cpu_baseline"] = cpu_baseline(params, cores, a.cpu_cap, a.cpu_sample)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

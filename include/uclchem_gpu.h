@@ -1,0 +1,138 @@
+/*
+ * uclchem_gpu.h -- C ABI of the MI355X batched chemistry solver (libuclchem_gpu.so).
+ *
+ * Drop-in boundary for UCLCHEM's per-point chemistry solve.  Each entry point replaces
+ * one reference interface for a whole batch of independent grid points:
+ *
+ *   ucg_solve_batch      <- the solveAbundances time loop with points as independent grid
+ *                           points: updateChemistry + DVODE_F90 per output interval
+ *                           (src/fortran_src/wrap.f90:549-697, chemistry.f90:145-292,
+ *                            dvode.f90:2084 DVODE_F90(F,NEQ,Y,T,TOUT,ITASK=1,ISTATE=1,OPTS))
+ *   ucg_rates            <- calculateReactionRates (src/fortran_src/rates.f90:21) with the
+ *                           ice bookkeeping that precedes it (chemistry.f90:167-200)
+ *   ucg_rhs              <- chemistry::F -> odes::GETYDOT (chemistry.f90:294, odes.f90:6)
+ *   ucg_jacobian         <- the Newton-matrix Jacobian that DVODE builds by finite
+ *                           differences in DVJAC (dvode.f90:8326-8360); analytic here
+ *   ucg_newton_solve     <- DVJAC's M = I - gamma J factorisation + DGESL solve
+ *                           (dvode.f90:8372-8387, 11982-12206)
+ *
+ * Conventions: plain C types only; arrays are row-major in C (point-major: point p's
+ * vector starts at p*len).  Host pointers unless the function name ends in _device.
+ * Species/reaction indices are 0-based.  The parameter vector of one point has
+ * UCG_NPARAM doubles in the order of the UCG_P_* constants.  Every function returns
+ * 0 on success or a negative UCG_E_* code; ucg_last_error() describes the failure.
+ * The library is re-entrant per network handle and keeps no per-point global state.
+ */
+#ifndef UCLCHEM_GPU_H
+#define UCLCHEM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UCG_NPARAM 24
+/* point parameters (defaultparameters.f90 names) */
+#define UCG_P_DENS 0          /* initialDens [cm^-3] */
+#define UCG_P_TEMP 1          /* initialTemp [K] */
+#define UCG_P_ZETA 2          /* zeta [1.3e-17 s^-1] */
+#define UCG_P_RADFIELD 3      /* radfield [Habing] */
+#define UCG_P_BASEAV 4        /* baseAv [mag] */
+#define UCG_P_ROUT 5          /* rout [pc] */
+#define UCG_P_RIN 6           /* rin [pc] */
+#define UCG_P_FINALTIME 7     /* finalTime [yr] */
+#define UCG_P_FREEFALL 8      /* freefall (0/1) */
+#define UCG_P_FINALDENS 9     /* finalDens [cm^-3] */
+#define UCG_P_RELTOL 10       /* reltol */
+#define UCG_P_ABSFAC 11       /* abstol_factor */
+#define UCG_P_ABSMIN 12       /* abstol_min */
+#define UCG_P_MODEL 13        /* 0 cloud, 1 hot core */
+#define UCG_P_TEMPINDX 14     /* hot-core mass index 1..6 */
+#define UCG_P_MAXTEMP 15      /* hot-core maxTemp [K] */
+#define UCG_P_FREEZEFAC 16    /* freezeFactor */
+#define UCG_P_ENDATFINALDENS 17
+#define UCG_P_FREEFALLFAC 18  /* freefallFactor */
+#define UCG_P_MXSTEP 19       /* MXSTEP per integrator call */
+
+/* status codes (UCLCHEM's own, src/fortran_src/constants.f90:22-27) */
+#define UCG_OK 0
+#define UCG_E_PARAM -1
+#define UCG_E_UNRECOVERABLE -4
+#define UCG_E_TOO_MANY_FAILS -5
+#define UCG_E_TIMEPOINTS -6
+/* library errors */
+#define UCG_E_NODEVICE -100
+#define UCG_E_HIP -101
+#define UCG_E_ARG -102
+#define UCG_E_HANDLE -103
+
+/* Network tables as emitted by the MakeRates backend (uclchem_amd/makerates_hip.py).
+ * All pointers are host pointers, copied to the device by ucg_network_load. */
+typedef struct ucg_network_desc {
+    int nspec, neq, nreac, nice, nsurfl, nbulkl, nfp, nswap;
+    int nloss, nprod, nnz, ncontrib, naff, nsw;
+    const double *alpha, *beta, *gama, *tmin, *tmax, *mass, *rstat;     /* rstat [nreac*5] */
+    const int *rtype, *re, *pr, *ice_kind, *n_d, *ops, *fac, *nfac;       /* re [nreac*3], pr [nreac*4], ops [nreac*6], fac [nreac*10] */
+    const signed char *extrapolate;
+    const int *type_ranges;                                               /* [23*2] */
+    const int *loss_ptr, *loss_idx, *prod_ptr, *prod_idx, *swap_list;
+    const int *surface_list, *bulk_list, *bulk_partner_of_surface, *surface_partner_of_bulk; /* partner arrays [neq] */
+    const int *freeze_partners, *is_surface, *is_bulk;                    /* is_* [neq] */
+    const int *jnz_row, *jnz_col, *jnz_ptr, *jc_k, *jc_m, *jc_c;
+    const int *aff_ptr, *aff_sp, *aff_cf, *sw_ptr, *sw_k, *sw_c;
+    int special_species[25];   /* nh nh2 nhe nc ncx nn no nf np nmg nmgx nna nli npah nsx nsix nclx nd n18o n15n n13c nelec nco nsurface nbulk */
+    int special_reactions[9];  /* H2Form_CT H2Form_ER H2Form_ERDes HFreeze EFreeze H2Freeze H2_hv CO_hv C_hv */
+} ucg_network_desc;
+
+/* Select the GPU and upload the constant photo-rate tables.  Called implicitly. */
+int ucg_init(int device);
+const char *ucg_last_error(void);
+
+int ucg_network_load(const ucg_network_desc *desc, int *handle);
+int ucg_network_free(int handle);
+
+/* Full single-point models for a batch of independent grid points.
+ *   params  [n][UCG_NPARAM]; y_init [n][neq] or NULL for UCLCHEM's elemental start
+ *   y_out   [n][neq] final state (abundances + density)
+ *   rec     [n][max_rec][4+nspec] output rows (time yr, density, gasTemp, Av, abundances)
+ *           like the reference's physicsarray/chemicalabunarray, or NULL with max_rec=0
+ *   n_rec   [n] rows written; status [n] UCLCHEM success flag per point
+ *   stats   [n][8] steps, f-evals, Newton-matrix setups, Newton iterations,
+ *           convergence failures, error-test failures, intervals, integrator retries
+ *   lu_bits 64: FP64 Newton-matrix factorisation; 32: FP32 factorisation with the FP64
+ *           residual (every Newton iterate is re-evaluated in FP64).
+ * Returns when all points are done. */
+int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out,
+                    int max_rec, double *rec, int *n_rec, int *status, int *stats, int lu_bits);
+
+/* Same on device-resident buffers (all pointers are device pointers, stream is a
+ * hipStream_t or NULL).  t_start/t_done [n] (nullable) receive the device wall clock
+ * (100 MHz ticks) when each point was taken and retired.  n_slots = resident
+ * workgroups (0: two per compute unit).  Asynchronous with respect to the host. */
+int ucg_solve_batch_device(int handle, int n, const double *params, const double *y_init, double *y_out,
+                           int max_rec, double *rec, int *n_rec, int *status, int *stats,
+                           double *t_start, double *t_done, int lu_bits, int n_slots, void *stream);
+
+/* ucg_solve_batch with per-point device timers [n][8] in seconds: rates, RHS, Jacobian
+ * assembly, LU, triangular solves, total (no output rows).  n_slots as above. */
+int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,
+                             double *timers, int lu_bits, int n_slots);
+
+/* Fine-grained kernels (host arrays).  phys [n][8] = density, gasTemp, dustTemp, Av,
+ * cloudSize [cm], phi (the CR-desorption yield carried from the previous call),
+ * abstol_factor, unused. */
+int ucg_rates(int handle, int n, const double *params, const double *phys, const double *y,
+              double *rate_out, double *y_out, double *scal_out);
+int ucg_rhs(int handle, int n, const double *params, const double *phys, const double *rate,
+            const double *y, double *ydot, double *rate_out);
+int ucg_jacobian(int handle, int n, const double *params, const double *phys, const double *rate,
+                 const double *y, double *jac);
+int ucg_newton_solve(int handle, int n, const double *params, const double *phys, const double *y,
+                     double gamma, const double *b, double *x, int lu_bits);
+
+/* Network sizes of a loaded handle. */
+int ucg_network_info(int handle, int *nspec, int *neq, int *nreac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+"""GPU parity of the fine-grained kernels against the oracle (C restatement, CPU).
+
+rates: <= 1e-12 relative per reaction; RHS: <= 1e-12 of the species' term magnitude;
+Jacobian: directional derivatives J.v against central differences of the oracle RHS
+(v scaled by y, so every column counts) to 1e-6 of sum_j |J_ij v_j|; Newton solve:
+backward error of (I - gamma J) x = b.
+"""
+import numpy as np
+import pytest
+
+from helpers import phys_of, random_state, term_magnitude
+
+pytestmark = pytest.mark.gpu
+
+
+def _states(tables, oracle, seed, n=8):
+    from uclchem_amd.params import make_params
+    rng = np.random.default_rng(seed)
+    P, PH, Y, pts = [], [], [], []
+    for trial in range(n):
+        p = make_params(initialdens=10 ** rng.uniform(2, 8), initialtemp=rng.uniform(10, 300),
+                        zeta=10 ** rng.uniform(0, 2), radfield=10 ** rng.uniform(0, 3), baseav=rng.uniform(0, 10),
+                        finaltime=1e6)
+        pt, y = random_state(tables, oracle, rng, p, bare_grains=(trial % 3 == 0))
+        P.append(p)
+        PH.append(phys_of(pt))
+        Y.append(y)
+        pts.append(pt)
+    return np.array(P), np.array(PH), np.array(Y), pts
+
+
+def test_rates_match_oracle(gpu_net, tables, oracle):
+    P, PH, Y, pts = _states(tables, oracle, 1, 12)
+    rg, yg, sg = gpu_net.rates(P, PH, Y)
+    for q in range(len(P)):
+        ro, yo, so = oracle.rates(pts[q], Y[q])
+        d = np.abs(rg[q] - ro) / np.maximum(np.abs(ro), 1e-300)
+        d[(rg[q] == 0) & (ro == 0)] = 0
+        assert d.max() <= 1e-12, (q, int(np.argmax(d)))
+        assert np.allclose(yg[q], yo, rtol=1e-14, atol=0)
+        assert abs(sg[q, 6] - so[6]) <= 1e-12 * abs(so[6])      # phi handed to the next call
+
+
+def test_rhs_matches_oracle(gpu_net, tables, oracle):
+    P, PH, Y, pts = _states(tables, oracle, 2, 8)
+    R, Yr, YD, MAG = [], [], [], []
+    for q in range(len(P)):
+        r, y, _ = oracle.rates(pts[q], Y[q])
+        yd, r2 = oracle.rhs(pts[q], r, y)
+        R.append(r)
+        Yr.append(y)
+        YD.append(yd)
+        MAG.append(term_magnitude(tables, r2, y))
+    ydg, rtg = gpu_net.rhs(P, PH, np.array(R), np.array(Yr))
+    err = np.abs(ydg - np.array(YD))
+    assert np.all(err <= 1e-12 * np.array(MAG))
+
+
+def test_jacobian_directional_derivatives(gpu_net, tables, oracle):
+    P, PH, Y, pts = _states(tables, oracle, 3, 6)
+    R, Yr = [], []
+    for q in range(len(P)):
+        r, y, _ = oracle.rates(pts[q], Y[q])
+        R.append(r)
+        Yr.append(y)
+    J = gpu_net.jacobian(P, PH, np.array(R), np.array(Yr))
+    rng = np.random.default_rng(0)
+    for q in range(len(P)):
+        y = Yr[q]
+        for rep in range(3):
+            v = y * rng.choice([-1.0, 1.0], tables.neq)
+            best = None
+            for eps in (1e-3, 1e-4, 1e-5):
+                fp, _ = oracle.rhs(pts[q], R[q], y + eps * v)
+                fm, _ = oracle.rhs(pts[q], R[q], y - eps * v)
+                e = np.abs(J[q] @ v - (fp - fm) / (2 * eps))
+                best = e if best is None else np.minimum(best, e)
+            scale = np.abs(J[q]) @ np.abs(v) + 1e-300
+            assert (best / scale).max() <= 1e-6, (q, rep)
+
+
+@pytest.mark.parametrize("lu_bits,tol", [(64, 1e-12), (32, 1e-5)])
+def test_newton_solve_backward_error(gpu_net, tables, lu_bits, tol):
+    """(I - gamma J) x = b on states along a real trajectory (config 1 rows)."""
+    from uclchem_amd import workloads
+    out = gpu_net.solve(workloads.static_cloud_point(), max_rec=60)
+    rows = out["rec"][0, :out["n_rec"][0]]
+    from uclchem_amd.params import make_params
+    p = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e6)
+    cs = (float(np.float32(0.05))) * 3.086e18
+    rng = np.random.default_rng(4)
+    for k in (10, 25, 40):
+        y = np.append(rows[k, 4:], rows[k, 1])
+        phys = np.array([rows[k, 1], rows[k, 2], rows[k, 2], rows[k, 3], cs, 1e5, 1e-14, 0.0])
+        for gamma in (1e4, 1e9):
+            b = rng.standard_normal(tables.neq)
+            x = gpu_net.newton_solve(p[None], phys[None], y[None], gamma, b[None], lu_bits=lu_bits)[0]
+            rate, _, _ = gpu_net.rates(p[None], phys[None], y[None])
+            J = gpu_net.jacobian(p[None], phys[None], rate, y[None])[0]
+            M = np.eye(tables.neq) - gamma * J
+            r = M @ x - b
+            berr = np.abs(r).max() / (np.abs(M).sum(axis=1).max() * np.abs(x).max() + np.abs(b).max())
+            assert berr <= tol, (k, gamma, berr)

@@ -1,0 +1,204 @@
+"""CPU tests pinning the oracle (no GPU).
+
+* the compiled reference (oracle/_ref) reproduces the reference's own example output;
+* the C restatement (oracle/liboracle.so) is bit-identical to the compiled reference for
+  rates and the RHS over seeded states, including consecutive calls (the phi leak);
+* the photo-rate helpers equal the reference's own procedures called directly;
+* the RHS conserves elements (the reference's tests/test_ode_conservation.py invariant);
+* the output schedule reproduces the example-output time column.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from helpers import element_totals, random_state, term_magnitude
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def test_golden_fixture_present():
+    g = np.load(os.path.join(GOLD, "static_full_t1e6.npz"))
+    assert g["abundances"].shape == (335,)
+    assert g["species"][0].strip() == "H"
+
+
+def test_reference_reproduces_example_output(reflib, tables):
+    """oracle/_ref runs config 1 and matches examples/example-output/static-full.dat:48."""
+    from oracle import make_params
+    g = np.load(os.path.join(GOLD, "static_full_t1e6.npz"))
+    rc, rows, stats = reflib.model(make_params(dens=1e4, temp=10.0, finaltime=1e6))
+    assert rc == 0
+    assert np.allclose(rows[:, 0], g["times"], rtol=1e-12)
+    final = rows[-1, 4:]
+    m = g["abundances"] > 1e-14
+    rel = np.abs(final[m] - g["abundances"][m]) / g["abundances"][m]
+    assert rel.max() < 1e-5          # the file prints 6 significant digits
+    assert np.allclose(g["physics"][1:5], [rows[-1, 1], rows[-1, 2], rows[-1, 2], rows[-1, 3]], rtol=1e-4)
+
+
+def test_tables_match_compiled_network(reflib, tables):
+    """MakeRates-backend tables equal the values compiled into the reference network module."""
+    n, s = tables.nreac, tables.nspec
+    for sym, arr in [("_QMnetworkEalpha", tables.alpha), ("_QMnetworkEbeta", tables.beta),
+                     ("_QMnetworkEgama", tables.gama), ("_QMnetworkECmintemps", tables.tmin),
+                     ("_QMnetworkECmaxtemps", tables.tmax), ("_QMnetworkECreducedmasses", tables.reduced_mass)]:
+        assert np.array_equal(reflib.network_array(sym, n), arr), sym
+    assert np.array_equal(reflib.network_array("_QMnetworkECmass", s), tables.mass)
+    ni = len(tables.ice_list)
+    assert np.array_equal(reflib.network_array("_QMnetworkEbindingenergy", ni), tables.binding_energy)
+    for k, sym in enumerate(["re1", "re2", "re3"]):
+        ref = reflib.network_array("_QMnetworkEC" + sym, n, ctypes.c_int64)
+        ours = np.where(tables.re[:, k] < 0, 9999, tables.re[:, k] + 1)
+        assert np.array_equal(ref, ours), sym
+    ref = reflib.network_array("_QMnetworkECgasicelist", ni, ctypes.c_int64)
+    assert np.array_equal(ref, tables.gas_ice_list + 1)
+    fp = reflib.network_array("_QMnetworkECfreezepartners", len(tables.freeze_partners), ctypes.c_int64)
+    assert np.array_equal(fp, tables.freeze_partners + 1)
+
+
+def test_generator_term_programs_match_odes_f90(tables):
+    """Every LOSS/PROD expression of the shipped odes.f90 is regenerated token for token."""
+    import re as _re
+    path = os.path.join("/root/reference/src/fortran_src/odes.f90")
+    if not os.path.exists(path):
+        pytest.skip("reference sources not mounted")
+    from uclchem_amd import makerates_hip as mh
+    src = _re.sub(r"&\n\s*&", "", open(path).read())
+
+    def ode_bit(k):
+        s = f"RATE({k + 1})" + "*D" * tables.n_d[k]
+        for op in tables.ops[k]:
+            if op == mh.OP_NONE:
+                continue
+            s += {mh.OP_BLR: "*bulkLayersReciprocal", mh.OP_SWAP: "*totalSwap/safeMantle",
+                  mh.OP_DIVSM: "/safeMantle"}.get(int(op), f"*Y({op + 1})")
+        return s
+    cur, checked = {}, 0
+    for ln in src.splitlines():
+        ln = ln.strip()
+        m = _re.match(r"(LOSS|PROD) = (.*)", ln)
+        if m:
+            cur[m.group(1)] = m.group(2)
+            continue
+        m = _re.match(r"YDOT\((\d+)\) = (.*)", ln)
+        if m:
+            i = int(m.group(1)) - 1
+            if i < tables.nspec - 2:
+                for kind, ptr, ix in (("LOSS", tables.loss_ptr, tables.loss_idx),
+                                      ("PROD", tables.prod_ptr, tables.prod_idx)):
+                    ours = "+".join(ode_bit(k) for k in ix[ptr[i]:ptr[i + 1]])
+                    theirs = cur.get(kind, "") if kind in m.group(2) else ""
+                    assert ours == theirs, (tables.species_names[i], kind)
+                    checked += 1
+            cur = {}
+    assert checked == 2 * (tables.nspec - 2)
+    ts = _re.search(r"totalSwap=(.*)", src).group(1).strip()
+    assert ts == "+".join(ode_bit(k) for k in tables.swap_list)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_rates_rhs_bit_exact_vs_reference(reflib, oracle, tables, seed):
+    from oracle import make_params
+    rng = np.random.default_rng(seed)
+    for trial in range(12):
+        T = rng.uniform(10, 300)
+        p = make_params(dens=10 ** rng.uniform(2, 8), temp=T, zeta=10 ** rng.uniform(0, 2),
+                        radfield=10 ** rng.uniform(0, 3), baseav=rng.uniform(0, 10), finaltime=1e6)
+        rc, y0r = reflib.init(p)
+        pt, y0 = oracle.init_point(p)
+        assert np.array_equal(y0r, y0)
+        _, y = random_state(tables, oracle, rng, p, bare_grains=(trial % 3 == 0))
+        for call in range(3):          # consecutive calls: phi carries over (rates.f90:118,322-326)
+            rr, yr, sr = reflib.rates(y)
+            ro, yo, so = oracle.rates(pt, y)
+            assert np.array_equal(rr, ro)
+            assert np.array_equal(yr, yo)
+            assert np.array_equal(sr, so)
+            assert np.array_equal(reflib.rhs(yr), oracle.rhs(pt, ro, yr)[0])
+            y = np.abs(y * (1 + 0.1 * rng.standard_normal(tables.neq)))
+            y[-1] = y0[-1]
+
+
+def test_photo_functions_vs_reference_procedures(reflib, oracle):
+    """The reference's own H2/CO photodissociation and shielding procedures, called directly."""
+    L = reflib.lib
+    d = ctypes.c_double
+    for f in ("_QMphotoreactionsPh2photodissrate", "_QMphotoreactionsPcophotodissrate",
+              "_QMphotoreactionsPcoselfshielding", "_QMphotoreactionsPscatter"):
+        getattr(L, f).restype = ctypes.c_double
+    O = oracle.lib
+    O.ucl_h2_photodiss.argtypes = [ctypes.c_double] * 4
+    O.ucl_co_photodiss.argtypes = [ctypes.c_double] * 4
+    O.ucl_co_self_shielding.argtypes = [ctypes.c_double] * 2
+    O.ucl_scatter.argtypes = [ctypes.c_double] * 2
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        nh2, nco = 10 ** rng.uniform(10, 24), 10 ** rng.uniform(8, 20)
+        g0, av = 10 ** rng.uniform(0, 3), rng.uniform(0, 20)
+        ref_h2 = L._QMphotoreactionsPh2photodissrate(ctypes.byref(d(nh2)), ctypes.byref(d(g0)), ctypes.byref(d(av)),
+                                                      ctypes.byref(d(1.0)))
+        assert O.ucl_h2_photodiss(nh2, g0, av, 1.0) == ref_h2
+        ref_co = L._QMphotoreactionsPcophotodissrate(ctypes.byref(d(nh2)), ctypes.byref(d(nco)), ctypes.byref(d(g0)),
+                                                      ctypes.byref(d(av)))
+        # COPhotoDissRate reads the module radfield; set through a point set-up
+        from oracle import make_params
+        reflib.init(make_params(radfield=g0))
+        ref_co = L._QMphotoreactionsPcophotodissrate(ctypes.byref(d(nh2)), ctypes.byref(d(nco)), ctypes.byref(d(g0)),
+                                                      ctypes.byref(d(av)))
+        assert O.ucl_co_photodiss(nh2, nco, g0, av) == pytest.approx(ref_co, rel=1e-14)
+        ref_ss = L._QMphotoreactionsPcoselfshielding(ctypes.byref(d(nh2)), ctypes.byref(d(nco)))
+        assert O.ucl_co_self_shielding(nh2, nco) == pytest.approx(ref_ss, rel=1e-14)
+
+
+def test_rhs_conserves_elements(oracle, tables):
+    """Sum_i n_elem(i) * ydot_i vanishes (tests/test_ode_conservation.py:6-10 of the reference)."""
+    from oracle import make_params
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        p = make_params(dens=10 ** rng.uniform(2, 7), temp=rng.uniform(10, 200), finaltime=1e6)
+        pt, y = random_state(tables, oracle, rng, p, bare_grains=(trial == 0))
+        rate, y, _ = oracle.rates(pt, y)
+        ydot, rate2 = oracle.rhs(pt, rate, y)
+        mag = term_magnitude(tables, rate2, y)
+        from uclchem_amd.makerates_hip import ELEMENTS
+        ec = tables.elem_counts[:tables.nspec].T.astype(float)
+        ec = np.delete(ec, ELEMENTS.index("E-"), axis=0)     # electrons obey charge, not element, balance
+        scale = ec @ mag[:tables.nspec]
+        sums = ec @ ydot[:tables.nspec]
+        assert np.all(np.abs(sums) <= 1e-13 * scale + 1e-300)
+        # charge is not an invariant: ions freeze out as neutral ices (C+ -> #C) and
+        # enforceChargeConservation is off by default (defaultparameters.f90:44)
+
+
+def test_schedule_matches_example_output(oracle):
+    g = np.load(os.path.join(GOLD, "static_full_t1e6.npz"))
+    t, times = 0.0, [0.0]
+    while t < 1e6:
+        t = oracle.next_target(0, t) / 3.16e7
+        times.append(t)
+    assert np.allclose(times, g["times"], rtol=1e-12)
+
+
+def test_oracle_planted_mistake_is_caught(reflib, oracle, tables):
+    """A dropped term in the oracle's RHS is detected by the bit-exact comparison."""
+    from oracle import make_params
+    p = make_params(dens=1e4, temp=10.0, finaltime=1e6)
+    reflib.init(p)
+    pt, y = oracle.init_point(p)
+    rate, y2, _ = oracle.rates(pt, y)
+    rr, _, _ = reflib.rates(y)
+    bad = rate.copy()
+    bad[tables.type_ranges[-1][0] + 5] *= 0.0     # drop one two-body reaction
+    ydot_bad, _ = oracle.rhs(pt, bad, y2)
+    assert not np.array_equal(ydot_bad, reflib.rhs(y2))
+
+
+def test_element_totals_helper(tables):
+    y = np.zeros(tables.neq)
+    y[tables.species_names.index("H2O")] = 1.0
+    tot = element_totals(tables, y)
+    from uclchem_amd.makerates_hip import ELEMENTS
+    assert tot[ELEMENTS.index("H")] == 2 and tot[ELEMENTS.index("O")] == 1

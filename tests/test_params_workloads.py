@@ -1,0 +1,37 @@
+"""CPU checks of the host-side parameter plumbing and seeded workloads."""
+import numpy as np
+
+from uclchem_amd import params, workloads
+
+
+def test_defaults_and_float32_literals():
+    p = params.make_params()
+    assert p[params.INDEX["rout"]] == float(np.float32(0.05))     # default-real literal
+    assert p[params.INDEX["reltol"]] == 1e-8
+    q = params.make_params({"initialDens": 1e4, "freefall": True})
+    assert q[params.INDEX["initialdens"]] == 1e4 and q[params.INDEX["freefall"]] == 1.0
+
+
+def test_unknown_key_rejected():
+    import pytest
+    with pytest.raises(KeyError):
+        params.make_params(notaparam=1)
+
+
+def test_grid_av_mapping():
+    g = workloads.static_cloud_grid(16)
+    nh = g[:, params.INDEX["initialdens"]]
+    av = g[:, params.INDEX["baseav"]] + params.DEFAULTS["rout"] * params.PC * nh / 1.6e21
+    assert np.all((av >= 1.0 - 1e-9) & (av <= 20.0 + 1e-9))
+    assert np.array_equal(g, workloads.static_cloud_grid(16))    # seeded
+
+
+def test_oracle_and_product_param_layouts_agree():
+    import oracle as orc
+    assert orc.P_NAMES[:len(params.NAMES)] == [
+        "dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall", "finaldens",
+        "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp", "freezefactor",
+        "endatfinaldensity", "freefallfactor", "mxstep"]
+    a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0)
+    b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0)
+    assert np.array_equal(a, b)

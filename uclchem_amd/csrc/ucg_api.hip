@@ -1,0 +1,534 @@
+// C ABI of libuclchem_gpu.so (declarations and reference citations: include/uclchem_gpu.h).
+// Single translation unit: the kernels are compiled together with their launchers.
+#include "ucg_solver.hip"
+#include "../../include/uclchem_gpu.h"
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::mutex g_mu;
+std::string g_err;
+bool g_inited = false;
+int g_device = -1;
+int g_num_cu = 0;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(UCG_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct NetHandle {
+    bool used = false;
+    ucg::DevNet dev{};
+    std::vector<void *> allocs;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    int *counter = nullptr;
+};
+std::vector<NetHandle> g_nets;
+
+// natural cubic spline second derivatives on the host (photoreactions.f90:340-418)
+void spline_host(const double *x, const double *y, int n, double *y2)
+{
+    double u[64];
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                    (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    y2[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+int init_locked(int device)
+{
+    if (g_inited && (device < 0 || device == g_device)) return 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(UCG_E_NODEVICE, "no HIP device visible");
+    if (device < 0) device = 0;
+    if (device >= ndev) return fail(UCG_E_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    g_num_cu = prop.multiProcessorCount;
+    // photo-rate tables (photoreactions.f90:14-39)
+    ucg::PhotoTables t;
+    const double lam[30] = {910.0, 950.0, 1000.0, 1050.0, 1110.0, 1180.0, 1250.0, 1390.0, 1490.0, 1600.0,
+                            1700.0, 1800.0, 1900.0, 2000.0, 2100.0, 2190.0, 2300.0, 2400.0, 2500.0, 2740.0,
+                            3440.0, 4000.0, 4400.0, 5500.0, 7000.0, 9000.0, 12500.0, 22000.0, 34000.0, 1.0e9};
+    const double xl[30] = {5.76, 5.18, 4.65, 4.16, 3.73, 3.40, 3.11, 2.74, 2.63, 2.62, 2.54, 2.50, 2.58, 2.78, 3.01,
+                           3.12, 2.86, 2.58, 2.35, 2.00, 1.58, 1.42, 1.32, 1.00, 0.75, 0.48, 0.28, 0.12, 0.05, 0.00};
+    const double sco[48] = {
+        0.000e+00, -1.408e-02, -1.099e-01, -4.400e-01, -1.154e+00, -1.888e+00, -2.760e+00, -4.001e+00,
+        -8.539e-02, -1.015e-01, -2.104e-01, -5.608e-01, -1.272e+00, -1.973e+00, -2.818e+00, -4.055e+00,
+        -1.451e-01, -1.612e-01, -2.708e-01, -6.273e-01, -1.355e+00, -2.057e+00, -2.902e+00, -4.122e+00,
+        -4.559e-01, -4.666e-01, -5.432e-01, -8.665e-01, -1.602e+00, -2.303e+00, -3.146e+00, -4.421e+00,
+        -1.303e+00, -1.312e+00, -1.367e+00, -1.676e+00, -2.305e+00, -3.034e+00, -3.758e+00, -5.077e+00,
+        -3.883e+00, -3.888e+00, -3.936e+00, -4.197e+00, -4.739e+00, -5.165e+00, -5.441e+00, -6.446e+00};
+    for (int i = 0; i < 30; i++) {
+        t.lam[i] = lam[i];
+        t.xl[i] = xl[i];
+    }
+    spline_host(t.lam, t.xl, 30, t.xl_d2);
+    for (int i = 0; i < 8; i++) t.nco[i] = 12.0 + i;
+    for (int i = 0; i < 6; i++) t.nh2[i] = 18.0 + i;
+    // the reference hands the 8x6 table to splie2 as 7 rows: its first 42 stored values
+    for (int j = 0; j < 7; j++)
+        for (int k = 0; k < 6; k++) t.ya[j][k] = sco[j + 7 * k];
+    for (int j = 0; j < 7; j++) spline_host(t.nh2, t.ya[j], 6, t.y2a[j]);
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(ucg::c_photo), &t, sizeof(t)));
+    g_device = device;
+    g_inited = true;
+    return 0;
+}
+
+template <typename T>
+int upload(NetHandle &h, const T *src, size_t n, const T *&dst)
+{
+    if (n == 0) {
+        dst = nullptr;
+        return 0;
+    }
+    if (!src) return fail(UCG_E_ARG, "null network table");
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, n * sizeof(T)));
+    h.allocs.push_back(p);
+    HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    dst = (const T *)p;
+    return 0;
+}
+
+NetHandle *get(int handle)
+{
+    if (handle < 0 || handle >= (int)g_nets.size() || !g_nets[handle].used) return nullptr;
+    return &g_nets[handle];
+}
+
+size_t lds_bytes(const ucg::DevNet &N)
+{
+    const int npad = (N.neq + 7) & ~7;
+    const int flux = N.nreac > 2 * npad ? N.nreac : 2 * npad;
+    return sizeof(double) * ((size_t)N.nreac + flux + 4 * npad + 16 + ucg::S_NSCAL + ucg::P_NPARAM);
+}
+
+int ensure_ws(NetHandle &h, size_t bytes)
+{
+    if (h.ws_bytes >= bytes) return 0;
+    if (h.ws) HIPCHK(hipFree(h.ws));
+    h.ws = nullptr;
+    h.ws_bytes = 0;
+    HIPCHK(hipMalloc(&h.ws, bytes));
+    h.ws_bytes = bytes;
+    return 0;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+template <typename T>
+int to_dev(DevBuf &b, const T *src, size_t n)
+{
+    if (!src || n == 0) return 0;
+    HIPCHK(hipMalloc(&b.p, n * sizeof(T)));
+    HIPCHK(hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+template <typename T>
+int alloc_dev(DevBuf &b, size_t n)
+{
+    if (n == 0) return 0;
+    HIPCHK(hipMalloc(&b.p, n * sizeof(T)));
+    return 0;
+}
+
+int check_params(int n, const double *params)
+{
+    for (int p = 0; p < n; p++) {
+        const double *q = params + (size_t)p * UCG_NPARAM;
+        if (!(q[UCG_P_DENS] > 0) || !(q[UCG_P_TEMP] > 0) || !(q[UCG_P_RELTOL] > 0) || !(q[UCG_P_ABSMIN] >= 0) ||
+            !(q[UCG_P_MXSTEP] >= 1) || !(q[UCG_P_ROUT] > q[UCG_P_RIN]))
+            return fail(UCG_E_PARAM, "invalid parameter vector for point " + std::to_string(p));
+        int model = (int)q[UCG_P_MODEL];
+        if (model != 0 && model != 1) return fail(UCG_E_PARAM, "model must be 0 (cloud) or 1 (hot core)");
+        if (model == 1 && ((int)q[UCG_P_TEMPINDX] < 1 || (int)q[UCG_P_TEMPINDX] > 6))
+            return fail(UCG_E_PARAM, "tempindx must be 1..6");
+    }
+    return 0;
+}
+
+int launch_solve(NetHandle &h, int n, const double *params, const double *y_init, double *y_out, int max_rec,
+                 double *rec, int *n_rec, int *status, int *stats, double *t_start, double *t_done, int lu_bits,
+                 int n_slots, hipStream_t stream, double *timers = nullptr)
+{
+    const ucg::DevNet &N = h.dev;
+    if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
+    if (n <= 0) return 0;
+    if (n_slots <= 0) n_slots = 2 * g_num_cu;
+    if (n_slots > n) n_slots = n;
+    const size_t elem = lu_bits == 64 ? sizeof(double) : sizeof(float);
+    size_t stride = (size_t)N.ld * N.neq * elem;
+    stride = (stride + 255) & ~(size_t)255;
+    int rc = ensure_ws(h, stride * n_slots);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(h.counter, 0, sizeof(int), stream));
+    ucg::SolveArgs A;
+    A.n_points = n;
+    A.params = params;
+    A.y_init = y_init;
+    A.y_out = y_out;
+    A.max_rec = rec ? max_rec : 0;
+    A.rec = rec;
+    A.n_rec = n_rec;
+    A.status = status;
+    A.stats = stats;
+    A.t_start = t_start;
+    A.t_done = t_done;
+    A.timers = timers;
+    A.ws = h.ws;
+    A.ws_stride = stride;
+    A.counter = h.counter;
+    const size_t lds = lds_bytes(N);
+    if (lu_bits == 64) {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::solve_points_kernel<double>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+    } else {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::solve_points_kernel<float>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int piece_common(NetHandle &h, int n, const double *params, const double *phys, ucg::PieceArgs &A,
+                 DevBuf &dp, DevBuf &dph)
+{
+    const ucg::DevNet &N = h.dev;
+    if (n <= 0) return fail(UCG_E_ARG, "n must be positive");
+    int rc = check_params(n, params);
+    if (rc) return rc;
+    if ((rc = to_dev(dp, params, (size_t)n * UCG_NPARAM))) return rc;
+    if ((rc = to_dev(dph, phys, (size_t)n * 8))) return rc;
+    memset(&A, 0, sizeof(A));
+    A.n = n;
+    A.params = (const double *)dp.p;
+    A.phys = (const double *)dph.p;
+    size_t stride = ((size_t)N.ld * N.neq * sizeof(double) + 255) & ~(size_t)255;
+    if ((rc = ensure_ws(h, stride * n))) return rc;
+    A.ws = h.ws;
+    A.ws_stride = stride;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ucg_last_error(void) { return g_err.c_str(); }
+
+int ucg_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return init_locked(device);
+}
+
+int ucg_network_load(const ucg_network_desc *d, int *handle)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = init_locked(-1);
+    if (rc) return rc;
+    if (!d || !handle) return fail(UCG_E_ARG, "null argument");
+    if (d->neq != d->nspec + 1 || d->neq > ucg::NT * ucg::EPT)
+        return fail(UCG_E_ARG, "network too large: NEQ must be <= " + std::to_string(ucg::NT * ucg::EPT));
+    NetHandle h;
+    h.used = true;
+    ucg::DevNet &N = h.dev;
+    N.nspec = d->nspec; N.neq = d->neq; N.nreac = d->nreac; N.nice = d->nice; N.nsurfl = d->nsurfl;
+    N.nbulkl = d->nbulkl; N.nfp = d->nfp; N.nswap = d->nswap; N.nnz = d->nnz;
+    N.ld = (d->neq + 15) & ~15;
+    const size_t R = d->nreac, Q = d->neq;
+#define UP(field, count) if ((rc = upload(h, d->field, (count), N.field))) return rc
+    UP(alpha, R); UP(beta, R); UP(gama, R); UP(tmin, R); UP(tmax, R); UP(mass, (size_t)d->nspec);
+    UP(rstat, 5 * R); UP(rtype, R); UP(re, 3 * R); UP(pr, 4 * R); UP(ice_kind, R); UP(n_d, R);
+    UP(ops, ucg::MAXOPS * R); UP(fac, ucg::MAXFAC * R); UP(nfac, R); UP(extrapolate, R);
+    UP(type_ranges, (size_t)2 * ucg::NTYPES);
+    UP(loss_ptr, (size_t)d->nspec + 1); UP(loss_idx, (size_t)d->nloss);
+    UP(prod_ptr, (size_t)d->nspec + 1); UP(prod_idx, (size_t)d->nprod); UP(swap_list, (size_t)d->nswap);
+    UP(surface_list, (size_t)d->nsurfl); UP(bulk_list, (size_t)d->nbulkl);
+    UP(bulk_partner_of_surface, Q); UP(surface_partner_of_bulk, Q);
+    UP(freeze_partners, (size_t)d->nfp); UP(is_surface, Q); UP(is_bulk, Q);
+    UP(jnz_row, (size_t)d->nnz); UP(jnz_col, (size_t)d->nnz); UP(jnz_ptr, (size_t)d->nnz + 1);
+    UP(jc_k, (size_t)d->ncontrib); UP(jc_m, (size_t)d->ncontrib); UP(jc_c, (size_t)d->ncontrib);
+    UP(aff_ptr, R + 1); UP(aff_sp, (size_t)d->naff); UP(aff_cf, (size_t)d->naff);
+    UP(sw_ptr, Q + 1); UP(sw_k, (size_t)d->nsw); UP(sw_c, (size_t)d->nsw);
+#undef UP
+    const int *ss = d->special_species;
+    N.nh = ss[0]; N.nh2 = ss[1]; N.nhe = ss[2]; N.nc = ss[3]; N.ncx = ss[4]; N.nn = ss[5]; N.no = ss[6];
+    N.nf = ss[7]; N.np = ss[8]; N.nmg = ss[9]; N.nmgx = ss[10]; N.nna = ss[11]; N.nli = ss[12];
+    N.npah = ss[13]; N.nsx = ss[14]; N.nsix = ss[15]; N.nclx = ss[16]; N.nd = ss[17]; N.n18o = ss[18];
+    N.n15n = ss[19]; N.n13c = ss[20]; N.nelec = ss[21]; N.nco = ss[22]; N.nsurface = ss[23]; N.nbulk = ss[24];
+    for (int i = 0; i < 25; i++)
+        if (ss[i] < 0 || ss[i] >= d->neq) return fail(UCG_E_ARG, "special species index out of range");
+    const int *sr = d->special_reactions;
+    N.r_h2form_ct = sr[0]; N.r_h2form_er = sr[1]; N.r_h2form_erdes = sr[2]; N.r_hfreeze = sr[3];
+    N.r_efreeze = sr[4]; N.r_h2freeze = sr[5]; N.r_h2_hv = sr[6]; N.r_co_hv = sr[7]; N.r_c_hv = sr[8];
+    for (int i = 0; i < 9; i++)
+        if (sr[i] < 0 || sr[i] >= d->nreac) return fail(UCG_E_ARG, "special reaction index out of range");
+    // the device rate evaluator pairs ranges element-wise (rates.f90:243-279, 104-224)
+    {
+        const int *tr = d->type_ranges;
+        auto len = [&](int t) { return tr[2 * t] < 0 ? 0 : tr[2 * t + 1] - tr[2 * t] + 1; };
+        if (len(ucg::T_LH) != len(ucg::T_LHDES)) return fail(UCG_E_ARG, "LH and LHDES ranges differ in length");
+        if (len(ucg::T_ER) != len(ucg::T_ERDES)) return fail(UCG_E_ARG, "ER and ERDES ranges differ in length");
+        for (int t : {ucg::T_DESOH2, ucg::T_DESCR, ucg::T_DEUVCR, ucg::T_THERM})
+            if (len(t) > 1 && len(t) < d->nfp) return fail(UCG_E_ARG, "desorption range shorter than freeze partners");
+    }
+    HIPCHK(hipMalloc(&h.counter, sizeof(int)));
+    h.allocs.push_back(h.counter);
+    size_t lds = lds_bytes(N);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, g_device));
+    if (lds > (size_t)prop.sharedMemPerBlock)
+        return fail(UCG_E_ARG, "network needs " + std::to_string(lds) + " B of LDS per workgroup, device allows " +
+                                   std::to_string(prop.sharedMemPerBlock));
+    g_nets.push_back(h);
+    *handle = (int)g_nets.size() - 1;
+    return 0;
+}
+
+int ucg_network_free(int handle)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->ws) (void)hipFree(h->ws);
+    *h = NetHandle();
+    return 0;
+}
+
+int ucg_network_info(int handle, int *nspec, int *neq, int *nreac)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (nspec) *nspec = h->dev.nspec;
+    if (neq) *neq = h->dev.neq;
+    if (nreac) *nreac = h->dev.nreac;
+    return 0;
+}
+
+int ucg_solve_batch_device(int handle, int n, const double *params, const double *y_init, double *y_out,
+                           int max_rec, double *rec, int *n_rec, int *status, int *stats, double *t_start,
+                           double *t_done, int lu_bits, int n_slots, void *stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !y_out || !status || !stats) return fail(UCG_E_ARG, "null buffer");
+    return launch_solve(*h, n, params, y_init, y_out, max_rec, rec, n_rec, status, stats, t_start, t_done,
+                        lu_bits, n_slots, (hipStream_t)stream);
+}
+
+int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,
+                             double *timers, int lu_bits, int n_slots)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    int rc = check_params(n, params);
+    if (rc) return rc;
+    const int neq = h->dev.neq;
+    DevBuf dp, dy, dst, dstat, dtm;
+    if ((rc = to_dev(dp, params, (size_t)n * UCG_NPARAM))) return rc;
+    if ((rc = alloc_dev<double>(dy, (size_t)n * neq))) return rc;
+    if ((rc = alloc_dev<int>(dst, (size_t)n))) return rc;
+    if ((rc = alloc_dev<int>(dstat, (size_t)n * 8))) return rc;
+    if ((rc = alloc_dev<double>(dtm, (size_t)n * 8))) return rc;
+    rc = launch_solve(*h, n, (const double *)dp.p, nullptr, (double *)dy.p, 0, nullptr, nullptr, (int *)dst.p,
+                      (int *)dstat.p, nullptr, nullptr, lu_bits, n_slots, nullptr, (double *)dtm.p);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(timers, dtm.p, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out, int max_rec,
+                    double *rec, int *n_rec, int *status, int *stats, int lu_bits)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (n <= 0 || !params || !y_out || !status || !stats) return fail(UCG_E_ARG, "bad arguments");
+    int rc = check_params(n, params);
+    if (rc) return rc;
+    const int neq = h->dev.neq, ns = h->dev.nspec;
+    if (!rec) max_rec = 0;
+    DevBuf dp, dy0, dy, drec, dnrec, dst, dstat;
+    if ((rc = to_dev(dp, params, (size_t)n * UCG_NPARAM))) return rc;
+    if (y_init && (rc = to_dev(dy0, y_init, (size_t)n * neq))) return rc;
+    if ((rc = alloc_dev<double>(dy, (size_t)n * neq))) return rc;
+    if (max_rec > 0 && (rc = alloc_dev<double>(drec, (size_t)n * max_rec * (4 + ns)))) return rc;
+    if ((rc = alloc_dev<int>(dnrec, (size_t)n))) return rc;
+    if ((rc = alloc_dev<int>(dst, (size_t)n))) return rc;
+    if ((rc = alloc_dev<int>(dstat, (size_t)n * 8))) return rc;
+    rc = launch_solve(*h, n, (const double *)dp.p, (const double *)dy0.p, (double *)dy.p, max_rec,
+                      (double *)drec.p, (int *)dnrec.p, (int *)dst.p, (int *)dstat.p, nullptr, nullptr, lu_bits, 0,
+                      nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
+    if (max_rec > 0) HIPCHK(hipMemcpy(rec, drec.p, (size_t)n * max_rec * (4 + ns) * sizeof(double), hipMemcpyDeviceToHost));
+    if (n_rec) HIPCHK(hipMemcpy(n_rec, dnrec.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_rates(int handle, int n, const double *params, const double *phys, const double *y, double *rate_out,
+              double *y_out, double *scal_out)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !phys || !y || !rate_out || !y_out || !scal_out) return fail(UCG_E_ARG, "null buffer");
+    const ucg::DevNet &N = h->dev;
+    ucg::PieceArgs A;
+    DevBuf dp, dph, dy, dr, dyo, ds;
+    int rc = piece_common(*h, n, params, phys, A, dp, dph);
+    if (rc) return rc;
+    if ((rc = to_dev(dy, y, (size_t)n * N.neq))) return rc;
+    if ((rc = alloc_dev<double>(dr, (size_t)n * N.nreac))) return rc;
+    if ((rc = alloc_dev<double>(dyo, (size_t)n * N.neq))) return rc;
+    if ((rc = alloc_dev<double>(ds, (size_t)n * 8))) return rc;
+    A.y = (const double *)dy.p;
+    A.rate_out = (double *)dr.p;
+    A.y_out = (double *)dyo.p;
+    A.scal_out = (double *)ds.p;
+    size_t lds = lds_bytes(N);
+    HIPCHK(hipFuncSetAttribute((const void *)ucg::rates_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(ucg::rates_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(rate_out, dr.p, (size_t)n * N.nreac * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y_out, dyo.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(scal_out, ds.p, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_rhs(int handle, int n, const double *params, const double *phys, const double *rate, const double *y,
+            double *ydot, double *rate_out)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !phys || !rate || !y || !ydot) return fail(UCG_E_ARG, "null buffer");
+    const ucg::DevNet &N = h->dev;
+    ucg::PieceArgs A;
+    DevBuf dp, dph, dy, drin, dyo, dro;
+    int rc = piece_common(*h, n, params, phys, A, dp, dph);
+    if (rc) return rc;
+    if ((rc = to_dev(dy, y, (size_t)n * N.neq))) return rc;
+    if ((rc = to_dev(drin, rate, (size_t)n * N.nreac))) return rc;
+    if ((rc = alloc_dev<double>(dyo, (size_t)n * N.neq))) return rc;
+    if (rate_out && (rc = alloc_dev<double>(dro, (size_t)n * N.nreac))) return rc;
+    A.y = (const double *)dy.p;
+    A.rate_in = (const double *)drin.p;
+    A.y_out = (double *)dyo.p;
+    A.rate_out = (double *)dro.p;
+    size_t lds = lds_bytes(N);
+    HIPCHK(hipFuncSetAttribute((const void *)ucg::rhs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(ucg::rhs_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(ydot, dyo.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    if (rate_out) HIPCHK(hipMemcpy(rate_out, dro.p, (size_t)n * N.nreac * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_jacobian(int handle, int n, const double *params, const double *phys, const double *rate, const double *y,
+                 double *jac)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !phys || !rate || !y || !jac) return fail(UCG_E_ARG, "null buffer");
+    const ucg::DevNet &N = h->dev;
+    ucg::PieceArgs A;
+    DevBuf dp, dph, dy, drin, dj;
+    int rc = piece_common(*h, n, params, phys, A, dp, dph);
+    if (rc) return rc;
+    if ((rc = to_dev(dy, y, (size_t)n * N.neq))) return rc;
+    if ((rc = to_dev(drin, rate, (size_t)n * N.nreac))) return rc;
+    if ((rc = alloc_dev<double>(dj, (size_t)n * N.neq * N.neq))) return rc;
+    A.y = (const double *)dy.p;
+    A.rate_in = (const double *)drin.p;
+    A.jac_out = (double *)dj.p;
+    size_t lds = lds_bytes(N);
+    HIPCHK(hipFuncSetAttribute((const void *)ucg::jac_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(ucg::jac_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(jac, dj.p, (size_t)n * N.neq * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_newton_solve(int handle, int n, const double *params, const double *phys, const double *y, double gamma,
+                     const double *b, double *x, int lu_bits)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !phys || !y || !b || !x) return fail(UCG_E_ARG, "null buffer");
+    if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
+    const ucg::DevNet &N = h->dev;
+    ucg::PieceArgs A;
+    DevBuf dp, dph, dy, db, dx;
+    int rc = piece_common(*h, n, params, phys, A, dp, dph);
+    if (rc) return rc;
+    if ((rc = to_dev(dy, y, (size_t)n * N.neq))) return rc;
+    if ((rc = to_dev(db, b, (size_t)n * N.neq))) return rc;
+    if ((rc = alloc_dev<double>(dx, (size_t)n * N.neq))) return rc;
+    A.y = (const double *)dy.p;
+    A.scal_out = (double *)db.p;   // right-hand sides
+    A.y_out = (double *)dx.p;
+    A.gamma = gamma;
+    size_t lds = lds_bytes(N);
+    if (lu_bits == 64) {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::newton_solve_kernel<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::newton_solve_kernel<double>, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    } else {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::newton_solve_kernel<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::newton_solve_kernel<float>, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(x, dx.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+}  // extern "C"

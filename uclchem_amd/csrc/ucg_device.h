@@ -1,0 +1,241 @@
+// Device-side data structures and per-point physics for the UCLCHEM chemistry solve.
+//
+// One network descriptor (DevNet) holds device pointers to the tables emitted by the
+// MakeRates backend (uclchem_amd/makerates_hip.py).  The physics restated here follows
+// src/fortran_src/{rates,surfacereactions,photoreactions,physics-core,cloud,hotcore}.f90;
+// default-kind (single precision) literals of the reference are written as float
+// literals widened to double so the arithmetic matches the compiled reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ucg {
+
+constexpr int NT = 256;          // threads per workgroup (one workgroup solves one grid point)
+constexpr int EPT = 2;           // state elements owned per thread => NEQ <= 512
+constexpr int MAXOPS = 6;
+constexpr int MAXFAC = 10;
+constexpr int NTYPES = 23;
+
+// reaction type codes (reaction.py:5-28 order, TWOBODY last)
+enum { T_PHOTON, T_CRP, T_CRPHOT, T_FREEZE, T_DESORB, T_THERM, T_DESOH2, T_DESCR, T_DEUVCR, T_H2FORM,
+       T_ER, T_ERDES, T_LH, T_LHDES, T_BULKSWAP, T_SURFSWAP, T_IONOPOL1, T_IONOPOL2, T_CRS, T_EXSOLID,
+       T_EXRELAX, T_GAR, T_TWOBODY };
+// term-program / Jacobian factor codes (>= 0: species index)
+enum { OP_NONE = -1, OP_BLR = -2, OP_SWAP = -3, OP_DIVSM = -4, FAC_D = -5 };
+
+// point-parameter vector layout (shared with include/uclchem_gpu.h)
+enum { P_DENS, P_TEMP, P_ZETA, P_RADFIELD, P_BASEAV, P_ROUT, P_RIN, P_FINALTIME, P_FREEFALL,
+       P_FINALDENS, P_RELTOL, P_ABSFAC, P_ABSMIN, P_MODEL, P_TEMPINDX, P_MAXTEMP, P_FREEZEFAC,
+       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_NPARAM = 24 };
+
+// physical constants (constants.f90, surfacereactions.f90:33-51)
+constexpr double K_BOLTZ = 1.38065040e-16;
+constexpr double AMU = 1.66053892e-24;
+constexpr double PI_F = (double)3.141592654f;
+constexpr double PC = 3.086e18;
+constexpr double SECONDS_PER_YEAR = 3.16e7;
+constexpr double GRAIN_RADIUS = 1.0e-5;
+constexpr double MAX_GRAIN_TEMP = 150.0;
+constexpr double MIN_SURFACE_ABUND = 1.0e-20;
+constexpr double GRAIN_CROSSSECTION_PER_H = 0.5 * (7.908e-22 + 8.473e-22);
+constexpr double GRAIN_SURFACEAREA_PER_H = 4.0 * GRAIN_CROSSSECTION_PER_H;
+constexpr double GAS_DUST_DENSITY_RATIO =
+    (4.0 * PI_F * (GRAIN_RADIUS * GRAIN_RADIUS * GRAIN_RADIUS) * 3.0 * 100.0) / (3.0 * AMU);
+constexpr double NUM_SITES_PER_GRAIN = GRAIN_RADIUS * GRAIN_RADIUS * 1.5e15 * 4.0 * PI_F;
+constexpr double SURFACE_COVERAGE0 = 0.5 * GAS_DUST_DENSITY_RATIO / NUM_SITES_PER_GRAIN;  // bulkGainFromMantleBuildUp
+constexpr double MIN_ABUND = 1.0e-30;
+
+struct DevNet {
+    int32_t nspec, neq, nreac, nice, nsurfl, nbulkl, nfp, nswap, nnz, ld;
+    const double *alpha, *beta, *gama, *tmin, *tmax, *mass, *rstat;  // rstat [nreac*5]
+    const int32_t *rtype, *re, *pr, *ice_kind, *n_d, *ops, *fac, *nfac;
+    const int8_t *extrapolate;
+    const int32_t *type_ranges;                                     // [NTYPES*2]
+    const int32_t *loss_ptr, *loss_idx, *prod_ptr, *prod_idx, *swap_list;
+    const int32_t *surface_list, *bulk_list, *bulk_partner_of_surface, *surface_partner_of_bulk;
+    const int32_t *freeze_partners, *is_surface, *is_bulk;          // is_*: [neq]
+    const int32_t *jnz_row, *jnz_col, *jnz_ptr, *jc_k, *jc_m, *jc_c;
+    const int32_t *aff_ptr, *aff_sp, *aff_cf, *sw_ptr, *sw_k, *sw_c;
+    // special species (0-based; missing -> nspec, the density slot)
+    int32_t nh, nh2, nhe, nc, ncx, nn, no, nf, np, nmg, nmgx, nna, nli, npah, nsx, nsix, nclx, nd,
+        n18o, n15n, n13c, nelec, nco, nsurface, nbulk;
+    int32_t r_h2form_ct, r_h2form_er, r_h2form_erdes, r_hfreeze, r_efreeze, r_h2freeze, r_h2_hv, r_co_hv,
+        r_c_hv;
+};
+
+// static spline tables of the photo-rate functions (photoreactions.f90:14-39),
+// second derivatives precomputed once on the host
+struct PhotoTables {
+    double lam[30], xl[30], xl_d2[30];
+    double nco[8], nh2[6];
+    double ya[7][6], y2a[7][6];   // the 7x6 view of SCO_GRID the reference's splie2 call sees
+};
+
+extern __constant__ PhotoTables c_photo;
+
+// ---------------------------------------------------------------- photo-rate helpers
+
+__device__ __forceinline__ double splint_dev(const double *xa, const double *ya, const double *y2a, int n,
+                                             double x)
+{
+    int jlo = 0, jhi = n + 1;
+    while (jhi - jlo != 1) {
+        int jm = (jhi + jlo) >> 1;
+        if (x > xa[jm - 1]) jlo = jm; else jhi = jm;
+    }
+    if (jlo == 0) { jlo = 1; jhi = 2; }
+    if (jlo == n) { jlo = n - 1; jhi = n; }
+    double h = xa[jhi - 1] - xa[jlo - 1];
+    double a = (xa[jhi - 1] - x) / h;
+    double b = (x - xa[jlo - 1]) / h;
+    return a * ya[jlo - 1] + b * ya[jhi - 1] +
+           ((a * a * a - a) * y2a[jlo - 1] + (b * b * b - b) * y2a[jhi - 1]) * (h * h) / 6.0;
+}
+
+__device__ __forceinline__ void spline_dev(const double *x, const double *y, int n, double *y2)
+{   // natural cubic spline (photoreactions.f90:340-418)
+    double u[8];
+    y2[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; i++) {
+        double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) /
+                    (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    y2[n - 1] = 0.0;
+    for (int k = n - 2; k >= 0; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+__device__ inline double scatter_dev(double x1, double av)
+{   // photoreactions.f90:138-196
+    const double c[6] = {1.0, 2.006, -1.438, 7.364e-01, -5.076e-01, -5.920e-02};
+    const double k1[6] = {7.514e-01, 8.490e-01, 1.013, 1.282, 2.005, 5.832};
+    double lv = x1;
+    if (x1 < c_photo.lam[0]) lv = c_photo.lam[0];
+    if (x1 > c_photo.lam[29]) lv = c_photo.lam[29];
+    double xl = splint_dev(c_photo.lam, c_photo.xl, c_photo.xl_d2, 30, lv);
+    if (xl < 0.0) xl = 0.0;
+    double tl = (av / 1.086) * xl;
+    double s = 0.0;
+    if (tl < 1.0) {
+        double expo = k1[0] * tl;
+        if (expo < 100.0) s = c[0] * exp(-expo);
+    } else {
+        for (int i = 1; i < 6; i++) {
+            double expo = k1[i] * tl;
+            if (expo < 100.0) s = s + c[i] * exp(-expo);
+        }
+    }
+    return s;
+}
+
+__device__ inline double h2_self_shielding_dev(double nh2)
+{   // photoreactions.f90:95-133 with dopplerWidth = turbVel/(xl*1e-8) = 1e5, radWidth = 8e7
+    const double doppler = 1.0 / (1000.0 * 1.0e-8), radwidth = 8.0e7;
+    double taud = 0.5 * nh2 * (double)1.5e-2f * 1.0e-2 / doppler;
+    double sj;
+    if (taud == 0.0) sj = 1.0;
+    else if (taud < 2.0) sj = exp(-0.6666667 * taud);
+    else if (taud < 10.0) sj = 0.638 * pow(taud, -1.25);
+    else if (taud < 100.0) sj = 0.505 * pow(taud, -1.15);
+    else sj = 0.344 * pow(taud, -1.0667);
+    double r = radwidth / (1.7724539 * doppler);
+    double t = 3.02 * pow(r * 1.0e3, -0.064);
+    double u = sqrt(taud * r) / t;
+    double sr = r / (t * sqrt(0.78539816 + u * u));
+    return sj + sr;
+}
+
+// H2 photodissociation given scatter(1000 A, Av) (photoreactions.f90:46-55)
+__device__ inline double h2_photodiss_dev(double nh2col, double radfield, double scat1000)
+{
+    return 5.18e-11 * (radfield / (double)1.7f) * scat1000 * h2_self_shielding_dev(nh2col);
+}
+
+__device__ inline double co_photodiss_dev(double nh2col, double ncocol, double radfield, double av)
+{   // photoreactions.f90:57-72, 238-287
+    double lognco = log10(ncocol + 1.0), lognh2 = log10(nh2col + 1.0);
+    if (lognco < c_photo.nco[0]) lognco = c_photo.nco[0];
+    if (lognh2 < c_photo.nh2[0]) lognh2 = c_photo.nh2[0];
+    if (lognco > c_photo.nco[6]) lognco = c_photo.nco[6];
+    if (lognh2 > c_photo.nh2[5]) lognh2 = c_photo.nh2[5];
+    double yy[7], y2[7];
+    for (int j = 0; j < 7; j++) yy[j] = splint_dev(c_photo.nh2, c_photo.ya[j], c_photo.y2a[j], 6, lognh2);
+    spline_dev(c_photo.nco, yy, 7, y2);
+    double ssf = pow(10.0, splint_dev(c_photo.nco, yy, y2, 7, lognco));
+    double lu = log10(fabs(ncocol) + 1.0), lw = log10(fabs(nh2col) + 1.0);
+    double lb = (5675.0 - 200.6 * lw) - (571.6 - 24.09 * lw) * lu + (18.22 - 0.7664 * lw) * (lu * lu);
+    if (lb > 1076.1) lb = 1076.1;
+    if (lb < 913.6) lb = 913.6;
+    return 2.0e-10 * (radfield / (double)1.7f) * ssf * scatter_dev(lb, av);
+}
+
+__device__ inline double c_ionization_dev(double alpha, double gamma, double t, double ncol, double nh2col,
+                                          double av, double radfield)
+{   // photoreactions.f90:74-87
+    double tauc = gamma * av + 1.1e-17 * ncol + (0.9 * pow(t, 0.27) * pow(nh2col / 1.59e21, 0.45));
+    return alpha * (radfield / (double)1.7f) * exp(-tauc);
+}
+
+__device__ inline double h2_form_efficiency_dev(double gas_temp, double dust_temp)
+{   // surfacereactions.f90:63-107
+    double tv = 1.45e5 * sqrt(gas_temp / 1.0e2);
+    double g2 = gas_temp / 1.0e2;
+    double stick = 1.0 / (1.0 + 0.04 * sqrt(gas_temp + dust_temp) + 0.2 * g2 + 0.08 * (g2 * g2));
+    const double hflux = 1.0e-10;
+    double q = 1.0 + sqrt((3.0e4 - 110.0) / (450.0 - 110.0));
+    double f1 = 0.005 * hflux / (2 * 3.0e12 * exp(-320.0 / dust_temp));
+    double f2 = 1.0 * (q * q) / 4.0 * exp(-110.0 / dust_temp);
+    double eps = 1.0 / (1.0 + 1.3e13 / (2 * hflux) * exp(-1.5 * 3.0e4 / dust_temp) * (q * q));
+    double sil = 1.0 / (1.0 + f1 + f2) * eps;
+    q = 1.0 + sqrt((3.0e4 - 260.0) / (800.0 - 260.0));
+    f1 = 0.005 * hflux / (2 * 3.0e12 * exp(-520.0 / dust_temp));
+    f2 = 1.0 * (q * q) / 4.0 * exp(-260.0 / dust_temp);
+    eps = 1.0 / (1.0 + 1.3e13 / (2 * hflux) * exp(-1.5 * 3.0e4 / dust_temp) * (q * q));
+    double gra = 1.0 / (1.0 + f1 + f2) * eps;
+    return 0.5 * tv * (8.473e-22 * sil + 7.908e-22 * gra) * stick;
+}
+
+__device__ __forceinline__ double sticking_dev(double s0, double tcrit, double t)
+{
+    double r = t / tcrit;
+    return s0 * (1.0 + 2.5 * r) / pow(1.0 + r, 2.5);
+}
+
+__device__ inline double densdot_dev(const double *p, double d)
+{   // physics-core.f90:90-103
+    if (d < p[P_FINALDENS] && p[P_FREEFALL] != 0.0) {
+        double n0 = p[P_DENS];
+        return p[P_FREEFALLFAC] * pow(pow(d, 4.0) / n0, (double)0.33f) *
+               pow(8.4e-30 * n0 * (pow(d / n0, (double)0.33f) - 1.0), 0.5);
+    }
+    return 0.0;
+}
+
+__device__ inline double next_target_dev(int model, double t)
+{
+    if (model == 1) {   // hotcore.f90:56-72
+        if (t > 1.0e6) return (t + 1.0e5) * SECONDS_PER_YEAR;
+        if (t > 1.0e5) return (t + 1.0e4) * SECONDS_PER_YEAR;
+        if (t > 1.0e4) return (t + 1000.0) * SECONDS_PER_YEAR;
+        if (t > 1000) return (t + 100.0) * SECONDS_PER_YEAR;
+        if (t > 100) return (t + 10.0) * SECONDS_PER_YEAR;
+        if (t > 0.0) return (t * 10.0) * SECONDS_PER_YEAR;
+        return SECONDS_PER_YEAR * 1.0e-7;
+    }
+    // cloud.f90:39-51
+    if (t >= 1.0e6) return (t + 1.0e5) * SECONDS_PER_YEAR;
+    if (t > 10.0) {
+        double om = 1.0;
+        int e = (int)floor(log10(t));
+        for (int i = 0; i < e; i++) om *= 10.0;
+        return ((floor(t / om) + 1.0) * om) * SECONDS_PER_YEAR;
+    }
+    if (t > 0.0) return 10 * t * SECONDS_PER_YEAR;
+    return SECONDS_PER_YEAR * 1.0e-7;
+}
+
+}  // namespace ucg

@@ -1,0 +1,1273 @@
+// Batched stiff chemistry solve for MI355X (gfx950).
+//
+// One workgroup (256 threads, 4 wave64) integrates one grid point at a time and then
+// takes the next point from a device work counter, so uneven points (step counts vary
+// by >30x) never stall a whole batch: this is the persistent "retire and refill"
+// kernel.  Per point it reproduces UCLCHEM's per-interval semantics
+// (src/fortran_src/wrap.f90:598-690, chemistry.f90:145-292): the output schedule,
+// rates frozen per interval attempt, a cold-start BDF integration per interval with
+// DVODE's ITASK=1 contract, the retry policy, the 1e-30 clamp and the physics update.
+//
+// Memory layout per workgroup slot: the dense Newton matrix (column-major, leading
+// dimension ld) in HBM (FP64 or FP32, template LUT), everything else in LDS or
+// registers: rate and flux arrays [NREAC], the RHS input/output and solve vectors
+// [NEQ], and each thread's slice of the Nordsieck history (registers).
+#include "ucg_device.h"
+#include "bdf_core.h"
+
+namespace ucg {
+
+__constant__ PhotoTables c_photo;
+
+// LDS scalar slots
+enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI, S_DENS, S_TGAS, S_TDUST,
+       S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_NSCAL = 24 };
+
+struct Lds {
+    double *rate, *flux, *y, *f, *b, *g, *red, *sc, *prm;
+};
+
+__device__ __forceinline__ int tid() { return (int)threadIdx.x; }
+
+__device__ inline double block_sum(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((tid() & 63) == 0) red[tid() >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) r += red[w];
+    return r;
+}
+
+__device__ inline void block_sum2(double &a, double &b, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+    }
+    __syncthreads();
+    if ((tid() & 63) == 0) {
+        red[tid() >> 6] = a;
+        red[NT / 64 + (tid() >> 6)] = b;
+    }
+    __syncthreads();
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        ra += red[w];
+        rb += red[NT / 64 + w];
+    }
+    a = ra;
+    b = rb;
+}
+
+__device__ inline double block_max(double v, double *red)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+    __syncthreads();
+    if ((tid() & 63) == 0) red[tid() >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) r = fmax(r, red[w]);
+    return r;
+}
+
+__device__ __forceinline__ bool type_active(const DevNet &N, int t)
+{
+    int a = N.type_ranges[2 * t], b = N.type_ranges[2 * t + 1];
+    if (t == T_TWOBODY || t == T_BULKSWAP || t == T_SURFSWAP) return a >= 0;
+    return a >= 0 && a != b;   // rates.f90 range blocks are skipped when idx1 == idx2
+}
+
+__device__ inline double freeze_out(const DevNet &N, const double *prm, int j, double T, double Td)
+{   // rates.f90:354-367
+    double f = 1.0 + N.beta[j] * 16.71e-4 / (GRAIN_RADIUS * T);
+    if (prm[P_FREEZEFAC] == 0.0 || Td > MAX_GRAIN_TEMP) return 0.0;
+    const double thermal_vel = sqrt(8.0 * K_BOLTZ / (PI_F * AMU));
+    return f * prm[P_FREEZEFAC] * N.alpha[j] * thermal_vel * sqrt(T / N.mass[N.re[3 * j]]) *
+           GRAIN_CROSSSECTION_PER_H;
+}
+
+// Rate coefficients for the current interval attempt (rates.f90:21-343 preceded by the
+// ice bookkeeping of chemistry.f90:167-200).  Reads y (LDS), writes rate (LDS), updates
+// y[BULK], y[SURFACE], the column densities, the ice scalars and phi.
+__device__ void block_rates(const DevNet &N, Lds &L)
+{
+    const int neq = N.neq;
+    const double *prm = L.prm;
+    double *y = L.y, *rate = L.rate, *sc = L.sc;
+    const double T = sc[S_TGAS], Td = sc[S_TDUST], av = sc[S_AV], dens = sc[S_DENS];
+    const double zeta = prm[P_ZETA], radfield = prm[P_RADFIELD];
+    if (tid() == 0 && prm[P_FREEFALL] == 0.0) y[neq - 1] = dens;
+    double sb = 0.0, ss = 0.0;
+    for (int i = tid(); i < N.nbulkl; i += NT) sb += y[N.bulk_list[i]];
+    for (int i = tid(); i < N.nsurfl; i += NT) ss += y[N.surface_list[i]];
+    block_sum2(sb, ss, L.red);
+    if (tid() == 0) {
+        y[N.nbulk] = sb;
+        y[N.nsurface] = ss;
+        double sm = fmax(1e-30, ss), sbk = fmax(1e-30, sb);
+        sc[S_SM] = sm;
+        sc[S_SBK] = sbk;
+        sc[S_BLR] = fmin(1.0, NUM_SITES_PER_GRAIN / (GAS_DUST_DENSITY_RATIO * sbk));
+        double half = sc[S_CS] / 1.0;
+        sc[S_H2COL] = 0.5 * y[N.nh2] * dens * half;
+        sc[S_COCOL] = 0.5 * y[N.nco] * dens * half;
+        sc[S_CCOL] = 0.5 * y[N.nc] * dens * half;
+    }
+    if (tid() == 64) sc[S_SCAT] = scatter_dev(1000.0, av);
+    __syncthreads();
+    const double sm = sc[S_SM], blr = sc[S_BLR], phi = sc[S_PHI];
+    const double photo_bulk_pow = pow((double)(1.0f - 0.007f), 1.0 + 0.5 / blr);
+    const double ice_ratio = (double)0.3f;
+    const bool act_crp = type_active(N, T_CRP), act_ph = type_active(N, T_PHOTON),
+               act_crph = type_active(N, T_CRPHOT), act_fr = type_active(N, T_FREEZE),
+               act_h2d = type_active(N, T_DESOH2), act_cr = type_active(N, T_DESCR),
+               act_uv = type_active(N, T_DEUVCR), act_th = type_active(N, T_THERM),
+               act_lh = type_active(N, T_LH), act_er = type_active(N, T_ER);
+    const int lh0 = N.type_ranges[2 * T_LH], lhd0 = N.type_ranges[2 * T_LHDES];
+    const int er0 = N.type_ranges[2 * T_ER], erd0 = N.type_ranges[2 * T_ERDES];
+    const double h2eff = h2_form_efficiency_dev(T, Td);
+    // phase 1: base rates (every entry written)
+    for (int k = tid(); k < N.nreac; k += NT) {
+        const int t = N.rtype[k];
+        double r = 0.0;
+        switch (t) {
+        case T_CRP:
+            if (act_crp) r = N.alpha[k] * zeta;
+            break;
+        case T_PHOTON:
+        case T_CRPHOT:
+            if ((t == T_PHOTON && act_ph) || (t == T_CRPHOT && act_crph)) {
+                if (t == T_PHOTON) r = N.alpha[k] * exp(-N.gama[k] * av) * radfield / (double)1.7f;
+                else r = N.alpha[k] * N.gama[k] * 1.0 / (1.0 - 0.5) * zeta * pow(T / 300, N.beta[k]);
+                if (N.ice_kind[k] == 2) r = r * ice_ratio * photo_bulk_pow;
+                else if (N.ice_kind[k] == 1) r = r * ice_ratio;
+            }
+            break;
+        case T_FREEZE:
+            if (act_fr) r = freeze_out(N, prm, k, T, Td);
+            break;
+        case T_DESOH2:
+            if (act_h2d && sm > MIN_SURFACE_ABUND) r = N.gama[k] > 1.21e3 ? 0.0 : (double)0.01f * h2eff;
+            break;
+        case T_DESCR:
+            if (act_cr && sm > MIN_SURFACE_ABUND)
+                r = N.gama[k] > 1.21e3 ? 0.0 : 4.0 * PI_F * zeta * 1.64e-4 * (GRAIN_SURFACEAREA_PER_H) * phi;
+            break;
+        case T_DEUVCR:
+            if (act_uv && sm > MIN_SURFACE_ABUND && zeta > 0) {
+                double v = GRAIN_CROSSSECTION_PER_H * (double)0.03f * 4.875e3 * zeta;
+                v = v * (1 + (radfield / 1.0e-3) * (1.0 / zeta) * exp((double)-1.8f * av));
+                r = N.gama[k] > 1.0e4 ? 0.0 : v;
+            }
+            break;
+        case T_THERM:
+            if (act_th && N.rstat[5 * k + 1] != 0.0) r = N.rstat[5 * k] * exp(-N.gama[k] / Td);
+            break;
+        case T_LH:
+            if (act_lh) {
+                const int c = lhd0 + (k - lh0);
+                double rl = 0.0, rd = 0.0;
+                if (Td < MAX_GRAIN_TEMP && sm > MIN_SURFACE_ABUND) {
+                    const double *st = N.rstat + 5 * k;
+                    double v1 = st[0], v2 = st[1], e1 = st[2], e2 = st[3], tunnel = st[4];
+                    double diffuse = v1 * exp(-0.5 * e1 / Td);
+                    diffuse = diffuse + v2 * exp(-0.5 * e2 / Td);
+                    double desorb = v1 * exp(-e1 / Td);
+                    desorb = desorb + v2 * exp(-e2 / Td);
+                    double reac = N.gama[k] / Td;
+                    if (reac > tunnel) reac = tunnel;
+                    reac = fmax(v1, v2) * exp(-reac);
+                    reac = reac / (reac + desorb + diffuse);
+                    rl = N.alpha[k] * reac * diffuse * GAS_DUST_DENSITY_RATIO / NUM_SITES_PER_GRAIN;
+                    rd = N.rstat[5 * c + 1] != 0.0 ? 0.0 : N.rstat[5 * c] * rl;
+                    rl = rl - rd;
+                }
+                rate[c] = rd;
+                r = rl;
+            }
+            break;
+        case T_LHDES:
+            if (act_lh) continue;   // written together with its LH partner
+            break;
+        case T_ER:
+            if (act_er) {
+                const int c = erd0 + (k - er0);
+                double rl = freeze_out(N, prm, k, T, Td) * exp(-N.gama[k] / Td);
+                double rd = N.rstat[5 * c + 1] != 0.0 ? 0.0 : N.rstat[5 * c] * rl;
+                rate[c] = rd;
+                r = rl - rd;
+            }
+            break;
+        case T_ERDES:
+            if (act_er) continue;
+            break;
+        case T_BULKSWAP:
+            if (!(Td > MAX_GRAIN_TEMP || sm < MIN_SURFACE_ABUND) && N.rstat[5 * k + 2] != 0.0)
+                r = N.rstat[5 * k] * exp(-N.rstat[5 * k + 1] / Td);
+            break;
+        case T_SURFSWAP:
+            r = (Td > MAX_GRAIN_TEMP || sm < MIN_SURFACE_ABUND) ? 0.0 : 1.0;
+            break;
+        case T_TWOBODY:
+            r = N.alpha[k] * pow(T / 300.0, N.beta[k]) * exp(-N.gama[k] / T);
+            break;
+        default:
+            break;
+        }
+        rate[k] = r;
+    }
+    __syncthreads();
+    // freeze-out specials (rates.f90:78-83)
+    if (tid() == 0 && act_fr) {
+        rate[N.r_efreeze] = rate[N.r_hfreeze];
+        rate[N.r_h2freeze] = sticking_dev(0.87, 87.0, T) * rate[N.r_h2freeze];
+        rate[N.r_hfreeze] = sticking_dev(1.0, 52.0, T) * rate[N.r_hfreeze];
+    }
+    __syncthreads();
+    // phase 2: freeze-out switch-off where desorption dominates (rates.f90:104-107,125-128,148-151,221-224);
+    // the thermal list is compared with its first nfp entries (non-conforming WHERE)
+    for (int q = tid(); q < N.nfp; q += NT) {
+        const int f = N.freeze_partners[q];
+        double rf = rate[f];
+        double yf = y[N.re[3 * f]];
+        bool off = false;
+        if (act_h2d) off |= (rf * yf < MIN_SURFACE_ABUND * rate[N.type_ranges[2 * T_DESOH2] + q]);
+        if (act_cr) off |= (rf * yf * dens < MIN_SURFACE_ABUND * rate[N.type_ranges[2 * T_DESCR] + q]);
+        if (act_uv) off |= (rf * yf * dens < MIN_SURFACE_ABUND * rate[N.type_ranges[2 * T_DEUVCR] + q]);
+        if (act_th) off |= (rf * yf * dens < MIN_SURFACE_ABUND * rate[N.type_ranges[2 * T_THERM] + q]);
+        if (off) rate[f] = 0.0;
+    }
+    __syncthreads();
+    // phase 3: thermal zeroing on bare grains, H2 formation specials, temperature masks
+    for (int k = tid(); k < N.nreac; k += NT) {
+        double r = rate[k];
+        if (N.rtype[k] == T_THERM && act_th && sm < MIN_SURFACE_ABUND) r = 0.0;
+        if (k == N.r_h2form_ct) r = h2_form_efficiency_dev(Td, Td);
+        if (k == N.r_h2form_er || k == N.r_h2form_erdes) r = 0.0;
+        if (!N.extrapolate[k] && T < N.tmin[k]) r = 0.0;
+        if (!N.extrapolate[k] && T > N.tmax[k]) r = 0.0;
+        rate[k] = r;
+    }
+    __syncthreads();
+    // photo overrides and the phi update (rates.f90:316-342)
+    if (tid() == 0) {
+        rate[N.r_h2_hv] = h2_photodiss_dev(sc[S_H2COL], radfield, sc[S_SCAT]);
+        double ph = radfield * exp(-2.5 * av) * sqrt(T) / (y[neq - 1] * y[N.nelec]);
+        sc[S_PHI] = fmin(fmax(ph, 1e2), 1e6);
+    }
+    if (tid() == 64) rate[N.r_co_hv] = co_photodiss_dev(sc[S_H2COL], sc[S_COCOL], radfield, av);
+    if (tid() == 128)
+        rate[N.r_c_hv] = c_ionization_dev(N.alpha[N.r_c_hv], N.gama[N.r_c_hv], T, sc[S_CCOL], sc[S_H2COL], av,
+                                          radfield);
+    __syncthreads();
+}
+
+// value of one reaction's ODE term, tokens in the order MakeRates emits them
+__device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const double *y,
+                                             double D, double blr, double TS, double sm)
+{
+    double v = rate[k];
+    const int nd = N.n_d[k];
+    for (int q = 0; q < nd; q++) v = v * D;
+    const int *op = N.ops + MAXOPS * k;
+#pragma unroll
+    for (int q = 0; q < MAXOPS; q++) {
+        const int o = op[q];
+        if (o == OP_NONE) break;
+        if (o >= 0) v = v * y[o];
+        else if (o == OP_BLR) v = v * blr;
+        else if (o == OP_SWAP) v = v * TS / sm;
+        else v = v / sm;
+    }
+    return v;
+}
+
+// RHS f(y) (chemistry.f90:294-352 + the generated GETYDOT, odes.f90).  Input L.y, output
+// L.f; also leaves the pre-transfer surface total in sc[S_SPRE] for the Jacobian.
+__device__ void block_rhs(const DevNet &N, Lds &L)
+{
+    const int neq = N.neq;
+    double *y = L.y, *f = L.f, *rate = L.rate, *flux = L.flux, *sc = L.sc;
+    const double *prm = L.prm;
+    const double D = y[neq - 1];
+    const double radfield = prm[P_RADFIELD];
+    if (tid() == 0) {
+        double h2col = 0.5 * y[N.nh2] * D * (sc[S_CS] / 1.0);
+        rate[N.r_h2_hv] = h2_photodiss_dev(h2col, radfield, sc[S_SCAT]);
+    }
+    if (tid() == 64) {
+        double h2col = 0.5 * y[N.nh2] * D * (sc[S_CS] / 1.0);
+        double cocol = 0.5 * y[N.nco] * D * (sc[S_CS] / 1.0);
+        rate[N.r_co_hv] = co_photodiss_dev(h2col, cocol, radfield, sc[S_AV]);
+    }
+    const double sm = fmax(1e-30, y[N.nsurface]);
+    const double sbk = fmax(1e-30, y[N.nbulk]);
+    const double blr = fmin(1.0, NUM_SITES_PER_GRAIN / (GAS_DUST_DENSITY_RATIO * sbk));
+    __syncthreads();
+    double ts = 0.0;
+    for (int q = tid(); q < N.nswap; q += NT) ts += term_value(N, N.swap_list[q], rate, y, D, blr, 0.0, sm);
+    const double TS = block_sum(ts, L.red);
+    for (int k = tid(); k < N.nreac; k += NT) flux[k] = term_value(N, k, rate, y, D, blr, TS, sm);
+    __syncthreads();
+    double pre[EPT];
+    double spre = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int i = tid() + e * NT;
+        pre[e] = 0.0;
+        if (i < N.nspec) {
+            double loss = 0.0, prod = 0.0;
+            const int l0 = N.loss_ptr[i], l1 = N.loss_ptr[i + 1];
+            const int p0 = N.prod_ptr[i], p1 = N.prod_ptr[i + 1];
+            for (int q = l0; q < l1; q++) loss += flux[N.loss_idx[q]];
+            for (int q = p0; q < p1; q++) prod += flux[N.prod_idx[q]];
+            pre[e] = prod - loss;
+            if (N.is_surface[i]) spre += pre[e];
+        }
+    }
+    const double S = block_sum(spre, L.red);
+    double tb = 0.0, tsu = 0.0;
+    double out[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int i = tid() + e * NT;
+        out[e] = pre[e];
+        if (i < N.nspec) {
+            // bulk<->surface transfer (odes.f90:4818-5153)
+            if (N.is_surface[i]) {
+                const int b = N.bulk_partner_of_surface[i];
+                if (b >= 0) {
+                    if (S < 0) out[e] = out[e] - S * fmin(1.0, sbk / sm) * y[b] / sbk;
+                    else out[e] = out[e] - S * SURFACE_COVERAGE0 * y[i];
+                }
+                tsu += out[e];
+            } else if (N.is_bulk[i]) {
+                const int s = N.surface_partner_of_bulk[i];
+                if (S < 0) out[e] = out[e] + S * fmin(1.0, sbk / sm) * y[i] / sbk;
+                else out[e] = out[e] + S * SURFACE_COVERAGE0 * y[s];
+                tb += out[e];
+            }
+        }
+    }
+    block_sum2(tb, tsu, L.red);
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const int i = tid() + e * NT;
+        if (i < neq) {
+            double v = out[e];
+            if (i == N.nbulk) v = tb;
+            if (i == N.nsurface) v = tsu;
+            if (i == neq - 1) v = densdot_dev(prm, D);
+            f[i] = v;
+        }
+    }
+    if (tid() == 0) sc[S_SPRE] = S;
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- analytic Jacobian
+
+__device__ __forceinline__ double fac_value(int c, const double *y, double D, double blr, double ism, double tsm)
+{
+    if (c >= 0) return y[c];
+    if (c == FAC_D) return D;
+    if (c == OP_BLR) return blr;
+    if (c == OP_DIVSM) return ism;
+    return tsm;   // OP_SWAP
+}
+
+// dJ/dy for the factor's own variable
+__device__ __forceinline__ double fac_deriv(int c, double dblr, double dism, double dtsm)
+{
+    if (c >= 0 || c == FAC_D) return 1.0;
+    if (c == OP_BLR) return dblr;
+    if (c == OP_DIVSM) return dism;
+    return dtsm;
+}
+
+// Newton matrix M = I - gamma J(y) in column-major storage, J from the generated
+// structure plus the structured couplings (photo rates, totalSwap, ice transfer and the
+// BULK/SURFACE totals).  With scale != 0 the matrix is equilibrated by the error weights:
+// M' = W M W^-1, W = diag(ewt) (held in L.g).
+template <typename LUT>
+__device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma)
+{
+    const int neq = N.neq, ld = N.ld;
+    double *y = L.y, *rate = L.rate, *sc = L.sc, *g = L.g;
+    double *scr = L.flux;   // scratch [>= 2*neq]
+    const double D = y[neq - 1];
+    const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
+    const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
+    const double blr = fmin(1.0, NUM_SITES_PER_GRAIN / (GAS_DUST_DENSITY_RATIO * sbk));
+    const double ism = 1.0 / sm;
+    const double dblr = (blr < 1.0 && ybulk > 1e-30) ? -blr / sbk : 0.0;
+    const double dism = (ysur > 1e-30) ? -ism * ism : 0.0;
+    // totalSwap
+    double ts = 0.0;
+    for (int q = tid(); q < N.nswap; q += NT) ts += term_value(N, N.swap_list[q], rate, y, D, blr, 0.0, sm);
+    const double TS = block_sum(ts, L.red);
+    const double tsm = TS / sm;
+    const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
+    // 1. zero J
+    for (int idx = tid(); idx < neq * ld; idx += NT) M[idx] = (LUT)0;
+    __syncthreads();
+    // 2. generated sparse part
+    for (int n = tid(); n < N.nnz; n += NT) {
+        double acc = 0.0;
+        const int c0 = N.jnz_ptr[n], c1 = N.jnz_ptr[n + 1];
+        for (int q = c0; q < c1; q++) {
+            const int k = N.jc_k[q], m = N.jc_m[q];
+            const int *fc = N.fac + MAXFAC * k;
+            const int nf = N.nfac[k];
+            double p = rate[k];
+            for (int mm = 0; mm < nf; mm++) {
+                const int c = fc[mm];
+                p *= (mm == m) ? fac_deriv(c, dblr, dism, dtsm) : fac_value(c, y, D, blr, ism, tsm);
+            }
+            acc += N.jc_c[q] * p;
+        }
+        M[(size_t)N.jnz_col[n] * ld + N.jnz_row[n]] = (LUT)acc;
+    }
+    // 3a. totalSwap dependence of SURFSWAP terms: J[i][b] += u_i * dTS/dy_b
+    for (int i = tid(); i < neq; i += NT) {
+        double u = 0.0;
+        for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {
+            const int k = N.sw_k[q];
+            const int *fc = N.fac + MAXFAC * k;
+            double w = rate[k];
+            for (int mm = 0; mm < N.nfac[k]; mm++) {
+                const int c = fc[mm];
+                w *= (c == OP_SWAP) ? ism : fac_value(c, y, D, blr, ism, tsm);
+            }
+            u += N.sw_c[q] * w;
+        }
+        scr[i] = u;
+        scr[neq + i] = 0.0;
+    }
+    __syncthreads();
+    if (tid() == 0) {
+        for (int q = 0; q < N.nswap; q++) {
+            const int k = N.swap_list[q];
+            scr[neq + N.ops[MAXOPS * k]] += rate[k] * blr;
+        }
+        scr[neq + N.nbulk] = (blr < 1.0 && ybulk > 1e-30) ? -TS / sbk : 0.0;
+    }
+    __syncthreads();
+    for (int idx = tid(); idx < neq * (N.nbulkl + 1); idx += NT) {
+        const int i = idx % neq, q = idx / neq;
+        const double u = scr[i];
+        if (u == 0.0) continue;
+        const int b = (q < N.nbulkl) ? N.bulk_list[q] : N.nbulk;
+        const double gb = scr[neq + b];
+        if (gb != 0.0) M[(size_t)b * ld + i] = (LUT)((double)M[(size_t)b * ld + i] + u * gb);
+    }
+    __syncthreads();
+    // 3b. photo-rate dependence on y(H2), y(CO) and density through the column densities,
+    //     and the free-fall density row
+    if (tid() == 0) {
+        const double cs = sc[S_CS], G0 = L.prm[P_RADFIELD], av = sc[S_AV];
+        const double h2col = 0.5 * y[N.nh2] * D * cs, cocol = 0.5 * y[N.nco] * D * cs;
+        const double eps = 1e-6;
+        const double kh = h2_photodiss_dev(h2col, G0, sc[S_SCAT]);
+        const double dkh = h2col > 0 ? (h2_photodiss_dev(h2col * (1 + eps), G0, sc[S_SCAT]) - kh) / (h2col * eps) : 0.0;
+        const double kc = co_photodiss_dev(h2col, cocol, G0, av);
+        const double dkch = h2col > 0 ? (co_photodiss_dev(h2col * (1 + eps), cocol, G0, av) - kc) / (h2col * eps) : 0.0;
+        const double dkcc = cocol > 0 ? (co_photodiss_dev(h2col, cocol * (1 + eps), G0, av) - kc) / (cocol * eps) : 0.0;
+        for (int which = 0; which < 2; which++) {
+            const int k = which ? N.r_co_hv : N.r_h2_hv;
+            const int *fc = N.fac + MAXFAC * k;
+            double pf = 1.0;
+            for (int mm = 0; mm < N.nfac[k]; mm++) pf *= fac_value(fc[mm], y, D, blr, ism, tsm);
+            double d_h2 = which ? dkch * 0.5 * D * cs : dkh * 0.5 * D * cs;
+            double d_co = which ? dkcc * 0.5 * D * cs : 0.0;
+            double d_d = which ? (dkch * 0.5 * y[N.nh2] * cs + dkcc * 0.5 * y[N.nco] * cs) : dkh * 0.5 * y[N.nh2] * cs;
+            for (int q = N.aff_ptr[k]; q < N.aff_ptr[k + 1]; q++) {
+                const int i = N.aff_sp[q];
+                const double c = N.aff_cf[q] * pf;
+                M[(size_t)N.nh2 * ld + i] = (LUT)((double)M[(size_t)N.nh2 * ld + i] + c * d_h2);
+                if (d_co != 0.0) M[(size_t)N.nco * ld + i] = (LUT)((double)M[(size_t)N.nco * ld + i] + c * d_co);
+                M[(size_t)(neq - 1) * ld + i] = (LUT)((double)M[(size_t)(neq - 1) * ld + i] + c * d_d);
+            }
+        }
+        const double dn = D * 1e-7;
+        M[(size_t)(neq - 1) * ld + neq - 1] =
+            (LUT)((densdot_dev(L.prm, D + dn) - densdot_dev(L.prm, D)) / dn);
+    }
+    __syncthreads();
+    // 4. ice transfer: rows of surface/bulk species get -/+ (t_s * grad S + S * grad t_s)
+    const double S = sc[S_SPRE];
+    for (int j = tid(); j < neq; j += NT) {
+        double gsum = 0.0;
+        for (int q = 0; q < N.nsurfl; q++) gsum += (double)M[(size_t)j * ld + N.surface_list[q]];
+        scr[j] = gsum;
+    }
+    __syncthreads();
+    const bool neg = S < 0;
+    const bool ratio_lt1 = (sbk / sm) < 1.0;
+    for (int idx = tid(); idx < neq * N.nsurfl; idx += NT) {
+        const int q = idx % N.nsurfl, j = idx / N.nsurfl;
+        const int s = N.surface_list[q], b = N.bulk_partner_of_surface[s];
+        if (b < 0) continue;
+        double t = neg ? (ratio_lt1 ? y[b] / sm : y[b] / sbk) : SURFACE_COVERAGE0 * y[s];
+        double v = t * scr[j];
+        // derivative of t_s
+        if (neg) {
+            if (j == b) v += S * (ratio_lt1 ? 1.0 / sm : 1.0 / sbk);
+            if (ratio_lt1 && j == N.nsurface && ysur > 1e-30) v += S * (-y[b] / (sm * sm));
+            if (!ratio_lt1 && j == N.nbulk && ybulk > 1e-30) v += S * (-y[b] / (sbk * sbk));
+        } else if (j == s) {
+            v += S * SURFACE_COVERAGE0;
+        }
+        if (v == 0.0) continue;
+        M[(size_t)j * ld + s] = (LUT)((double)M[(size_t)j * ld + s] - v);
+        M[(size_t)j * ld + b] = (LUT)((double)M[(size_t)j * ld + b] + v);
+    }
+    __syncthreads();
+    // 5. BULK / SURFACE rows are the sums of their member rows
+    for (int j = tid(); j < neq; j += NT) {
+        double sb = 0.0, ss = 0.0;
+        for (int q = 0; q < N.nbulkl; q++) sb += (double)M[(size_t)j * ld + N.bulk_list[q]];
+        for (int q = 0; q < N.nsurfl; q++) ss += (double)M[(size_t)j * ld + N.surface_list[q]];
+        M[(size_t)j * ld + N.nbulk] = (LUT)sb;
+        M[(size_t)j * ld + N.nsurface] = (LUT)ss;
+    }
+    __syncthreads();
+    // 6. M = I - gamma W J W^-1
+    for (int idx = tid(); idx < neq * neq; idx += NT) {
+        const int j = idx / neq, i = idx % neq;
+        double v = -gamma * (double)M[(size_t)j * ld + i] * (g[i] / g[j]);
+        if (i == j) v += 1.0;
+        M[(size_t)j * ld + i] = (LUT)v;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- dense LU (no pivoting)
+
+constexpr int LU_NB = 16;
+
+// Right-looking blocked LU of the column-major n x n matrix M (unit lower L, upper U
+// stored in place).  Returns nonzero (uniformly) on a zero or non-finite pivot.
+template <typename LUT>
+__device__ int block_lu(LUT *M, int n, int ld, double *red)
+{
+    for (int kb = 0; kb < n; kb += LU_NB) {
+        const int nb = min(LU_NB, n - kb);
+        // panel: columns kb..kb+nb-1
+        for (int j = kb; j < kb + nb; j++) {
+            __syncthreads();
+            const LUT piv = M[(size_t)j * ld + j];
+            if (!(piv == piv) || piv == (LUT)0 || isinf((double)piv)) return 1;
+            const LUT inv = (LUT)1 / piv;
+            for (int i = j + 1 + tid(); i < n; i += NT) {
+                LUT l = M[(size_t)j * ld + i] * inv;
+                M[(size_t)j * ld + i] = l;
+                for (int c = j + 1; c < kb + nb; c++) M[(size_t)c * ld + i] -= l * M[(size_t)c * ld + j];
+            }
+        }
+        __syncthreads();
+        const int c0 = kb + nb;
+        if (c0 >= n) break;
+        // U12 = L11^-1 A12 (one column per thread)
+        for (int c = c0 + tid(); c < n; c += NT) {
+            LUT a[LU_NB];
+#pragma unroll
+            for (int r = 0; r < LU_NB; r++) a[r] = (r < nb) ? M[(size_t)c * ld + kb + r] : (LUT)0;
+#pragma unroll
+            for (int r = 1; r < LU_NB; r++) {
+                if (r < nb) {
+                    LUT acc = a[r];
+#pragma unroll
+                    for (int p = 0; p < r; p++) acc -= M[(size_t)(kb + p) * ld + kb + r] * a[p];
+                    a[r] = acc;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < LU_NB; r++)
+                if (r < nb) M[(size_t)c * ld + kb + r] = a[r];
+        }
+        __syncthreads();
+        // A22 -= L21 U12 (threads own rows; U12 columns are read uniformly)
+        for (int i = c0 + tid(); i < n; i += NT) {
+            LUT l[LU_NB];
+#pragma unroll
+            for (int p = 0; p < LU_NB; p++) l[p] = (p < nb) ? M[(size_t)(kb + p) * ld + i] : (LUT)0;
+            for (int c = c0; c < n; c++) {
+                const LUT *u = M + (size_t)c * ld + kb;
+                LUT acc = (LUT)0;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++) acc += l[p] * ((p < nb) ? u[p] : (LUT)0);
+                M[(size_t)c * ld + i] -= acc;
+            }
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
+constexpr int SB = 64;   // triangular-solve block (one wave64 per diagonal block)
+
+// Solve (LU) x = b in place on the LDS vector b (FP64 accumulation).
+template <typename LUT>
+__device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
+{
+    const int lane = tid() & 63, wave = tid() >> 6;
+    // forward, unit lower
+    for (int kb = 0; kb < n; kb += SB) {
+        const int nb = min(SB, n - kb);
+        if (wave == 0) {
+            double x = (lane < nb) ? b[kb + lane] : 0.0;
+            for (int c = 0; c < nb - 1; c++) {
+                double xc = __shfl(x, c);
+                if (lane > c && lane < nb) x -= (double)M[(size_t)(kb + c) * ld + kb + lane] * xc;
+            }
+            if (lane < nb) b[kb + lane] = x;
+        }
+        __syncthreads();
+        for (int i = kb + nb + tid(); i < n; i += NT) {
+            double acc = b[i];
+            for (int p = 0; p < nb; p++) acc -= (double)M[(size_t)(kb + p) * ld + i] * b[kb + p];
+            b[i] = acc;
+        }
+        __syncthreads();
+    }
+    // backward, upper
+    const int nblk = (n + SB - 1) / SB;
+    for (int blk = nblk - 1; blk >= 0; blk--) {
+        const int kb = blk * SB;
+        const int nb = min(SB, n - kb);
+        if (wave == 0) {
+            double x = (lane < nb) ? b[kb + lane] : 0.0;
+            for (int c = nb - 1; c >= 0; c--) {
+                if (lane == c) x = x / (double)M[(size_t)(kb + c) * ld + kb + c];
+                double xc = __shfl(x, c);
+                if (lane < c) x -= (double)M[(size_t)(kb + c) * ld + kb + lane] * xc;
+            }
+            if (lane < nb) b[kb + lane] = x;
+        }
+        __syncthreads();
+        for (int i = tid(); i < kb; i += NT) {
+            double acc = b[i];
+            for (int p = 0; p < nb; p++) acc -= (double)M[(size_t)(kb + p) * ld + i] * b[kb + p];
+            b[i] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- BDF ops policy
+
+template <typename LUT>
+struct GpuOps {
+    const DevNet &N;
+    Lds &L;
+    LUT *M;
+    int neq;
+    double z[6][EPT], acor[EPT], ewt[EPT], atol[EPT], f0[EPT];
+    uint64_t t_rhs = 0, t_jac = 0, t_lu = 0, t_sol = 0;   // device wall-clock ticks per phase
+
+    __device__ GpuOps(const DevNet &n_, Lds &l_, LUT *m_) : N(n_), L(l_), M(m_), neq(n_.neq) {}
+
+    __device__ __forceinline__ bool own(int e) const { return tid() + e * NT < neq; }
+    __device__ __forceinline__ int idx(int e) const { return tid() + e * NT; }
+
+    // z[j][e] with a run-time j but compile-time storage indices (keeps z in VGPRs)
+    __device__ __forceinline__ double zget(int j, int e) const
+    {
+        switch (j) {
+        case 0: return z[0][e];
+        case 1: return z[1][e];
+        case 2: return z[2][e];
+        case 3: return z[3][e];
+        case 4: return z[4][e];
+        default: return z[5][e];
+        }
+    }
+    __device__ __forceinline__ void zset(int j, int e, double v)
+    {
+        switch (j) {
+        case 0: z[0][e] = v; break;
+        case 1: z[1][e] = v; break;
+        case 2: z[2][e] = v; break;
+        case 3: z[3][e] = v; break;
+        case 4: z[4][e] = v; break;
+        default: z[5][e] = v; break;
+        }
+    }
+    __device__ double wrms(int v)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                double x = (v == bdf::ACOR) ? acor[e] : zget(v, e);
+                double t = x * ewt[e];
+                acc += t * t;
+            }
+        return sqrt(block_sum(acc, L.red) / neq);
+    }
+    __device__ double wrms_acor_minus(double c, int zj)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                double t = (acor[e] - c * zget(zj, e)) * ewt[e];
+                acc += t * t;
+            }
+        return sqrt(block_sum(acc, L.red) / neq);
+    }
+    __device__ bool set_ewt(double rtol)
+    {
+        double bad = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                double d = rtol * fabs(z[0][e]) + atol[e];
+                if (!(d > 0.0)) bad = 1.0;
+                ewt[e] = 1.0 / d;
+            }
+        return block_max(bad, L.red) == 0.0;
+    }
+    __device__ void start_corrector()
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                acor[e] = 0.0;
+                L.y[idx(e)] = z[0][e];
+            }
+        __syncthreads();
+    }
+    __device__ void rhs()
+    {
+        uint64_t t0 = wall_clock64();
+        block_rhs(N, L);
+        t_rhs += wall_clock64() - t0;
+    }
+    __device__ void save_f0()
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) f0[e] = L.f[idx(e)];
+    }
+    __device__ void scale_into_z_from(int j, double a, int src)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) zset(j, e, a * ((src == bdf::F0) ? f0[e] : L.f[idx(e)]));
+    }
+    __device__ void scale_into_z(int j, double a, int src)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++) zset(j, e, a * zget(src, e));
+    }
+    __device__ void axpy_zz(int j, double a, int src)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++) zset(j, e, zget(j, e) + a * zget(src, e));
+    }
+    __device__ void rescale_z(int q, double eta)
+    {
+        double f = eta;
+#pragma unroll
+        for (int j = 1; j <= bdf::QMAX; j++) {
+            if (j <= q) {
+#pragma unroll
+                for (int e = 0; e < EPT; e++) z[j][e] *= f;
+                f *= eta;
+            }
+        }
+    }
+    __device__ void predict(int q)
+    {
+#pragma unroll
+        for (int k = 1; k <= bdf::QMAX; k++)
+#pragma unroll
+            for (int j = bdf::QMAX; j >= k; j--)
+                if (k <= q && j <= q)
+#pragma unroll
+                    for (int e = 0; e < EPT; e++) z[j - 1][e] += z[j][e];
+    }
+    __device__ void unpredict(int q)
+    {
+#pragma unroll
+        for (int k = 1; k <= bdf::QMAX; k++)
+#pragma unroll
+            for (int j = bdf::QMAX; j >= k; j--)
+                if (k <= q && j <= q)
+#pragma unroll
+                    for (int e = 0; e < EPT; e++) z[j - 1][e] -= z[j][e];
+    }
+    __device__ void complete(int q, const double *l)
+    {
+#pragma unroll
+        for (int j = 0; j <= bdf::QMAX; j++)
+            if (j <= q)
+#pragma unroll
+                for (int e = 0; e < EPT; e++) z[j][e] += l[j] * acor[e];
+    }
+    __device__ void copy_acor_to_z(int j)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++) zset(j, e, acor[e]);
+    }
+    __device__ void form_residual(double gamma, double rl1)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                const int i = idx(e);
+                // equilibrated system: b' = W b
+                L.b[i] = (gamma * L.f[i] - (rl1 * z[1][e] + acor[e])) * ewt[e];
+            }
+        __syncthreads();
+    }
+    __device__ void solve()
+    {
+        uint64_t t0 = wall_clock64();
+        block_lu_solve<LUT>(M, neq, N.ld, L.b);
+        t_sol += wall_clock64() - t0;
+    }
+    __device__ double apply_correction(double scale)
+    {
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                const int i = idx(e);
+                double dw = scale * L.b[i];     // weighted correction W d
+                double d = dw / ewt[e];
+                acc += dw * dw;
+                acor[e] += d;
+                L.y[i] = z[0][e] + acor[e];
+            }
+        double r = sqrt(block_sum(acc, L.red) / neq);
+        return r;
+    }
+    __device__ int setup(double gamma)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) L.g[idx(e)] = ewt[e];
+        __syncthreads();
+        uint64_t t0 = wall_clock64();
+        block_newton_matrix<LUT>(N, L, M, gamma);
+        uint64_t t1 = wall_clock64();
+        int r = block_lu<LUT>(M, neq, N.ld, L.red);
+        uint64_t t2 = wall_clock64();
+        t_jac += t1 - t0;
+        t_lu += t2 - t1;
+        return r;
+    }
+    __device__ double hub_inverse(double fac)
+    {
+        double m = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) m = fmax(m, fabs(f0[e]) / (fac * fabs(z[0][e]) + atol[e]));
+        return block_max(m, L.red);
+    }
+    __device__ double ydd_norm(double t0, double hg)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) L.y[idx(e)] = z[0][e] + hg * f0[e];
+        __syncthreads();
+        block_rhs(N, L);
+        double acc = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; e++)
+            if (own(e)) {
+                double t = (L.f[idx(e)] - f0[e]) / hg * ewt[e];
+                acc += t * t;
+            }
+        return sqrt(block_sum(acc, L.red) / neq);
+    }
+    __device__ void finish_at_tn() {}
+    __device__ void interpolate(double s, int q)
+    {
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = bdf::QMAX; j >= 0; j--)
+                if (j <= q) acc = (j == q) ? z[j][e] : acc * s + z[j][e];
+            z[0][e] = acc;
+        }
+    }
+};
+
+// ---------------------------------------------------------------- persistent solve kernel
+
+constexpr int MAX_INTERVALS = 100000;
+
+struct SolveArgs {
+    int n_points;
+    const double *params;     // [n][P_NPARAM]
+    const double *y_init;     // [n][neq] or null (elemental start, chemistry.f90:46-141)
+    double *y_out;            // [n][neq]
+    int max_rec;              // rows per point in rec (0: none)
+    double *rec;              // [n][max_rec][4 + nspec]
+    int *n_rec;               // [n]
+    int *status;              // [n]
+    int *stats;               // [n][8]: nst nfe nje nni ncfn netf n_intervals n_retries
+    double *timers;           // [n][8] seconds: rates, rhs, jacobian, lu, solve, total, -, - (or null)
+    double *t_start;          // [n] device wall clock when the point was taken, or null
+    double *t_done;           // [n] device wall clock at retirement, or null
+    void *ws;                 // slot workspaces
+    size_t ws_stride;         // bytes per slot
+    int *counter;             // work queue head
+};
+
+__device__ void elemental_start(const DevNet &N, const double *prm, double *y, double density)
+{   // chemistry.f90:63-105 with the default elemental abundances (defaultparameters.f90:68-90)
+    const int neq = N.neq;
+    for (int i = tid(); i < neq; i += NT) y[i] = MIN_ABUND;
+    __syncthreads();
+    if (tid() == 0) {
+        y[N.no] = 3.34e-04;
+        y[N.nn] = 6.18e-05;
+        y[N.nmg] = 2.256e-06;
+        y[N.np] = 7.78e-08;
+        y[N.nf] = 3.6e-08;
+        y[N.nna] = 0.0;
+        y[N.nli] = 0.0;
+        y[N.npah] = 0.0;
+        y[N.nsx] = 3.51e-6;
+        y[N.nsix] = 1.78e-06;
+        y[N.nclx] = 3.39e-08;
+        y[N.nc] = 1.0e-10;
+        y[N.ncx] = 1.77e-04;
+        y[N.n18o] = 0.0;
+        y[N.n15n] = 0.0;
+        y[N.n13c] = 0.0;
+        y[N.nelec] = y[N.ncx] + y[N.nsix] + y[N.nsx] + y[N.nclx] + y[N.nmgx];
+        y[N.nh] = 0.5;
+        y[N.nh2] = 0.5 * (1.0 - 0.5);
+        y[N.nd] = 0.0;
+        y[N.nhe] = (double)0.1f;
+        y[neq - 1] = density;
+    }
+    __syncthreads();
+}
+
+__device__ void record_row(const DevNet &N, Lds &L, const SolveArgs &A, int pidx, int row, double tyears)
+{
+    if (A.max_rec <= 0 || row >= A.max_rec) return;
+    const int ns = N.nspec;
+    double *r = A.rec + ((size_t)pidx * A.max_rec + row) * (4 + ns);
+    if (tid() == 0) {
+        r[0] = tyears;
+        r[1] = L.sc[S_DENS];
+        r[2] = L.sc[S_TGAS];
+        r[3] = L.sc[S_AV];
+    }
+    for (int i = tid(); i < ns; i += NT) r[4 + i] = L.y[i];
+}
+
+template <typename LUT>
+__global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
+{
+    extern __shared__ double lds_raw[];
+    const int neq = N.neq, nr = N.nreac;
+    const int npad = (neq + 7) & ~7;
+    Lds L;
+    L.rate = lds_raw;
+    L.flux = L.rate + nr;
+    L.y = L.flux + (nr > 2 * npad ? nr : 2 * npad);
+    L.f = L.y + npad;
+    L.b = L.f + npad;
+    L.g = L.b + npad;
+    L.red = L.g + npad;
+    L.sc = L.red + 16;
+    L.prm = L.sc + S_NSCAL;
+    __shared__ int s_point;
+    LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
+
+    for (;;) {
+        if (tid() == 0) s_point = atomicAdd(A.counter, 1);
+        __syncthreads();
+        const int pidx = s_point;
+        __syncthreads();
+        if (pidx >= A.n_points) break;   // every wave of the workgroup leaves here
+
+        if (A.t_start && tid() == 0) A.t_start[pidx] = (double)wall_clock64();
+        // ---- point set-up: parameters, physics (physics-core.f90:42-73, cloud.f90:20-31)
+        const double *pin = A.params + (size_t)pidx * P_NPARAM;
+        for (int i = tid(); i < P_NPARAM; i += NT) L.prm[i] = pin[i];
+        __syncthreads();
+        const double *prm = L.prm;
+        if (tid() == 0) {
+            double cs = (prm[P_ROUT] - prm[P_RIN]) * PC;
+            L.sc[S_CS] = cs;
+            L.sc[S_TGAS] = prm[P_TEMP];
+            L.sc[S_TDUST] = prm[P_TEMP];
+            double dens = prm[P_DENS];
+            L.sc[S_COLDENS] = 1.0 * cs / 1.0 * prm[P_DENS];
+            L.sc[S_AV] = prm[P_BASEAV] + L.sc[S_COLDENS] / 1.6e21;
+            if (prm[P_FREEFALL] != 0.0) dens = (double)1.001f * prm[P_DENS];
+            L.sc[S_DENS] = dens;
+            L.sc[S_PHI] = 1.0e5;
+            L.sc[S_ABSFAC] = prm[P_ABSFAC];
+        }
+        __syncthreads();
+        if (A.y_init) {
+            for (int i = tid(); i < neq; i += NT) L.y[i] = A.y_init[(size_t)pidx * neq + i];
+            __syncthreads();
+        } else {
+            elemental_start(N, prm, L.y, L.sc[S_DENS]);
+        }
+        GpuOps<LUT> o(N, L, M);
+        bdf::Stats st;
+        int row = 0, nint = 0, nretry = 0, flag = 0;
+        record_row(N, L, A, pidx, row++, 0.0);
+        double t = 0.0, tyears = 0.0;
+        const int model = (int)prm[P_MODEL];
+        const double reltol = prm[P_RELTOL], absmin = prm[P_ABSMIN];
+        const int mxstep = (int)prm[P_MXSTEP];
+        const bool end_dens = prm[P_ENDATFINALDENS] != 0.0;
+        // ---- time loop (wrap.f90:657-690).  The reference loops forever when the
+        // integrator cannot advance t at all; here such a point stops with
+        // INT_TOO_MANY_FAILS_ERROR after 10 intervals without progress.
+        int stuck = 0;
+        uint64_t t_rates = 0;
+        const uint64_t t_point0 = wall_clock64();
+        while ((end_dens && L.sc[S_DENS] < prm[P_FINALDENS]) || (!end_dens && tyears < prm[P_FINALTIME])) {
+            if (nint >= MAX_INTERVALS || stuck > 10) {
+                flag = -5;
+                break;
+            }
+            const double t_before = t;
+            tyears = t / SECONDS_PER_YEAR;
+            double target = next_target_dev(model, tyears);
+            const double orig = target;
+            int loops = 0;
+            while (t < target && loops < 10) {   // updateChemistry retry loop (chemistry.f90:164-223)
+                const uint64_t tr0 = wall_clock64();
+                block_rates(N, L);
+                t_rates += wall_clock64() - tr0;
+                const double absfac = L.sc[S_ABSFAC];
+#pragma unroll
+                for (int e = 0; e < EPT; e++)
+                    if (o.own(e)) {
+                        double yi = L.y[o.idx(e)];
+                        o.z[0][e] = yi;
+                        double a = absfac * yi;
+                        o.atol[e] = a < absmin ? absmin : a;
+                    }
+                __syncthreads();
+                int is = bdf::integrate(o, t, target, reltol, mxstep, st);
+                if (is == -2 && tid() == 0) L.sc[S_ABSFAC] = absfac * 10.0;
+                if (is < 0) nretry++;
+                if (is == -3) {
+                    flag = -4;   // INT_UNRECOVERABLE_ERROR (constants.f90:25)
+                }
+                // clamp (chemistry.f90:215) and density bookkeeping (:216)
+#pragma unroll
+                for (int e = 0; e < EPT; e++)
+                    if (o.own(e)) {
+                        double yi = o.z[0][e];
+                        if (!(yi >= MIN_ABUND)) yi = (yi == yi) ? MIN_ABUND : yi;
+                        L.y[o.idx(e)] = yi;
+                    }
+                __syncthreads();
+                if (tid() == 0) L.sc[S_DENS] = L.y[neq - 1];
+                __syncthreads();
+                if (flag) break;
+                loops++;
+                target = orig;
+            }
+            if (flag) break;
+            nint++;
+            stuck = (t > t_before) ? 0 : stuck + 1;
+            tyears = target / SECONDS_PER_YEAR;
+            // physics update (physics-core.f90:75-88, hotcore.f90:78-90)
+            if (tid() == 0) {
+                const double cs = L.sc[S_CS];
+                L.sc[S_COLDENS] = cs / 1.0 * L.sc[S_DENS];
+                L.sc[S_AV] = prm[P_BASEAV] + L.sc[S_COLDENS] / 1.6e21;
+                L.sc[S_TDUST] = L.sc[S_TGAS];
+                if (model == 1) {
+                    const double tempa[6] = {1.927e-1, 4.8560e-2, 7.8470e-3, 9.6966e-4, 1.706e-4, 4.74e-7};
+                    const float tempb[6] = {0.5339f, 0.6255f, 0.8395f, 1.085f, 1.289f, 1.98f};
+                    const int ti = (int)prm[P_TEMPINDX] - 1;
+                    const double maxt = prm[P_MAXTEMP];
+                    if (L.sc[S_TGAS] < maxt) {
+                        double gt = (cs / (prm[P_ROUT] * PC)) * ((double)1.0f / (double)1.0f);
+                        gt = pow(gt, (double)-0.5f);
+                        gt = prm[P_TEMP] + ((tempa[ti] * pow(t / SECONDS_PER_YEAR, (double)tempb[ti])) * gt);
+                        if (gt > maxt) gt = maxt;
+                        L.sc[S_TGAS] = gt;
+                    }
+                    L.sc[S_TDUST] = L.sc[S_TGAS];
+                }
+            }
+            __syncthreads();
+            if (row < A.max_rec) record_row(N, L, A, pidx, row, tyears);
+            row++;
+        }
+        if (A.max_rec > 0 && row > A.max_rec && flag == 0) flag = -6;   // NOT_ENOUGH_TIMEPOINTS_ERROR
+        // ---- retire the point
+        for (int i = tid(); i < neq; i += NT) A.y_out[(size_t)pidx * neq + i] = L.y[i];
+        if (tid() == 0) {
+            A.status[pidx] = flag;
+            if (A.n_rec) A.n_rec[pidx] = row < A.max_rec ? row : A.max_rec;
+            int *s = A.stats + (size_t)pidx * 8;
+            s[0] = st.nst; s[1] = st.nfe; s[2] = st.nje; s[3] = st.nni;
+            s[4] = st.ncfn; s[5] = st.netf; s[6] = nint; s[7] = nretry;
+            if (A.t_done) A.t_done[pidx] = (double)wall_clock64();
+            if (A.timers) {
+                const double tick = 1.0e-8;   // wall_clock64 runs at 100 MHz
+                double *tm = A.timers + (size_t)pidx * 8;
+                tm[0] = t_rates * tick; tm[1] = o.t_rhs * tick; tm[2] = o.t_jac * tick; tm[3] = o.t_lu * tick;
+                tm[4] = o.t_sol * tick; tm[5] = (wall_clock64() - t_point0) * tick; tm[6] = 0; tm[7] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- fine-grained kernels (tests)
+
+struct PieceArgs {
+    int n;
+    const double *params;   // [n][P_NPARAM]
+    const double *y;        // [n][neq]
+    const double *rate_in;  // [n][nreac] (rhs/jac) or null
+    const double *phys;     // [n][8]: density, Tgas, Tdust, Av, cloudSize, phi, abstol_factor, unused
+    double *rate_out;       // [n][nreac]
+    double *y_out;          // [n][neq] (rates: y after the ice reset; rhs: ydot)
+    double *scal_out;       // [n][8]
+    double *jac_out;        // [n][neq*neq] column-major, J (not M)
+    double gamma;
+    void *ws;
+    size_t ws_stride;
+};
+
+__device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
+{
+    for (int i = tid(); i < P_NPARAM; i += NT) L.prm[i] = A.params[(size_t)p * P_NPARAM + i];
+    for (int i = tid(); i < N.neq; i += NT) L.y[i] = A.y[(size_t)p * N.neq + i];
+    if (tid() == 0) {
+        const double *ph = A.phys + (size_t)p * 8;
+        L.sc[S_DENS] = ph[0];
+        L.sc[S_TGAS] = ph[1];
+        L.sc[S_TDUST] = ph[2];
+        L.sc[S_AV] = ph[3];
+        L.sc[S_CS] = ph[4];
+        L.sc[S_PHI] = ph[5];
+        L.sc[S_ABSFAC] = ph[6];
+    }
+    if (A.rate_in)
+        for (int k = tid(); k < N.nreac; k += NT) L.rate[k] = A.rate_in[(size_t)p * N.nreac + k];
+    __syncthreads();
+    if (tid() == 64) L.sc[S_SCAT] = scatter_dev(1000.0, L.sc[S_AV]);
+    __syncthreads();
+}
+
+__device__ void piece_lds(const DevNet &N, Lds &L, double *lds_raw)
+{
+    const int neq = N.neq, nr = N.nreac;
+    const int npad = (neq + 7) & ~7;
+    L.rate = lds_raw;
+    L.flux = L.rate + nr;
+    L.y = L.flux + (nr > 2 * npad ? nr : 2 * npad);
+    L.f = L.y + npad;
+    L.b = L.f + npad;
+    L.g = L.b + npad;
+    L.red = L.g + npad;
+    L.sc = L.red + 16;
+    L.prm = L.sc + S_NSCAL;
+}
+
+__global__ __launch_bounds__(NT) void rates_kernel(DevNet N, PieceArgs A)
+{
+    extern __shared__ double lds_raw[];
+    Lds L;
+    piece_lds(N, L, lds_raw);
+    const int p = blockIdx.x;
+    if (p >= A.n) return;
+    piece_setup(N, L, A, p);
+    block_rates(N, L);
+    for (int k = tid(); k < N.nreac; k += NT) A.rate_out[(size_t)p * N.nreac + k] = L.rate[k];
+    for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = L.y[i];
+    if (tid() == 0) {
+        double *s = A.scal_out + (size_t)p * 8;
+        s[0] = L.sc[S_H2COL]; s[1] = L.sc[S_COCOL]; s[2] = L.sc[S_CCOL]; s[3] = L.sc[S_SM];
+        s[4] = L.sc[S_SBK]; s[5] = L.sc[S_BLR]; s[6] = L.sc[S_PHI]; s[7] = L.sc[S_AV];
+    }
+}
+
+__global__ __launch_bounds__(NT) void rhs_kernel(DevNet N, PieceArgs A)
+{
+    extern __shared__ double lds_raw[];
+    Lds L;
+    piece_lds(N, L, lds_raw);
+    const int p = blockIdx.x;
+    if (p >= A.n) return;
+    piece_setup(N, L, A, p);
+    block_rhs(N, L);
+    for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = L.f[i];
+    if (A.rate_out)
+        for (int k = tid(); k < N.nreac; k += NT) A.rate_out[(size_t)p * N.nreac + k] = L.rate[k];
+}
+
+__global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
+{
+    extern __shared__ double lds_raw[];
+    Lds L;
+    piece_lds(N, L, lds_raw);
+    const int p = blockIdx.x;
+    if (p >= A.n) return;
+    piece_setup(N, L, A, p);
+    block_rhs(N, L);   // leaves the pre-transfer surface total and the photo rates at y
+    // J itself: gamma = -1 and unit weights give M = I + J; subtract I on output
+    for (int i = tid(); i < N.neq; i += NT) L.g[i] = 1.0;
+    __syncthreads();
+    double *M = (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
+    block_newton_matrix<double>(N, L, M, -1.0);
+    const int neq = N.neq;
+    for (int idx = tid(); idx < neq * neq; idx += NT) {
+        const int j = idx / neq, i = idx % neq;
+        double v = M[(size_t)j * N.ld + i];
+        if (i == j) v -= 1.0;
+        A.jac_out[(size_t)p * neq * neq + idx] = v;
+    }
+}
+
+// LU + solve of M = I - gamma J: returns x = M^-1 b (b passed in y, x returned in y_out)
+template <typename LUT>
+__global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
+{
+    extern __shared__ double lds_raw[];
+    Lds L;
+    piece_lds(N, L, lds_raw);
+    const int p = blockIdx.x;
+    if (p >= A.n) return;
+    // y holds the state for J; the right-hand side is rate_in reinterpreted as [n][neq]
+    piece_setup(N, L, A, p);
+    block_rates(N, L);
+    block_rhs(N, L);
+    for (int i = tid(); i < N.neq; i += NT) L.g[i] = 1.0;
+    __syncthreads();
+    LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
+    block_newton_matrix<LUT>(N, L, M, A.gamma);
+    int bad = block_lu<LUT>(M, N.neq, N.ld, L.red);
+    for (int i = tid(); i < N.neq; i += NT) L.b[i] = A.scal_out[(size_t)p * N.neq + i];
+    __syncthreads();
+    if (!bad) block_lu_solve<LUT>(M, N.neq, N.ld, L.b);
+    for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = bad ? NAN : L.b[i];
+}
+
+template __global__ void solve_points_kernel<double>(DevNet, SolveArgs);
+template __global__ void solve_points_kernel<float>(DevNet, SolveArgs);
+template __global__ void newton_solve_kernel<double>(DevNet, PieceArgs);
+template __global__ void newton_solve_kernel<float>(DevNet, PieceArgs);
+
+}  // namespace ucg

@@ -1,0 +1,232 @@
+"""ctypes binding of libuclchem_gpu.so (the C ABI declared in include/uclchem_gpu.h).
+
+The HIP library is the only compute path: if it is missing or no GPU is visible every
+call raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import makerates_hip as mh
+from .params import NPARAM
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libuclchem_gpu.so")
+
+SPECIAL_SPECIES_ORDER = ["nh", "nh2", "nhe", "nc", "ncx", "nn", "no", "nf", "np", "nmg", "nmgx", "nna",
+                         "nli", "npah", "nsx", "nsix", "nclx", "nd", "n18o", "n15n", "n13c", "nelec", "nco",
+                         "nsurface", "nbulk"]
+SPECIAL_REACTION_ORDER = ["nR_H2Form_CT", "nR_H2Form_ER", "nR_H2Form_ERDes", "nR_HFreeze", "nR_EFreeze",
+                          "nR_H2Freeze", "nR_H2_hv", "nR_CO_hv", "nR_C_hv"]
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in
+                 ["nspec", "neq", "nreac", "nice", "nsurfl", "nbulkl", "nfp", "nswap",
+                  "nloss", "nprod", "nnz", "ncontrib", "naff", "nsw"]] +
+                [(n, ctypes.c_void_p) for n in
+                 ["alpha", "beta", "gama", "tmin", "tmax", "mass", "rstat", "rtype", "re", "pr",
+                  "ice_kind", "n_d", "ops", "fac", "nfac", "extrapolate", "type_ranges", "loss_ptr",
+                  "loss_idx", "prod_ptr", "prod_idx", "swap_list", "surface_list", "bulk_list",
+                  "bulk_partner_of_surface", "surface_partner_of_bulk", "freeze_partners",
+                  "is_surface", "is_bulk", "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c",
+                  "aff_ptr", "aff_sp", "aff_cf", "sw_ptr", "sw_k", "sw_c"]] +
+                [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 9)])
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library once; raise loudly when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpuError(f"{LIB_PATH} is not built; run `python -m uclchem_amd.build` "
+                           "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ucg_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise GpuError(f"{what} failed ({rc}): {lib().ucg_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Network:
+    """A network resident on the GPU (tables uploaded once, immutable)."""
+
+    def __init__(self, tables: mh.NetworkTables, device: int = 0):
+        self.t = tables
+        self.nspec, self.neq, self.nreac = tables.nspec, tables.neq, tables.nreac
+        _check(lib().ucg_init(ctypes.c_int(device)), "ucg_init")
+        a = tables.arrays
+        keep = {}
+
+        def i32(x):
+            arr = np.ascontiguousarray(x, dtype=np.int32)
+            keep[id(arr)] = arr
+            return arr
+
+        def f64(x):
+            arr = np.ascontiguousarray(x, dtype=np.float64)
+            keep[id(arr)] = arr
+            return arr
+
+        neq = tables.neq
+        bps = np.full(neq, -1, dtype=np.int32)
+        for q, s in enumerate(a["surface_list"]):
+            bps[s] = a["bulk_partner_of_surface"][q]
+        spb = np.full(neq, -1, dtype=np.int32)
+        for q, b in enumerate(a["bulk_list"]):
+            spb[b] = a["surface_partner_of_bulk"][q]
+        is_s = np.zeros(neq, dtype=np.int32)
+        is_s[a["surface_list"]] = 1
+        is_b = np.zeros(neq, dtype=np.int32)
+        is_b[a["bulk_list"]] = 1
+        d = _Desc()
+        d.nspec, d.neq, d.nreac = tables.nspec, neq, tables.nreac
+        d.nice = len(a["ice_list"])
+        d.nsurfl, d.nbulkl = len(a["surface_list"]), len(a["bulk_list"])
+        d.nfp, d.nswap = len(a["freeze_partners"]), len(a["swap_list"])
+        d.nloss, d.nprod = len(a["loss_idx"]), len(a["prod_idx"])
+        d.nnz, d.ncontrib = len(a["jnz_row"]), len(a["jc_k"])
+        d.naff, d.nsw = len(a["aff_sp"]), len(a["sw_k"])
+        for n in ["alpha", "beta", "gama", "tmin", "tmax", "mass", "rstat"]:
+            setattr(d, n, _p(f64(a[n])).value)
+        for n in ["rtype", "re", "pr", "ice_kind", "n_d", "ops", "fac", "nfac", "type_ranges", "loss_ptr",
+                  "loss_idx", "prod_ptr", "prod_idx", "swap_list", "surface_list", "bulk_list",
+                  "freeze_partners", "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c", "aff_ptr",
+                  "aff_sp", "aff_cf", "sw_ptr", "sw_k", "sw_c"]:
+            setattr(d, n, _p(i32(a[n])).value)
+        ex = np.ascontiguousarray(a["extrapolate"], dtype=np.int8)
+        keep[id(ex)] = ex
+        d.extrapolate = _p(ex).value
+        d.bulk_partner_of_surface = _p(i32(bps)).value
+        d.surface_partner_of_bulk = _p(i32(spb)).value
+        d.is_surface = _p(i32(is_s)).value
+        d.is_bulk = _p(i32(is_b)).value
+        for i, k in enumerate(SPECIAL_SPECIES_ORDER):
+            d.special_species[i] = tables.scalars[k]
+        for i, k in enumerate(SPECIAL_REACTION_ORDER):
+            d.special_reactions[i] = tables.scalars[k]
+        h = ctypes.c_int(-1)
+        _check(lib().ucg_network_load(ctypes.byref(d), ctypes.byref(h)), "ucg_network_load")
+        self.handle = h.value
+
+    # ------------------------------------------------------------ full solves
+    def solve(self, params: np.ndarray, y_init: Optional[np.ndarray] = None, max_rec: int = 0,
+              lu_bits: int = 64):
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        if params.shape[1] != NPARAM:
+            raise ValueError(f"params must be [n, {NPARAM}]")
+        y0 = None
+        if y_init is not None:
+            y0 = np.ascontiguousarray(np.atleast_2d(y_init), dtype=np.float64)
+            if y0.shape != (n, self.neq):
+                raise ValueError("y_init must be [n, neq]")
+        y_out = np.zeros((n, self.neq))
+        rec = np.zeros((n, max_rec, 4 + self.nspec)) if max_rec > 0 else None
+        n_rec = np.zeros(n, dtype=np.int32)
+        status = np.zeros(n, dtype=np.int32)
+        stats = np.zeros((n, 8), dtype=np.int32)
+        rc = lib().ucg_solve_batch(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(y0), _p(y_out),
+                                   ctypes.c_int(max_rec), _p(rec), _p(n_rec), _p(status), _p(stats),
+                                   ctypes.c_int(lu_bits))
+        _check(rc, "ucg_solve_batch")
+        return dict(y=y_out, rec=rec, n_rec=n_rec, status=status, stats=stats)
+
+    def solve_profiled(self, params, lu_bits=64, n_slots=0):
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        y_out = np.zeros((n, self.neq))
+        status = np.zeros(n, dtype=np.int32)
+        stats = np.zeros((n, 8), dtype=np.int32)
+        timers = np.zeros((n, 8))
+        _check(lib().ucg_solve_batch_profiled(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(y_out),
+                                              _p(status), _p(stats), _p(timers), ctypes.c_int(lu_bits),
+                                              ctypes.c_int(n_slots)), "ucg_solve_batch_profiled")
+        return dict(y=y_out, status=status, stats=stats, timers=timers)
+
+    def solve_device(self, params, y_out, status, stats, y_init=None, rec=None, n_rec=None, max_rec=0,
+                     t_start=None, t_done=None, lu_bits=64, n_slots=0, stream=None):
+        """Device-resident solve; arguments are torch CUDA tensors (or None)."""
+        def ptr(t):
+            return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        n = params.shape[0]
+        rc = lib().ucg_solve_batch_device(ctypes.c_int(self.handle), ctypes.c_int(n), ptr(params), ptr(y_init),
+                                          ptr(y_out), ctypes.c_int(max_rec), ptr(rec), ptr(n_rec), ptr(status),
+                                          ptr(stats), ptr(t_start), ptr(t_done), ctypes.c_int(lu_bits),
+                                          ctypes.c_int(n_slots), ctypes.c_void_p(stream or 0))
+        _check(rc, "ucg_solve_batch_device")
+
+    # ------------------------------------------------------------ pieces
+    def rates(self, params, phys, y):
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        rate = np.zeros((n, self.nreac))
+        y_out = np.zeros((n, self.neq))
+        scal = np.zeros((n, 8))
+        _check(lib().ucg_rates(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(y),
+                               _p(rate), _p(y_out), _p(scal)), "ucg_rates")
+        return rate, y_out, scal
+
+    def rhs(self, params, phys, rate, y):
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
+        rate = np.ascontiguousarray(np.atleast_2d(rate), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        ydot = np.zeros((n, self.neq))
+        rate_out = np.zeros((n, self.nreac))
+        _check(lib().ucg_rhs(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(rate), _p(y),
+                             _p(ydot), _p(rate_out)), "ucg_rhs")
+        return ydot, rate_out
+
+    def jacobian(self, params, phys, rate, y):
+        """Returns J[n, neq, neq] with J[p, i, j] = d ydot_i / d y_j."""
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
+        rate = np.ascontiguousarray(np.atleast_2d(rate), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        jac = np.zeros((n, self.neq, self.neq))   # column-major per point: [p, j, i]
+        _check(lib().ucg_jacobian(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(rate),
+                                  _p(y), _p(jac)), "ucg_jacobian")
+        return np.ascontiguousarray(np.transpose(jac, (0, 2, 1)))
+
+    def newton_solve(self, params, phys, y, gamma, b, lu_bits=64):
+        """x = (I - gamma J(y))^-1 b with the library's LU (rates evaluated at y)."""
+        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        n = params.shape[0]
+        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float64)
+        x = np.zeros((n, self.neq))
+        _check(lib().ucg_newton_solve(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(y),
+                                      ctypes.c_double(gamma), _p(b), _p(x), ctypes.c_int(lu_bits)),
+               "ucg_newton_solve")
+        return x
+
+
+def load_network(name: str = "default", device: int = 0) -> Network:
+    return Network(mh.load_network(mh.network_path(name)), device=device)

@@ -58,6 +58,8 @@ OP_BLR = -2        # * bulkLayersReciprocal
 OP_SWAP = -3       # * totalSwap / safeMantle
 OP_DIVSM = -4      # / safeMantle
 MAX_OPS = 6
+FAC_D = -5         # Jacobian factor code for one power of the density
+MAX_FAC = 10
 
 # special species looked up by name (network.py:1238-1276); missing ones map to the
 # density slot nspec (0-based), exactly like MakeRates' dummy index nspec+1.
@@ -243,6 +245,9 @@ class NetworkTables:
 def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
                  name: str = "network") -> NetworkTables:
     names = [s.name for s in species]
+    for r in reactions:
+        if r.rtype in UNSUPPORTED_TYPES:
+            raise ValueError(f"reaction type {r.rtype} is not supported by the device rate evaluator")
     nspec = len(names)
     neq = nspec + 1
     nreac = len(reactions)
@@ -430,7 +435,92 @@ def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
     # refractory bulk species are skipped in the transfer branch (io_functions.py:624-629)
     refractory = np.zeros(nspec, dtype=np.int8)
 
+    # --- analytic-Jacobian structure
+    # factor list of every term: FAC_D per density power, then the ops in emission order
+    fac = np.full((nreac, MAX_FAC), OP_NONE, dtype=np.int32)
+    nfac = np.zeros(nreac, dtype=np.int32)
+    for k in range(nreac):
+        f = [FAC_D] * int(n_d[k]) + [int(o) for o in ops[k] if o != OP_NONE]
+        if len(f) > MAX_FAC:
+            raise ValueError("term has too many factors")
+        fac[k, :len(f)] = f
+        nfac[k] = len(f)
+    # species affected by each reaction with stoichiometric sign (the ER rule moves the
+    # gas reactant's loss to its surface twin, io_functions.py:566-574)
+    aff: List[Dict[int, int]] = [dict() for _ in range(nreac)]
+    for k, r in enumerate(reactions):
+        for sname in r.reactants:
+            if sname in idx:
+                tgt = idx["#" + sname] if (r.rtype == "ER" and not sname.startswith("#")) else idx[sname]
+                aff[k][tgt] = aff[k].get(tgt, 0) - 1
+        for sname in r.products:
+            if sname in idx:
+                aff[k][idx[sname]] = aff[k].get(idx[sname], 0) + 1
+    aff_ptr = np.zeros(nreac + 1, dtype=np.int32)
+    aff_sp, aff_cf = [], []
+    for k in range(nreac):
+        items = [(i, c) for i, c in sorted(aff[k].items()) if c != 0]
+        aff_ptr[k + 1] = aff_ptr[k] + len(items)
+        aff_sp += [i for i, _ in items]
+        aff_cf += [c for _, c in items]
+
+    def fac_var(code):
+        if code >= 0:
+            return code
+        return {FAC_D: neq - 1, OP_BLR: nbulk_i, OP_DIVSM: nsurf_i, OP_SWAP: nsurf_i}[code]
+
+    nsurf_i = idx.get("SURFACE", dummy)
+    nbulk_i = idx.get("BULK", dummy)
+    contrib: Dict[tuple, Dict[tuple, int]] = {}
+    for k in range(nreac):
+        for m in range(int(nfac[k])):
+            v = fac_var(int(fac[k, m]))
+            for i, c in aff[k].items():
+                if c == 0:
+                    continue
+                d = contrib.setdefault((v, i), {})
+                d[(k, m)] = d.get((k, m), 0) + c
+    keys = sorted(contrib.keys())                  # column-major order (col v, then row i)
+    jnz_row = np.array([i for v, i in keys], dtype=np.int32)
+    jnz_col = np.array([v for v, i in keys], dtype=np.int32)
+    jnz_ptr = np.zeros(len(keys) + 1, dtype=np.int32)
+    jc_k, jc_m, jc_c = [], [], []
+    for n_, key in enumerate(keys):
+        items = sorted((km, c) for km, c in contrib[key].items() if c != 0)
+        jnz_ptr[n_ + 1] = jnz_ptr[n_] + len(items)
+        jc_k += [km[0] for km, _ in items]
+        jc_m += [km[1] for km, _ in items]
+        jc_c += [c for _, c in items]
+    # SURFSWAP terms depend on every bulk species through totalSwap: per-species
+    # coefficient lists for the rank-1 correction u_i = sum_k c_ik * w_k
+    sw_lists: List[List[tuple]] = [[] for _ in range(neq)]
+    for k in range(nreac):
+        if any(int(o) == OP_SWAP for o in ops[k]):
+            for i, c in aff[k].items():
+                if c != 0:
+                    sw_lists[i].append((k, c))
+    sw_ptr = np.zeros(neq + 1, dtype=np.int32)
+    for i in range(neq):
+        sw_ptr[i + 1] = sw_ptr[i] + len(sw_lists[i])
+    sw_k = np.array([k for l in sw_lists for k, _ in l], dtype=np.int32)
+    sw_c = np.array([c for l in sw_lists for _, c in l], dtype=np.int32)
+    swap_bulk = np.array([int(ops[k][0]) for k in swap_list], dtype=np.int32)  # bulk reactant of each BULKSWAP
+
+    # --- static per-reaction constants for the device rate evaluator (network-only
+    # parts of rates.f90 / surfacereactions.f90; temperature-dependent parts stay on device)
+    rstat, ice_kind = _static_rate_constants(names, species, reactions, re, pr, rtype, alpha, gama,
+                                             redmass, fortran_real(mass_raw), ice, gas_of_ice,
+                                             fortran_real([species[i].binding_energy for i in ice]),
+                                             fortran_real([species[i].enthalpy for i in ice]),
+                                             n_atoms, bulk, idx, dummy)
+
     arrays = dict(
+        rstat=rstat, ice_kind=ice_kind,
+        fac=fac, nfac=nfac, aff_ptr=aff_ptr, aff_sp=np.array(aff_sp, dtype=np.int32),
+        aff_cf=np.array(aff_cf, dtype=np.int32), jnz_row=jnz_row, jnz_col=jnz_col, jnz_ptr=jnz_ptr,
+        jc_k=np.array(jc_k, dtype=np.int32), jc_m=np.array(jc_m, dtype=np.int32),
+        jc_c=np.array(jc_c, dtype=np.int32), sw_ptr=sw_ptr, sw_k=sw_k, sw_c=sw_c,
+        swap_bulk=swap_bulk,
         mass=fortran_real(mass_raw), mass_raw=mass_raw, n_atoms=n_atoms, elem_counts=elem_counts,
         charge=charge,
         surface_list=np.array(surface, dtype=np.int32), bulk_list=np.array(bulk, dtype=np.int32),
@@ -454,6 +544,120 @@ def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
     for k, v in spec_idx.items():
         scalars[k] = int(v)
     return NetworkTables(name, names, nspec, neq, nreac, arrays, scalars)
+
+
+# ---------------------------------------------------------------- static rate constants
+
+K_BOLTZ = 1.38065040e-16
+AMU = 1.66053892e-24
+REDUCED_PLANCK = 1.054571628e-27
+PI_F = float(np.float32(3.141592654))           # constants.f90:9 is a default-real literal
+SURFACE_SITE_DENSITY = 1.5e15
+CHEMICAL_BARRIER_THICKNESS = 1.40e-8
+VDIFF_PREFACTOR = 2.0 * K_BOLTZ * SURFACE_SITE_DENSITY / PI_F / PI_F / AMU
+
+
+def _static_rate_constants(names, species, reactions, re, pr, rtype, alpha, gama, redmass, mass,
+                           ice, gas_of_ice, be, fe, n_atoms, bulk, idx, dummy):
+    """Per-reaction constants that depend only on the network.
+
+    rstat[k] holds, by type:
+      THERM     [vdiff(re1), valid]                     (rates.f90:205-214)
+      BULKSWAP  [vdiff(re1), E_bind(re1), valid]        (surfacereactions.f90:131-150)
+      LH        [vdiff1, vdiff2, E1, E2, tunnel]        (surfacereactions.f90:158-211)
+      LHDES/ERDES [desorption fraction, re1 is bulk]    (surfacereactions.f90:218-297)
+    ice_kind[k]: 1 surface, 2 bulk first reactant (PHOTON/CRPHOT ice factors, rates.f90:47-56)."""
+    import math
+    nreac = len(reactions)
+    nice = len(ice)
+    ice_arr = np.array(ice)
+    gas_arr = np.array(gas_of_ice)
+    vdiff = np.array([math.sqrt(VDIFF_PREFACTOR * be[p] / mass[ice[p]]) for p in range(nice)])
+    bulk_set = set(bulk)
+    surf_set = set(i for i, n in enumerate(names) if n.startswith("#"))
+
+    def ice_pos(sp):
+        hits = np.where(ice_arr == sp)[0]
+        return int(hits[-1]) if len(hits) else -1
+
+    def desorption_fraction(k):
+        r1, r2 = int(re[k, 0]), int(re[k, 1])
+        ri1 = ri2 = 0
+        pidx = [-1, -1, -1, -1]
+        for i in range(nice):
+            if ice_arr[i] == r1: ri1 = i
+            if gas_arr[i] == r1: ri1 = i
+            if ice_arr[i] == r2: ri2 = i
+            if gas_arr[i] == r2: ri2 = i
+            for q in range(4):
+                pq = int(pr[k, q])
+                if pq < 0:
+                    continue
+                if ice_arr[i] == pq: pidx[q] = i
+                if gas_arr[i] == pq: pidx[q] = i
+        maxbe = prodh = eps = 0.0
+        for q in range(4):
+            if pidx[q] >= 0:
+                maxbe = max(maxbe, be[pidx[q]])
+                prodh = prodh + fe[pidx[q]]
+                eps = eps + mass[pidx[q]]        # indexed by ice position, as the reference does
+        eps = ((eps - 120.0) / (eps + 120.0)) ** 2
+        dh = fe[ri1] + fe[ri2] - prodh
+        dh = dh * 4.184e03 / (1.38054e-23 * 6.02214129e23)
+        dh = dh + gama[k]
+        if dh == 0.0:
+            dh = float(np.float32(1e-30))
+        dof = int(n_atoms[pidx[0]]) if pidx[0] >= 0 else 0
+        for q in range(1, 4):
+            if pidx[q] >= 0:
+                dof = max(dof, int(n_atoms[pidx[q]]))
+        dof = 3 * dof
+        f = math.exp((-maxbe * float(np.float32(dof))) / (eps * dh))
+        if dh < 0.0:
+            f = 0.0
+        f = f / 10
+        ngn, ngo, ngoh, nh = idx.get("#N", dummy), idx.get("#O", dummy), idx.get("#OH", dummy), idx.get("H", dummy)
+        if r1 == ngn and r2 == ngn:
+            f = 0.5
+        if (r1 == ngo and r2 == nh) or (r1 == nh and r2 == ngo):
+            f = float(np.float32(0.3))
+        if (r1 == ngoh and r2 == nh) or (r1 == nh and r2 == ngoh):
+            f = 0.25
+        return f
+
+    rstat = np.zeros((nreac, 5))
+    ice_kind = np.zeros(nreac, dtype=np.int32)
+    for k in range(nreac):
+        t = ALL_TYPES[int(rtype[k])]
+        r1 = int(re[k, 0])
+        if t in ("PHOTON", "CRPHOT"):
+            ice_kind[k] = 2 if r1 in bulk_set else (1 if r1 in surf_set else 0)
+        elif t == "THERM":
+            p = ice_pos(r1)
+            if p >= 0:
+                rstat[k, 0], rstat[k, 1] = vdiff[p], 1.0
+        elif t == "BULKSWAP":
+            p = ice_pos(r1)
+            if p >= 0:
+                rstat[k, 0], rstat[k, 1], rstat[k, 2] = vdiff[p], be[p], 1.0
+        elif t == "LH":
+            i1, i2 = int(re[k, 0]), int(re[k, 1])
+            p1, p2 = ice_pos(i1), ice_pos(i2)
+            i1 = p1 if p1 >= 0 else i1
+            i2 = p2 if p2 >= 0 else i2
+            rm = redmass[k]
+            if rm == 0.0:
+                m1, m2 = mass[ice[i1]], mass[ice[i2]]
+                rm = m1 * m2 / (m1 + m2)
+            tunnel = 2.0 * CHEMICAL_BARRIER_THICKNESS / REDUCED_PLANCK * math.sqrt(2.0 * AMU * rm * K_BOLTZ * gama[k])
+            rstat[k] = (vdiff[i1], vdiff[i2], be[i1], be[i2], tunnel)
+        elif t in ("LHDES", "ERDES"):
+            rstat[k, 0] = desorption_fraction(k)
+            rstat[k, 1] = 1.0 if r1 in bulk_set else 0.0
+    return rstat, ice_kind
+
+
+UNSUPPORTED_TYPES = ("IONOPOL1", "IONOPOL2", "CRS", "EXSOLID", "EXRELAX", "GAR", "DESORB")
 
 
 # ---------------------------------------------------------------- persistence

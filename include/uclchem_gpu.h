@@ -88,6 +88,10 @@ int ucg_init(int device);
 const char *ucg_last_error(void);
 
 int ucg_network_load(const ucg_network_desc *desc, int *handle);
+/* Same from the flat descriptor file MakeRates' HIP backend writes beside the network
+ * (uclchem_amd/networks/<name>.ucgnet), for callers without Python: the Fortran core's
+ * counterpart of USE network (src/fortran_src/network.f90:1-5169). */
+int ucg_network_load_file(const char *path, int *handle);
 int ucg_network_free(int handle);
 
 /* Full single-point models for a batch of independent grid points.

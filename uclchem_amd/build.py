@@ -1,4 +1,5 @@
-"""Build libuclchem_gpu.so (HIP, gfx950) in-tree with hipcc."""
+"""Build libuclchem_gpu.so (HIP, gfx950) in-tree with hipcc, the network files the C ABI
+loads directly, and the Fortran binding with its example caller."""
 from __future__ import annotations
 
 import os
@@ -19,6 +20,45 @@ FLAGS = [
 ]
 
 
+FC = os.environ.get("UCG_FC", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FORTRAN_EXAMPLE = os.path.join(FDIR, "cloud_grid")
+NET_DIR = os.path.join(HERE, "networks")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """The ISO_C_BINDING module and the example caller (needs a Fortran compiler)."""
+    if not os.path.exists(FC):
+        if verbose:
+            print(f"{FC} not found: Fortran binding not built", flush=True)
+        return None
+    bdir = os.path.join(FDIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    steps = [
+        [FC, "-O2", "-J", bdir, "-c", os.path.join(FDIR, "uclchem_gpu.f90"), "-o", os.path.join(bdir, "uclchem_gpu.o")],
+        [FC, "-O2", "-J", bdir, "-c", os.path.join(FDIR, "cloud_grid.f90"), "-o", os.path.join(bdir, "cloud_grid.o")],
+        [FC, "-o", FORTRAN_EXAMPLE + ".tmp", os.path.join(bdir, "cloud_grid.o"), os.path.join(bdir, "uclchem_gpu.o"),
+         "-L" + HERE, "-luclchem_gpu", "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+    os.replace(FORTRAN_EXAMPLE + ".tmp", FORTRAN_EXAMPLE)
+    return FORTRAN_EXAMPLE
+
+
+def write_network_files(verbose: bool = False) -> None:
+    """<name>.ucgnet beside every <name>.npz (the file the C ABI loads directly)."""
+    from . import makerates_hip as mh
+    for f in sorted(os.listdir(NET_DIR)):
+        if f.endswith(".npz"):
+            out = os.path.join(NET_DIR, f[:-4] + ".ucgnet")
+            mh.save_network_bin(mh.load_network(os.path.join(NET_DIR, f)), out)
+            if verbose:
+                print(f"wrote {out}", flush=True)
+
+
 def build(verbose: bool = False, extra=None) -> str:
     src = os.path.join(CSRC, "ucg_api.hip")
     tmp = LIB + ".tmp"
@@ -27,6 +67,8 @@ def build(verbose: bool = False, extra=None) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    write_network_files(verbose)
+    build_fortran(verbose)
     return LIB
 
 

@@ -4,6 +4,7 @@
 #include "../../include/uclchem_gpu.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -202,6 +203,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.t_start = t_start;
     A.t_done = t_done;
     A.timers = timers;
+    A.equilibrate = getenv("UCG_EQUILIBRATE") ? atoi(getenv("UCG_EQUILIBRATE")) : 0;
     A.ws = h.ws;
     A.ws_stride = stride;
     A.counter = h.counter;
@@ -251,6 +253,8 @@ int ucg_init(int device)
     return init_locked(device);
 }
 
+static int load_tables(const ucg_network_desc *d, NetHandle &h);
+
 int ucg_network_load(const ucg_network_desc *d, int *handle)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -261,6 +265,120 @@ int ucg_network_load(const ucg_network_desc *d, int *handle)
         return fail(UCG_E_ARG, "network too large: NEQ must be <= " + std::to_string(ucg::NT * ucg::EPT));
     NetHandle h;
     h.used = true;
+    if ((rc = load_tables(d, h))) {   // release whatever was uploaded before the failure
+        for (void *p : h.allocs) (void)hipFree(p);
+        return rc;
+    }
+    g_nets.push_back(h);
+    *handle = (int)g_nets.size() - 1;
+    return 0;
+}
+
+// Reads the flat descriptor file written by makerates_hip.save_network_bin.
+int ucg_network_load_file(const char *path, int *handle)
+{
+    if (!path || !handle) return fail(UCG_E_ARG, "null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(UCG_E_ARG, std::string("cannot open network file ") + path);
+    std::vector<char> buf;
+    {
+        char tmp[1 << 16];
+        size_t r;
+        while ((r = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + r);
+        fclose(f);
+    }
+    if (buf.size() < 12 || memcmp(buf.data(), "UCGNET01", 8) != 0)
+        return fail(UCG_E_ARG, std::string("not a UCGNET01 network file: ") + path);
+    int32_t nrec;
+    memcpy(&nrec, buf.data() + 8, 4);
+    size_t off = 12;
+    struct Rec { int dtype; int64_t count; const char *data; };
+    std::vector<std::pair<std::string, Rec>> recs;
+    for (int r = 0; r < nrec; r++) {
+        if (off + 36 > buf.size()) return fail(UCG_E_ARG, "truncated network file");
+        char name[25] = {0};
+        memcpy(name, buf.data() + off, 24);
+        Rec rc;
+        memcpy(&rc.dtype, buf.data() + off + 24, 4);
+        memcpy(&rc.count, buf.data() + off + 28, 8);
+        off += 36;
+        const size_t esz = rc.dtype == 0 ? 8 : rc.dtype == 1 ? 4 : rc.dtype == 2 ? 1 : 0;
+        if (!esz || rc.count < 0 || off + esz * (size_t)rc.count > buf.size())
+            return fail(UCG_E_ARG, std::string("bad record ") + name + " in network file");
+        rc.data = buf.data() + off;
+        off += esz * (size_t)rc.count;
+        off += (8 - (esz * (size_t)rc.count) % 8) % 8;
+        recs.push_back({name, rc});
+    }
+    auto find = [&](const char *name, int dtype, int64_t need, const void **out) -> int {
+        for (auto &kv : recs)
+            if (kv.first == name) {
+                if (kv.second.dtype != dtype || (need >= 0 && kv.second.count != need))
+                    return fail(UCG_E_ARG, std::string("network file record ") + name + " has the wrong type or size");
+                *out = kv.second.data;
+                return 0;
+            }
+        return fail(UCG_E_ARG, std::string("network file lacks record ") + name);
+    };
+    ucg_network_desc d;
+    memset(&d, 0, sizeof d);
+    const void *p;
+    int rc;
+    if ((rc = find("sizes", 1, 14, &p))) return rc;
+    int32_t sz[14];
+    memcpy(sz, p, sizeof sz);
+    d.nspec = sz[0]; d.neq = sz[1]; d.nreac = sz[2]; d.nice = sz[3]; d.nsurfl = sz[4]; d.nbulkl = sz[5];
+    d.nfp = sz[6]; d.nswap = sz[7]; d.nloss = sz[8]; d.nprod = sz[9]; d.nnz = sz[10]; d.ncontrib = sz[11];
+    d.naff = sz[12]; d.nsw = sz[13];
+    // copies keep the int32/double arrays aligned whatever the file offsets are
+    std::vector<std::vector<double>> keep_d;
+    std::vector<std::vector<int>> keep_i;
+    std::vector<std::vector<signed char>> keep_c;
+    auto getd = [&](const char *nm, int64_t n, const double **dst) -> int {
+        int r = find(nm, 0, n, &p);
+        if (r) return r;
+        keep_d.emplace_back((size_t)n);
+        memcpy(keep_d.back().data(), p, 8 * (size_t)n);
+        *dst = keep_d.back().data();
+        return 0;
+    };
+    auto geti = [&](const char *nm, int64_t n, const int **dst) -> int {
+        int r = find(nm, 1, n, &p);
+        if (r) return r;
+        keep_i.emplace_back((size_t)n);
+        memcpy(keep_i.back().data(), p, 4 * (size_t)n);
+        *dst = keep_i.back().data();
+        return 0;
+    };
+    const int64_t R = d.nreac, Q = d.neq, S = d.nspec;
+    keep_d.reserve(8);
+    keep_i.reserve(40);
+#define GD(f, n) if ((rc = getd(#f, (n), &d.f))) return rc
+#define GI(f, n) if ((rc = geti(#f, (n), &d.f))) return rc
+    GD(alpha, R); GD(beta, R); GD(gama, R); GD(tmin, R); GD(tmax, R); GD(mass, S); GD(rstat, 5 * R);
+    GI(rtype, R); GI(re, 3 * R); GI(pr, 4 * R); GI(ice_kind, R); GI(n_d, R); GI(ops, ucg::MAXOPS * R);
+    GI(fac, ucg::MAXFAC * R); GI(nfac, R); GI(type_ranges, 2 * ucg::NTYPES);
+    GI(loss_ptr, S + 1); GI(loss_idx, d.nloss); GI(prod_ptr, S + 1); GI(prod_idx, d.nprod);
+    GI(swap_list, d.nswap); GI(surface_list, d.nsurfl); GI(bulk_list, d.nbulkl);
+    GI(bulk_partner_of_surface, Q); GI(surface_partner_of_bulk, Q); GI(freeze_partners, d.nfp);
+    GI(is_surface, Q); GI(is_bulk, Q); GI(jnz_row, d.nnz); GI(jnz_col, d.nnz); GI(jnz_ptr, d.nnz + 1);
+    GI(jc_k, d.ncontrib); GI(jc_m, d.ncontrib); GI(jc_c, d.ncontrib); GI(aff_ptr, R + 1);
+    GI(aff_sp, d.naff); GI(aff_cf, d.naff); GI(sw_ptr, Q + 1); GI(sw_k, d.nsw); GI(sw_c, d.nsw);
+#undef GD
+#undef GI
+    if ((rc = find("extrapolate", 2, R, &p))) return rc;
+    keep_c.emplace_back((const signed char *)p, (const signed char *)p + R);
+    d.extrapolate = keep_c.back().data();
+    if ((rc = find("special_species", 1, 25, &p))) return rc;
+    memcpy(d.special_species, p, sizeof d.special_species);
+    if ((rc = find("special_reactions", 1, 9, &p))) return rc;
+    memcpy(d.special_reactions, p, sizeof d.special_reactions);
+    return ucg_network_load(&d, handle);
+}
+
+static int load_tables(const ucg_network_desc *d, NetHandle &h)
+{
+    int rc;
     ucg::DevNet &N = h.dev;
     N.nspec = d->nspec; N.neq = d->neq; N.nreac = d->nreac; N.nice = d->nice; N.nsurfl = d->nsurfl;
     N.nbulkl = d->nbulkl; N.nfp = d->nfp; N.nswap = d->nswap; N.nnz = d->nnz;
@@ -270,6 +388,15 @@ int ucg_network_load(const ucg_network_desc *d, int *handle)
     UP(alpha, R); UP(beta, R); UP(gama, R); UP(tmin, R); UP(tmax, R); UP(mass, (size_t)d->nspec);
     UP(rstat, 5 * R); UP(rtype, R); UP(re, 3 * R); UP(pr, 4 * R); UP(ice_kind, R); UP(n_d, R);
     UP(ops, ucg::MAXOPS * R); UP(fac, ucg::MAXFAC * R); UP(nfac, R); UP(extrapolate, R);
+    {   // packed term records {n_d, ops[6], nfac}
+        std::vector<int> prog(8 * R);
+        for (size_t k = 0; k < R; k++) {
+            prog[8 * k] = d->n_d[k];
+            for (int q = 0; q < ucg::MAXOPS; q++) prog[8 * k + 1 + q] = d->ops[ucg::MAXOPS * k + q];
+            prog[8 * k + 7] = d->nfac[k];
+        }
+        if ((rc = upload(h, prog.data(), prog.size(), N.prog))) return rc;
+    }
     UP(type_ranges, (size_t)2 * ucg::NTYPES);
     UP(loss_ptr, (size_t)d->nspec + 1); UP(loss_idx, (size_t)d->nloss);
     UP(prod_ptr, (size_t)d->nspec + 1); UP(prod_idx, (size_t)d->nprod); UP(swap_list, (size_t)d->nswap);
@@ -310,8 +437,6 @@ int ucg_network_load(const ucg_network_desc *d, int *handle)
     if (lds > (size_t)prop.sharedMemPerBlock)
         return fail(UCG_E_ARG, "network needs " + std::to_string(lds) + " B of LDS per workgroup, device allows " +
                                    std::to_string(prop.sharedMemPerBlock));
-    g_nets.push_back(h);
-    *handle = (int)g_nets.size() - 1;
     return 0;
 }
 

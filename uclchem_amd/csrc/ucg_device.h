@@ -50,6 +50,7 @@ struct DevNet {
     int32_t nspec, neq, nreac, nice, nsurfl, nbulkl, nfp, nswap, nnz, ld;
     const double *alpha, *beta, *gama, *tmin, *tmax, *mass, *rstat;  // rstat [nreac*5]
     const int32_t *rtype, *re, *pr, *ice_kind, *n_d, *ops, *fac, *nfac;
+    const int32_t *prog;   // [nreac*8] {n_d, ops[6], nfac}, 32-byte aligned records
     const int8_t *extrapolate;
     const int32_t *type_ranges;                                     // [NTYPES*2]
     const int32_t *loss_ptr, *loss_idx, *prod_ptr, *prod_idx, *swap_list;

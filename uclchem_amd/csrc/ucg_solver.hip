@@ -267,14 +267,16 @@ __device__ void block_rates(const DevNet &N, Lds &L)
     __syncthreads();
 }
 
-// value of one reaction's ODE term, tokens in the order MakeRates emits them
+// value of one reaction's ODE term, tokens in the order MakeRates emits them.
+// prog[k] packs {n_d, op0..op5, nfac} into two 16-byte words.
 __device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const double *y,
                                              double D, double blr, double TS, double sm)
 {
+    const int4 *pk = reinterpret_cast<const int4 *>(N.prog) + 2 * k;
+    const int4 w0 = pk[0], w1 = pk[1];
+    const int op[MAXOPS] = {w0.y, w0.z, w0.w, w1.x, w1.y, w1.z};
     double v = rate[k];
-    const int nd = N.n_d[k];
-    for (int q = 0; q < nd; q++) v = v * D;
-    const int *op = N.ops + MAXOPS * k;
+    for (int q = 0; q < w0.x; q++) v = v * D;
 #pragma unroll
     for (int q = 0; q < MAXOPS; q++) {
         const int o = op[q];
@@ -324,7 +326,9 @@ __device__ void block_rhs(const DevNet &N, Lds &L)
             double loss = 0.0, prod = 0.0;
             const int l0 = N.loss_ptr[i], l1 = N.loss_ptr[i + 1];
             const int p0 = N.prod_ptr[i], p1 = N.prod_ptr[i + 1];
+#pragma unroll 4
             for (int q = l0; q < l1; q++) loss += flux[N.loss_idx[q]];
+#pragma unroll 4
             for (int q = p0; q < p1; q++) prod += flux[N.prod_idx[q]];
             pre[e] = prod - loss;
             if (N.is_surface[i]) spre += pre[e];
@@ -395,7 +399,8 @@ __device__ __forceinline__ double fac_deriv(int c, double dblr, double dism, dou
 // BULK/SURFACE totals).  With scale != 0 the matrix is equilibrated by the error weights:
 // M' = W M W^-1, W = diag(ewt) (held in L.g).
 template <typename LUT>
-__device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma)
+__device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma, bool convert = true,
+                                    bool scaled = false)
 {
     const int neq = N.neq, ld = N.ld;
     double *y = L.y, *rate = L.rate, *sc = L.sc, *g = L.g;
@@ -469,15 +474,27 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
     __syncthreads();
     // 3b. photo-rate dependence on y(H2), y(CO) and density through the column densities,
     //     and the free-fall density row
-    if (tid() == 0) {
+    {   // the six photo-rate evaluations run in parallel lanes of wave 0
         const double cs = sc[S_CS], G0 = L.prm[P_RADFIELD], av = sc[S_AV];
         const double h2col = 0.5 * y[N.nh2] * D * cs, cocol = 0.5 * y[N.nco] * D * cs;
         const double eps = 1e-6;
-        const double kh = h2_photodiss_dev(h2col, G0, sc[S_SCAT]);
-        const double dkh = h2col > 0 ? (h2_photodiss_dev(h2col * (1 + eps), G0, sc[S_SCAT]) - kh) / (h2col * eps) : 0.0;
-        const double kc = co_photodiss_dev(h2col, cocol, G0, av);
-        const double dkch = h2col > 0 ? (co_photodiss_dev(h2col * (1 + eps), cocol, G0, av) - kc) / (h2col * eps) : 0.0;
-        const double dkcc = cocol > 0 ? (co_photodiss_dev(h2col, cocol * (1 + eps), G0, av) - kc) / (cocol * eps) : 0.0;
+        const int t = tid();
+        if (t < 6) {
+            const double h = (t == 1 || t == 4) ? h2col * (1 + eps) : h2col;
+            const double c = (t == 5) ? cocol * (1 + eps) : cocol;
+            scr[t] = (t < 2) ? h2_photodiss_dev(h, G0, sc[S_SCAT]) : (t >= 3 ? co_photodiss_dev(h, c, G0, av) : 0.0);
+        }
+    }
+    __syncthreads();
+    if (tid() == 0) {
+        const double cs = sc[S_CS];
+        const double h2col = 0.5 * y[N.nh2] * D * cs, cocol = 0.5 * y[N.nco] * D * cs;
+        const double eps = 1e-6;
+        const double kh = scr[0];
+        const double dkh = h2col > 0 ? (scr[1] - kh) / (h2col * eps) : 0.0;
+        const double kc = scr[3];
+        const double dkch = h2col > 0 ? (scr[4] - kc) / (h2col * eps) : 0.0;
+        const double dkcc = cocol > 0 ? (scr[5] - kc) / (cocol * eps) : 0.0;
         for (int which = 0; which < 2; which++) {
             const int k = which ? N.r_co_hv : N.r_h2_hv;
             const int *fc = N.fac + MAXFAC * k;
@@ -503,6 +520,7 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
     const double S = sc[S_SPRE];
     for (int j = tid(); j < neq; j += NT) {
         double gsum = 0.0;
+#pragma unroll 8
         for (int q = 0; q < N.nsurfl; q++) gsum += (double)M[(size_t)j * ld + N.surface_list[q]];
         scr[j] = gsum;
     }
@@ -531,18 +549,30 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
     // 5. BULK / SURFACE rows are the sums of their member rows
     for (int j = tid(); j < neq; j += NT) {
         double sb = 0.0, ss = 0.0;
+#pragma unroll 8
         for (int q = 0; q < N.nbulkl; q++) sb += (double)M[(size_t)j * ld + N.bulk_list[q]];
+#pragma unroll 8
         for (int q = 0; q < N.nsurfl; q++) ss += (double)M[(size_t)j * ld + N.surface_list[q]];
         M[(size_t)j * ld + N.nbulk] = (LUT)sb;
         M[(size_t)j * ld + N.nsurface] = (LUT)ss;
     }
     __syncthreads();
-    // 6. M = I - gamma W J W^-1
-    for (int idx = tid(); idx < neq * neq; idx += NT) {
-        const int j = idx / neq, i = idx % neq;
-        double v = -gamma * (double)M[(size_t)j * ld + i] * (g[i] / g[j]);
-        if (i == j) v += 1.0;
-        M[(size_t)j * ld + i] = (LUT)v;
+    if (!convert) return;
+    // 6. M = I - gamma W J W^-1 (W = I unless the caller equilibrates)
+    if (!scaled) {
+        for (int idx = tid(); idx < neq * neq; idx += NT) {
+            const int j = idx / neq, i = idx - j * neq;
+            double v = -gamma * (double)M[(size_t)j * ld + i];
+            if (i == j) v += 1.0;
+            M[(size_t)j * ld + i] = (LUT)v;
+        }
+    } else {
+        for (int idx = tid(); idx < neq * neq; idx += NT) {
+            const int j = idx / neq, i = idx - j * neq;
+            double v = -gamma * (double)M[(size_t)j * ld + i] * (g[i] / g[j]);
+            if (i == j) v += 1.0;
+            M[(size_t)j * ld + i] = (LUT)v;
+        }
     }
     __syncthreads();
 }
@@ -552,57 +582,128 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
 constexpr int LU_NB = 16;
 
 // Right-looking blocked LU of the column-major n x n matrix M (unit lower L, upper U
-// stored in place).  Returns nonzero (uniformly) on a zero or non-finite pivot.
+// stored in place).  Per block column: the 16x16 diagonal block is factored by wave 0 in
+// registers (lane r holds row r, pivot rows exchanged with wave shuffles), then
+// L21 = A21 U11^-1 (one row per thread), U12 = L11^-1 A12 (one column per thread) and
+// the trailing update A22 -= L21 U12 (threads own rows, four columns per pass).
+// `stage` is LDS scratch of >= 2*16*16 doubles.  Returns nonzero (uniformly) on a zero
+// or non-finite pivot.
 template <typename LUT>
-__device__ int block_lu(LUT *M, int n, int ld, double *red)
+__device__ int block_lu(LUT *M, int n, int ld, double *red, double *stage)
 {
+    double *U11 = stage;              // U11[p*16+q] = U(kb+p, kb+q)
+    double *L11 = stage + LU_NB * LU_NB;
     for (int kb = 0; kb < n; kb += LU_NB) {
         const int nb = min(LU_NB, n - kb);
-        // panel: columns kb..kb+nb-1
-        for (int j = kb; j < kb + nb; j++) {
-            __syncthreads();
-            const LUT piv = M[(size_t)j * ld + j];
-            if (!(piv == piv) || piv == (LUT)0 || isinf((double)piv)) return 1;
-            const LUT inv = (LUT)1 / piv;
-            for (int i = j + 1 + tid(); i < n; i += NT) {
-                LUT l = M[(size_t)j * ld + i] * inv;
-                M[(size_t)j * ld + i] = l;
-                for (int c = j + 1; c < kb + nb; c++) M[(size_t)c * ld + i] -= l * M[(size_t)c * ld + j];
-            }
-        }
         __syncthreads();
-        const int c0 = kb + nb;
-        if (c0 >= n) break;
-        // U12 = L11^-1 A12 (one column per thread)
-        for (int c = c0 + tid(); c < n; c += NT) {
-            LUT a[LU_NB];
+        // 1. diagonal block in registers of wave 0
+        if (tid() < 64) {
+            const int r = tid();
+            double a[LU_NB];
 #pragma unroll
-            for (int r = 0; r < LU_NB; r++) a[r] = (r < nb) ? M[(size_t)c * ld + kb + r] : (LUT)0;
+            for (int q = 0; q < LU_NB; q++) a[q] = (r < nb && q < nb) ? (double)M[(size_t)(kb + q) * ld + kb + r] : 0.0;
+            double bad = 0.0;
 #pragma unroll
-            for (int r = 1; r < LU_NB; r++) {
-                if (r < nb) {
-                    LUT acc = a[r];
+            for (int k = 0; k < LU_NB; k++) {
+                if (k < nb) {
+                    const double piv = __shfl(a[k], k);
+                    if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = 1.0;
+                    const double l = (r > k && r < nb) ? a[k] / piv : 0.0;
+                    if (r > k && r < nb) a[k] = l;
 #pragma unroll
-                    for (int p = 0; p < r; p++) acc -= M[(size_t)(kb + p) * ld + kb + r] * a[p];
-                    a[r] = acc;
+                    for (int q = k + 1; q < LU_NB; q++) {
+                        const double uq = __shfl(a[q], k);
+                        if (r > k && r < nb) a[q] -= l * uq;
+                    }
                 }
             }
+            if (r < nb) {
 #pragma unroll
-            for (int r = 0; r < LU_NB; r++)
-                if (r < nb) M[(size_t)c * ld + kb + r] = a[r];
+                for (int q = 0; q < LU_NB; q++) {
+                    if (q < nb) {
+                        M[(size_t)(kb + q) * ld + kb + r] = (LUT)a[q];
+                        U11[r * LU_NB + q] = (q >= r) ? a[q] : 0.0;
+                        L11[r * LU_NB + q] = (q < r) ? a[q] : 0.0;
+                    }
+                }
+            }
+            if (r == 0) red[8] = bad;
         }
         __syncthreads();
-        // A22 -= L21 U12 (threads own rows; U12 columns are read uniformly)
+        if (red[8] != 0.0) return 1;
+        const int c0 = kb + nb;
+        if (c0 >= n) break;
+        // 2. L21 = A21 U11^-1 (rows) and U12 = L11^-1 A12 (columns)
+        for (int w = tid(); w < 2 * (n - c0); w += NT) {
+            double a[LU_NB];
+            if (w < n - c0) {
+                const int i = c0 + w;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++) a[p] = (p < nb) ? (double)M[(size_t)(kb + p) * ld + i] : 0.0;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++) {
+                    if (p < nb) {
+                        double acc = a[p];
+#pragma unroll
+                        for (int q = 0; q < p; q++) acc -= a[q] * U11[q * LU_NB + p];
+                        a[p] = acc / U11[p * LU_NB + p];
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++)
+                    if (p < nb) M[(size_t)(kb + p) * ld + i] = (LUT)a[p];
+            } else {
+                const int c = c0 + (w - (n - c0));
+#pragma unroll
+                for (int r = 0; r < LU_NB; r++) a[r] = (r < nb) ? (double)M[(size_t)c * ld + kb + r] : 0.0;
+#pragma unroll
+                for (int r = 1; r < LU_NB; r++) {
+                    if (r < nb) {
+                        double acc = a[r];
+#pragma unroll
+                        for (int p = 0; p < r; p++) acc -= L11[r * LU_NB + p] * a[p];
+                        a[r] = acc;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < LU_NB; r++)
+                    if (r < nb) M[(size_t)c * ld + kb + r] = (LUT)a[r];
+            }
+        }
+        __syncthreads();
+        // 3. A22 -= L21 U12: threads own rows, U12 columns read uniformly, 4 columns a pass
         for (int i = c0 + tid(); i < n; i += NT) {
             LUT l[LU_NB];
 #pragma unroll
             for (int p = 0; p < LU_NB; p++) l[p] = (p < nb) ? M[(size_t)(kb + p) * ld + i] : (LUT)0;
-            for (int c = c0; c < n; c++) {
-                const LUT *u = M + (size_t)c * ld + kb;
-                LUT acc = (LUT)0;
+            int c = c0;
+            for (; c + 4 <= n; c += 4) {
+                const LUT *u0 = M + (size_t)c * ld + kb;
+                const LUT *u1 = u0 + ld;
+                const LUT *u2 = u1 + ld;
+                const LUT *u3 = u2 + ld;
+                LUT x0 = 0, x1 = 0, x2 = 0, x3 = 0;
 #pragma unroll
-                for (int p = 0; p < LU_NB; p++) acc += l[p] * ((p < nb) ? u[p] : (LUT)0);
-                M[(size_t)c * ld + i] -= acc;
+                for (int p = 0; p < LU_NB; p++) {
+                    if (p < nb) {
+                        x0 += l[p] * u0[p];
+                        x1 += l[p] * u1[p];
+                        x2 += l[p] * u2[p];
+                        x3 += l[p] * u3[p];
+                    }
+                }
+                M[(size_t)c * ld + i] -= x0;
+                M[(size_t)(c + 1) * ld + i] -= x1;
+                M[(size_t)(c + 2) * ld + i] -= x2;
+                M[(size_t)(c + 3) * ld + i] -= x3;
+            }
+            for (; c < n; c++) {
+                const LUT *u = M + (size_t)c * ld + kb;
+                LUT x = 0;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++)
+                    if (p < nb) x += l[p] * u[p];
+                M[(size_t)c * ld + i] -= x;
             }
         }
     }
@@ -610,7 +711,7 @@ __device__ int block_lu(LUT *M, int n, int ld, double *red)
     return 0;
 }
 
-constexpr int SB = 64;   // triangular-solve block (one wave64 per diagonal block)
+constexpr int SB = 32;   // triangular-solve block: wave 0 holds the diagonal block rows
 
 // Solve (LU) x = b in place on the LDS vector b (FP64 accumulation).
 template <typename LUT>
@@ -621,16 +722,22 @@ __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
     for (int kb = 0; kb < n; kb += SB) {
         const int nb = min(SB, n - kb);
         if (wave == 0) {
+            double row[SB];
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                row[c] = (lane < nb && c < lane) ? (double)M[(size_t)(kb + c) * ld + kb + lane] : 0.0;
             double x = (lane < nb) ? b[kb + lane] : 0.0;
-            for (int c = 0; c < nb - 1; c++) {
-                double xc = __shfl(x, c);
-                if (lane > c && lane < nb) x -= (double)M[(size_t)(kb + c) * ld + kb + lane] * xc;
+#pragma unroll
+            for (int c = 0; c < SB - 1; c++) {
+                const double xc = __shfl(x, c);
+                if (c < lane) x -= row[c] * xc;
             }
             if (lane < nb) b[kb + lane] = x;
         }
         __syncthreads();
         for (int i = kb + nb + tid(); i < n; i += NT) {
             double acc = b[i];
+#pragma unroll 8
             for (int p = 0; p < nb; p++) acc -= (double)M[(size_t)(kb + p) * ld + i] * b[kb + p];
             b[i] = acc;
         }
@@ -642,17 +749,23 @@ __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
         const int kb = blk * SB;
         const int nb = min(SB, n - kb);
         if (wave == 0) {
+            double row[SB];
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                row[c] = (lane < nb && c < nb && c >= lane) ? (double)M[(size_t)(kb + c) * ld + kb + lane] : 1.0;
             double x = (lane < nb) ? b[kb + lane] : 0.0;
-            for (int c = nb - 1; c >= 0; c--) {
-                if (lane == c) x = x / (double)M[(size_t)(kb + c) * ld + kb + c];
-                double xc = __shfl(x, c);
-                if (lane < c) x -= (double)M[(size_t)(kb + c) * ld + kb + lane] * xc;
+#pragma unroll
+            for (int c = SB - 1; c >= 0; c--) {
+                if (c < nb && lane == c) x = x / row[c];
+                const double xc = __shfl(x, c);
+                if (c < nb && lane < c) x -= row[c] * xc;
             }
             if (lane < nb) b[kb + lane] = x;
         }
         __syncthreads();
         for (int i = tid(); i < kb; i += NT) {
             double acc = b[i];
+#pragma unroll 8
             for (int p = 0; p < nb; p++) acc -= (double)M[(size_t)(kb + p) * ld + i] * b[kb + p];
             b[i] = acc;
         }
@@ -670,6 +783,7 @@ struct GpuOps {
     int neq;
     double z[6][EPT], acor[EPT], ewt[EPT], atol[EPT], f0[EPT];
     uint64_t t_rhs = 0, t_jac = 0, t_lu = 0, t_sol = 0;   // device wall-clock ticks per phase
+    bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M
 
     __device__ GpuOps(const DevNet &n_, Lds &l_, LUT *m_) : N(n_), L(l_), M(m_), neq(n_.neq) {}
 
@@ -824,7 +938,7 @@ struct GpuOps {
             if (own(e)) {
                 const int i = idx(e);
                 // equilibrated system: b' = W b
-                L.b[i] = (gamma * L.f[i] - (rl1 * z[1][e] + acor[e])) * ewt[e];
+                L.b[i] = (gamma * L.f[i] - (rl1 * z[1][e] + acor[e])) * (equilibrate ? ewt[e] : 1.0);
             }
         __syncthreads();
     }
@@ -841,8 +955,9 @@ struct GpuOps {
         for (int e = 0; e < EPT; e++)
             if (own(e)) {
                 const int i = idx(e);
-                double dw = scale * L.b[i];     // weighted correction W d
-                double d = dw / ewt[e];
+                double d = scale * L.b[i];
+                if (equilibrate) d = d / ewt[e];  // the equilibrated solve returns W d
+                double dw = d * ewt[e];
                 acc += dw * dw;
                 acor[e] += d;
                 L.y[i] = z[0][e] + acor[e];
@@ -854,12 +969,12 @@ struct GpuOps {
     {
 #pragma unroll
         for (int e = 0; e < EPT; e++)
-            if (own(e)) L.g[idx(e)] = ewt[e];
+            if (own(e)) L.g[idx(e)] = equilibrate ? ewt[e] : 1.0;
         __syncthreads();
         uint64_t t0 = wall_clock64();
-        block_newton_matrix<LUT>(N, L, M, gamma);
+        block_newton_matrix<LUT>(N, L, M, gamma, true, equilibrate);
         uint64_t t1 = wall_clock64();
-        int r = block_lu<LUT>(M, neq, N.ld, L.red);
+        int r = block_lu<LUT>(M, neq, N.ld, L.red, L.flux);
         uint64_t t2 = wall_clock64();
         t_jac += t1 - t0;
         t_lu += t2 - t1;
@@ -923,6 +1038,7 @@ struct SolveArgs {
     void *ws;                 // slot workspaces
     size_t ws_stride;         // bytes per slot
     int *counter;             // work queue head
+    int equilibrate;          // factor the error-weight-equilibrated Newton matrix
 };
 
 __device__ void elemental_start(const DevNet &N, const double *prm, double *y, double density)
@@ -1024,6 +1140,7 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
             elemental_start(N, prm, L.y, L.sc[S_DENS]);
         }
         GpuOps<LUT> o(N, L, M);
+        o.equilibrate = A.equilibrate != 0;
         bdf::Stats st;
         int row = 0, nint = 0, nretry = 0, flag = 0;
         record_row(N, L, A, pidx, row++, 0.0);
@@ -1227,17 +1344,12 @@ __global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
     block_rhs(N, L);   // leaves the pre-transfer surface total and the photo rates at y
-    // J itself: gamma = -1 and unit weights give M = I + J; subtract I on output
-    for (int i = tid(); i < N.neq; i += NT) L.g[i] = 1.0;
-    __syncthreads();
     double *M = (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
-    block_newton_matrix<double>(N, L, M, -1.0);
+    block_newton_matrix<double>(N, L, M, 0.0, false);   // J itself, before forming I - gamma J
     const int neq = N.neq;
     for (int idx = tid(); idx < neq * neq; idx += NT) {
         const int j = idx / neq, i = idx % neq;
-        double v = M[(size_t)j * N.ld + i];
-        if (i == j) v -= 1.0;
-        A.jac_out[(size_t)p * neq * neq + idx] = v;
+        A.jac_out[(size_t)p * neq * neq + idx] = M[(size_t)j * N.ld + i];
     }
 }
 
@@ -1258,7 +1370,7 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
     __syncthreads();
     LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
     block_newton_matrix<LUT>(N, L, M, A.gamma);
-    int bad = block_lu<LUT>(M, N.neq, N.ld, L.red);
+    int bad = block_lu<LUT>(M, N.neq, N.ld, L.red, L.flux);
     for (int i = tid(); i < N.neq; i += NT) L.b[i] = A.scal_out[(size_t)p * N.neq + i];
     __syncthreads();
     if (!bad) block_lu_solve<LUT>(M, N.neq, N.ld, L.b);

@@ -1,0 +1,149 @@
+! ISO_C_BINDING interface of libuclchem_gpu.so for the UCLCHEM Fortran core.
+!
+! The C prototypes are in include/uclchem_gpu.h.  Array layouts line up with the
+! reference's own column-major arrays, so no copies are needed:
+!   params(UCG_NPARAM, npoints)          <-> C params [n][UCG_NPARAM]
+!   abund(NEQ, npoints)                  <-> C y_init / y_out [n][NEQ]
+!                                            (chemistry.f90:40,55 abund(NEQ,points))
+!   rec(4+nspec, max_rec, npoints)       <-> C rec [n][max_rec][4+nspec]
+!                                            (one row per output time: time, density,
+!                                             gasTemp, Av, abundances, as wrap.f90 writes
+!                                             physicsarray/chemicalabunarray)
+! Parameter slots are given 1-based here (UCG_P_* + 1 of the C header).
+MODULE uclchem_gpu
+    USE, INTRINSIC :: ISO_C_BINDING
+    IMPLICIT NONE
+    PRIVATE
+    PUBLIC :: ucg_init, ucg_network_load_file_f, ucg_network_info, ucg_solve_batch_f, ucg_error_message, &
+              ucg_default_params
+    INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 24
+    INTEGER, PARAMETER, PUBLIC :: UCGP_DENS = 1, UCGP_TEMP = 2, UCGP_ZETA = 3, UCGP_RADFIELD = 4, &
+        UCGP_BASEAV = 5, UCGP_ROUT = 6, UCGP_RIN = 7, UCGP_FINALTIME = 8, UCGP_FREEFALL = 9, &
+        UCGP_FINALDENS = 10, UCGP_RELTOL = 11, UCGP_ABSFAC = 12, UCGP_ABSMIN = 13, UCGP_MODEL = 14, &
+        UCGP_TEMPINDX = 15, UCGP_MAXTEMP = 16, UCGP_FREEZEFAC = 17, UCGP_ENDATFINALDENS = 18, &
+        UCGP_FREEFALLFAC = 19, UCGP_MXSTEP = 20
+
+    INTERFACE
+        INTEGER(C_INT) FUNCTION ucg_init(device) BIND(C, name="ucg_init")
+            IMPORT :: C_INT
+            INTEGER(C_INT), VALUE :: device
+        END FUNCTION ucg_init
+
+        INTEGER(C_INT) FUNCTION c_network_load_file(path, handle) BIND(C, name="ucg_network_load_file")
+            IMPORT :: C_INT, C_CHAR
+            CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path
+            INTEGER(C_INT), INTENT(OUT) :: handle
+        END FUNCTION c_network_load_file
+
+        INTEGER(C_INT) FUNCTION ucg_network_info(handle, nspec, neq, nreac) BIND(C, name="ucg_network_info")
+            IMPORT :: C_INT
+            INTEGER(C_INT), VALUE :: handle
+            INTEGER(C_INT), INTENT(OUT) :: nspec, neq, nreac
+        END FUNCTION ucg_network_info
+
+        INTEGER(C_INT) FUNCTION c_solve_batch(handle, n, params, y_init, y_out, max_rec, rec, n_rec, status, &
+                                              stats, lu_bits) BIND(C, name="ucg_solve_batch")
+            IMPORT :: C_INT, C_DOUBLE, C_PTR
+            INTEGER(C_INT), VALUE :: handle, n, max_rec, lu_bits
+            REAL(C_DOUBLE), INTENT(IN) :: params(*)
+            TYPE(C_PTR), VALUE :: y_init
+            REAL(C_DOUBLE), INTENT(OUT) :: y_out(*)
+            TYPE(C_PTR), VALUE :: rec
+            INTEGER(C_INT), INTENT(OUT) :: n_rec(*), status(*), stats(*)
+        END FUNCTION c_solve_batch
+
+        TYPE(C_PTR) FUNCTION c_last_error() BIND(C, name="ucg_last_error")
+            IMPORT :: C_PTR
+        END FUNCTION c_last_error
+
+        INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, name="strlen")
+            IMPORT :: C_PTR, C_SIZE_T
+            TYPE(C_PTR), VALUE :: s
+        END FUNCTION c_strlen
+    END INTERFACE
+
+CONTAINS
+
+    ! Load a network file written by MakeRates' HIP backend (<name>.ucgnet).
+    INTEGER FUNCTION ucg_network_load_file_f(path, handle) RESULT(rc)
+        CHARACTER(LEN=*), INTENT(IN) :: path
+        INTEGER, INTENT(OUT) :: handle
+        INTEGER(C_INT) :: h
+        rc = c_network_load_file(TRIM(path)//C_NULL_CHAR, h)
+        handle = h
+    END FUNCTION ucg_network_load_file_f
+
+    ! Solve npoints independent grid points (the solveAbundances loop of wrap.f90:549-697
+    ! per point).  abund_init absent: UCLCHEM's elemental start.  rec optional.
+    INTEGER FUNCTION ucg_solve_batch_f(handle, params, abund_out, status, stats, abund_init, rec, n_rec, &
+                                       lu_bits) RESULT(rc)
+        INTEGER, INTENT(IN) :: handle
+        REAL(C_DOUBLE), INTENT(IN) :: params(:, :)
+        REAL(C_DOUBLE), INTENT(OUT) :: abund_out(:, :)
+        INTEGER(C_INT), INTENT(OUT) :: status(:), stats(:, :)
+        REAL(C_DOUBLE), INTENT(IN), OPTIONAL, TARGET, CONTIGUOUS :: abund_init(:, :)
+        REAL(C_DOUBLE), INTENT(OUT), OPTIONAL, TARGET, CONTIGUOUS :: rec(:, :, :)
+        INTEGER(C_INT), INTENT(OUT), OPTIONAL :: n_rec(:)
+        INTEGER, INTENT(IN), OPTIONAL :: lu_bits
+        INTEGER(C_INT) :: n, bits, max_rec
+        INTEGER(C_INT), ALLOCATABLE :: nrec_local(:)
+        TYPE(C_PTR) :: p_init, p_rec
+        n = SIZE(params, 2)
+        bits = 64
+        IF (PRESENT(lu_bits)) bits = lu_bits
+        p_init = C_NULL_PTR
+        IF (PRESENT(abund_init)) p_init = C_LOC(abund_init)
+        p_rec = C_NULL_PTR
+        max_rec = 0
+        IF (PRESENT(rec)) THEN
+            p_rec = C_LOC(rec)
+            max_rec = SIZE(rec, 2)
+        END IF
+        ALLOCATE(nrec_local(n))
+        rc = c_solve_batch(INT(handle, C_INT), n, params, p_init, abund_out, max_rec, p_rec, nrec_local, status, &
+                           stats, bits)
+        IF (PRESENT(n_rec)) n_rec = nrec_local
+    END FUNCTION ucg_solve_batch_f
+
+    FUNCTION ucg_error_message() RESULT(msg)
+        CHARACTER(LEN=:), ALLOCATABLE :: msg
+        TYPE(C_PTR) :: p
+        CHARACTER(KIND=C_CHAR), POINTER :: chars(:)
+        INTEGER :: i, n
+        p = c_last_error()
+        n = INT(c_strlen(p))
+        CALL C_F_POINTER(p, chars, [n])
+        ALLOCATE(CHARACTER(LEN=n) :: msg)
+        DO i = 1, n
+            msg(i:i) = chars(i)
+        END DO
+    END FUNCTION ucg_error_message
+
+    ! UCLCHEM's defaults (defaultparameters.f90:18-122), rout as the compiled reference
+    ! holds it (a default-kind literal, float32-rounded).
+    SUBROUTINE ucg_default_params(p)
+        REAL(C_DOUBLE), INTENT(OUT) :: p(UCG_NPARAM)
+        p = 0.0D0
+        p(UCGP_DENS) = 1.0D2
+        p(UCGP_TEMP) = 10.0D0
+        p(UCGP_ZETA) = 1.0D0
+        p(UCGP_RADFIELD) = 1.0D0
+        p(UCGP_BASEAV) = 2.0D0
+        p(UCGP_ROUT) = REAL(0.05, C_DOUBLE)
+        p(UCGP_RIN) = 0.0D0
+        p(UCGP_FINALTIME) = 5.0D6
+        p(UCGP_FREEFALL) = 0.0D0
+        p(UCGP_FINALDENS) = 1.0D5
+        p(UCGP_RELTOL) = 1.0D-8
+        p(UCGP_ABSFAC) = 1.0D-14
+        p(UCGP_ABSMIN) = 1.0D-25
+        p(UCGP_MODEL) = 0.0D0
+        p(UCGP_TEMPINDX) = 1.0D0
+        p(UCGP_MAXTEMP) = 300.0D0
+        p(UCGP_FREEZEFAC) = 1.0D0
+        p(UCGP_ENDATFINALDENS) = 0.0D0
+        p(UCGP_FREEFALLFAC) = 1.0D0
+        p(UCGP_MXSTEP) = 10000.0D0
+    END SUBROUTINE ucg_default_params
+
+END MODULE uclchem_gpu

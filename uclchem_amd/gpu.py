@@ -16,12 +16,6 @@ from .params import NPARAM
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libuclchem_gpu.so")
 
-SPECIAL_SPECIES_ORDER = ["nh", "nh2", "nhe", "nc", "ncx", "nn", "no", "nf", "np", "nmg", "nmgx", "nna",
-                         "nli", "npah", "nsx", "nsix", "nclx", "nd", "n18o", "n15n", "n13c", "nelec", "nco",
-                         "nsurface", "nbulk"]
-SPECIAL_REACTION_ORDER = ["nR_H2Form_CT", "nR_H2Form_ER", "nR_H2Form_ERDes", "nR_HFreeze", "nR_EFreeze",
-                          "nR_H2Freeze", "nR_H2_hv", "nR_CO_hv", "nR_C_hv"]
-
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 
@@ -72,60 +66,28 @@ def _p(a):
 class Network:
     """A network resident on the GPU (tables uploaded once, immutable)."""
 
-    def __init__(self, tables: mh.NetworkTables, device: int = 0):
+    def __init__(self, tables: mh.NetworkTables, device: int = 0, from_file: Optional[str] = None):
+        """Upload `tables`; with `from_file` the library reads the same tables from a
+        .ucgnet file itself (ucg_network_load_file), as a Fortran caller would."""
         self.t = tables
         self.nspec, self.neq, self.nreac = tables.nspec, tables.neq, tables.nreac
         _check(lib().ucg_init(ctypes.c_int(device)), "ucg_init")
-        a = tables.arrays
-        keep = {}
-
-        def i32(x):
-            arr = np.ascontiguousarray(x, dtype=np.int32)
-            keep[id(arr)] = arr
-            return arr
-
-        def f64(x):
-            arr = np.ascontiguousarray(x, dtype=np.float64)
-            keep[id(arr)] = arr
-            return arr
-
-        neq = tables.neq
-        bps = np.full(neq, -1, dtype=np.int32)
-        for q, s in enumerate(a["surface_list"]):
-            bps[s] = a["bulk_partner_of_surface"][q]
-        spb = np.full(neq, -1, dtype=np.int32)
-        for q, b in enumerate(a["bulk_list"]):
-            spb[b] = a["surface_partner_of_bulk"][q]
-        is_s = np.zeros(neq, dtype=np.int32)
-        is_s[a["surface_list"]] = 1
-        is_b = np.zeros(neq, dtype=np.int32)
-        is_b[a["bulk_list"]] = 1
+        if from_file is not None:
+            h = ctypes.c_int(-1)
+            _check(lib().ucg_network_load_file(from_file.encode(), ctypes.byref(h)), "ucg_network_load_file")
+            self.handle = h.value
+            return
+        sizes, arrays, ssp, srx = mh.device_descriptor(tables)
+        self._keep = arrays
         d = _Desc()
-        d.nspec, d.neq, d.nreac = tables.nspec, neq, tables.nreac
-        d.nice = len(a["ice_list"])
-        d.nsurfl, d.nbulkl = len(a["surface_list"]), len(a["bulk_list"])
-        d.nfp, d.nswap = len(a["freeze_partners"]), len(a["swap_list"])
-        d.nloss, d.nprod = len(a["loss_idx"]), len(a["prod_idx"])
-        d.nnz, d.ncontrib = len(a["jnz_row"]), len(a["jc_k"])
-        d.naff, d.nsw = len(a["aff_sp"]), len(a["sw_k"])
-        for n in ["alpha", "beta", "gama", "tmin", "tmax", "mass", "rstat"]:
-            setattr(d, n, _p(f64(a[n])).value)
-        for n in ["rtype", "re", "pr", "ice_kind", "n_d", "ops", "fac", "nfac", "type_ranges", "loss_ptr",
-                  "loss_idx", "prod_ptr", "prod_idx", "swap_list", "surface_list", "bulk_list",
-                  "freeze_partners", "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c", "aff_ptr",
-                  "aff_sp", "aff_cf", "sw_ptr", "sw_k", "sw_c"]:
-            setattr(d, n, _p(i32(a[n])).value)
-        ex = np.ascontiguousarray(a["extrapolate"], dtype=np.int8)
-        keep[id(ex)] = ex
-        d.extrapolate = _p(ex).value
-        d.bulk_partner_of_surface = _p(i32(bps)).value
-        d.surface_partner_of_bulk = _p(i32(spb)).value
-        d.is_surface = _p(i32(is_s)).value
-        d.is_bulk = _p(i32(is_b)).value
-        for i, k in enumerate(SPECIAL_SPECIES_ORDER):
-            d.special_species[i] = tables.scalars[k]
-        for i, k in enumerate(SPECIAL_REACTION_ORDER):
-            d.special_reactions[i] = tables.scalars[k]
+        for k, v in sizes.items():
+            setattr(d, k, v)
+        for k, v in arrays.items():
+            setattr(d, k, _p(v).value)
+        for i in range(25):
+            d.special_species[i] = int(ssp[i])
+        for i in range(9):
+            d.special_reactions[i] = int(srx[i])
         h = ctypes.c_int(-1)
         _check(lib().ucg_network_load(ctypes.byref(d), ctypes.byref(h)), "ucg_network_load")
         self.handle = h.value

@@ -681,6 +681,90 @@ def load_network(path: str) -> NetworkTables:
                          arrays, scalars)
 
 
+# ------------------------------------------------------------------ device descriptor
+SPECIAL_SPECIES_ORDER = ["nh", "nh2", "nhe", "nc", "ncx", "nn", "no", "nf", "np", "nmg", "nmgx", "nna",
+                         "nli", "npah", "nsx", "nsix", "nclx", "nd", "n18o", "n15n", "n13c", "nelec", "nco",
+                         "nsurface", "nbulk"]
+SPECIAL_REACTION_ORDER = ["nR_H2Form_CT", "nR_H2Form_ER", "nR_H2Form_ERDes", "nR_HFreeze", "nR_EFreeze",
+                          "nR_H2Freeze", "nR_H2_hv", "nR_CO_hv", "nR_C_hv"]
+DESC_SIZES = ["nspec", "neq", "nreac", "nice", "nsurfl", "nbulkl", "nfp", "nswap",
+              "nloss", "nprod", "nnz", "ncontrib", "naff", "nsw"]
+DESC_F64 = ["alpha", "beta", "gama", "tmin", "tmax", "mass", "rstat"]
+DESC_I32 = ["rtype", "re", "pr", "ice_kind", "n_d", "ops", "fac", "nfac", "type_ranges", "loss_ptr",
+            "loss_idx", "prod_ptr", "prod_idx", "swap_list", "surface_list", "bulk_list",
+            "bulk_partner_of_surface", "surface_partner_of_bulk", "freeze_partners", "is_surface", "is_bulk",
+            "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c", "aff_ptr", "aff_sp", "aff_cf",
+            "sw_ptr", "sw_k", "sw_c"]
+DESC_I8 = ["extrapolate"]
+
+
+def device_descriptor(t: NetworkTables):
+    """Sizes, typed arrays and special indices of ucg_network_desc (include/uclchem_gpu.h).
+
+    Partner arrays are expanded to length neq (-1 where a species has no partner) and
+    the surface/bulk class flags are added, so the device indexes them directly."""
+    a = t.arrays
+    neq = t.neq
+    bps = np.full(neq, -1, dtype=np.int32)
+    bps[a["surface_list"]] = a["bulk_partner_of_surface"]
+    spb = np.full(neq, -1, dtype=np.int32)
+    spb[a["bulk_list"]] = a["surface_partner_of_bulk"]
+    is_s = np.zeros(neq, dtype=np.int32)
+    is_s[a["surface_list"]] = 1
+    is_b = np.zeros(neq, dtype=np.int32)
+    is_b[a["bulk_list"]] = 1
+    arrays = {}
+    for n in DESC_F64:
+        arrays[n] = np.ascontiguousarray(a[n], dtype=np.float64)
+    for n in DESC_I32:
+        if n == "bulk_partner_of_surface":
+            arrays[n] = bps
+        elif n == "surface_partner_of_bulk":
+            arrays[n] = spb
+        elif n == "is_surface":
+            arrays[n] = is_s
+        elif n == "is_bulk":
+            arrays[n] = is_b
+        else:
+            arrays[n] = np.ascontiguousarray(a[n], dtype=np.int32)
+    arrays["extrapolate"] = np.ascontiguousarray(a["extrapolate"], dtype=np.int8)
+    sizes = dict(nspec=t.nspec, neq=neq, nreac=t.nreac, nice=len(a["ice_list"]),
+                 nsurfl=len(a["surface_list"]), nbulkl=len(a["bulk_list"]), nfp=len(a["freeze_partners"]),
+                 nswap=len(a["swap_list"]), nloss=len(a["loss_idx"]), nprod=len(a["prod_idx"]),
+                 nnz=len(a["jnz_row"]), ncontrib=len(a["jc_k"]), naff=len(a["aff_sp"]), nsw=len(a["sw_k"]))
+    special_species = np.array([t.scalars[k] for k in SPECIAL_SPECIES_ORDER], dtype=np.int32)
+    special_reactions = np.array([t.scalars[k] for k in SPECIAL_REACTION_ORDER], dtype=np.int32)
+    return sizes, arrays, special_species, special_reactions
+
+
+BIN_MAGIC = b"UCGNET01"
+_BIN_DTYPES = {np.dtype(np.float64): 0, np.dtype(np.int32): 1, np.dtype(np.int8): 2}
+
+
+def save_network_bin(t: NetworkTables, path: str) -> None:
+    """Write the device descriptor as a flat little-endian file that the C ABI loads
+    directly (ucg_network_load_file), for callers without Python (the Fortran core).
+
+    Layout: magic "UCGNET01", int32 record count, then per record a 24-byte
+    NUL-padded name, int32 dtype (0 f64, 1 i32, 2 i8), int64 element count, the data,
+    and zero padding to a multiple of 8 bytes."""
+    sizes, arrays, ssp, srx = device_descriptor(t)
+    recs = [("sizes", np.array([sizes[k] for k in DESC_SIZES], dtype=np.int32)),
+            ("special_species", ssp), ("special_reactions", srx)]
+    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8]
+    import struct
+    with open(path, "wb") as f:
+        f.write(BIN_MAGIC)
+        f.write(struct.pack("<i", len(recs)))
+        for name, arr in recs:
+            arr = np.ascontiguousarray(arr)
+            f.write(name.encode().ljust(24, b"\0"))
+            f.write(struct.pack("<iq", _BIN_DTYPES[arr.dtype], arr.size))
+            raw = arr.astype(arr.dtype.newbyteorder("<"), copy=False).tobytes()
+            f.write(raw)
+            f.write(b"\0" * ((-len(raw)) % 8))
+
+
 NETWORK_DIR = os.path.join(os.path.dirname(__file__), "networks")
 
 
@@ -701,4 +785,5 @@ if __name__ == "__main__":  # regenerate the default network from MakeRates' CSV
     a = ap.parse_args()
     tb = build_from_csv(a.species, a.reactions, a.name)
     save_network(tb, network_path(a.name))
-    print(f"wrote {network_path(a.name)}: {tb.nspec} species, {tb.nreac} reactions")
+    save_network_bin(tb, network_path(a.name)[:-4] + ".ucgnet")
+    print(f"wrote {network_path(a.name)} (+ .ucgnet): {tb.nspec} species, {tb.nreac} reactions")

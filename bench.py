@@ -123,20 +123,36 @@ def main():
     d_t1 = torch.zeros(n, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    def run(lu_bits, npts):
+    side = torch.cuda.Stream(dev)                 # default priority, like every stream here
+    host_done = torch.empty(n, dtype=torch.float64, pin_memory=True)
+
+    def run(lu_bits, npts, label):
+        d_t1.zero_()
         net.solve_device(d_params[:npts], d_y[:npts], d_status[:npts], d_stats[:npts], t_start=d_t0[:npts],
                          t_done=d_t1[:npts], lu_bits=lu_bits, stream=stream.cuda_stream)
+        done_ev = torch.cuda.Event()
+        done_ev.record(stream)
+        t_last = time.time()
+        while not done_ev.query():                # progress from the kernel's own retirement stamps
+            time.sleep(0.5)
+            if time.time() - t_last > 20.0:
+                with torch.cuda.stream(side):
+                    host_done[:npts].copy_(d_t1[:npts], non_blocking=True)
+                side.synchronize()
+                print(f"[bench] {label}: {int((host_done[:npts] > 0).sum())}/{npts} points retired",
+                      file=sys.stderr, flush=True)
+                t_last = time.time()
 
     results = {}
     for lu_bits in ([a.lu] + ([32] if (a.also_fp32 and a.lu == 64) else [])):
         for _ in range(a.warmup):
-            run(lu_bits, min(n, 256))
+            run(lu_bits, min(n, 256), f"fp{lu_bits} warmup")
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t_wall = time.time()
         ev0.record(stream)
         for _ in range(a.steps):
-            run(lu_bits, n)
+            run(lu_bits, n, f"fp{lu_bits}")
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.time() - t_wall

@@ -81,6 +81,13 @@ typedef struct ucg_network_desc {
     const int *aff_ptr, *aff_sp, *aff_cf, *sw_ptr, *sw_k, *sw_c;
     int special_species[25];   /* nh nh2 nhe nc ncx nn no nf np nmg nmgx nna nli npah nsx nsix nclx nd n18o n15n n13c nelec nco nsurface nbulk */
     int special_reactions[9];  /* H2Form_CT H2Form_ER H2Form_ERDes HFreeze EFreeze H2Freeze H2_hv CO_hv C_hv */
+    /* static-pattern sparse Newton plan (uclchem_amd/newton_plan.py): sizes are
+     * N ns T nsp nlev ntg npair nph nphs ngs; arrays as named there */
+    int nw_sizes[10];
+    const int *nw_perm, *nw_inv, *nw_sp_ptr, *nw_sp_col, *nw_sp_dpos, *nw_lev_ptr, *nw_lev_rows, *nw_ph_ptr;
+    const int *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
+    const int *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
+    const int *nw_misc, *nw_nbrow, *nw_nsrow;
 } ucg_network_desc;
 
 /* Select the GPU and upload the constant photo-rate tables.  Called implicitly. */
@@ -132,6 +139,15 @@ int ucg_jacobian(int handle, int n, const double *params, const double *phys, co
                  const double *y, double *jac);
 int ucg_newton_solve(int handle, int n, const double *params, const double *phys, const double *y,
                      double gamma, const double *b, double *x, int lu_bits);
+
+/* Linear solver of the Newton iterations for a handle (DVJAC/DGEFA/DGESL's job,
+ * dvode.f90:8182-8390, 11982-12206):
+ *   UCG_LINSOL_SPARSE (default): the static-pattern plan built with the network
+ *     (fill-reducing order fixed at build time, bordered rank-one couplings, dense tail);
+ *   UCG_LINSOL_DENSE: blocked LU of the full NEQ x NEQ matrix. */
+#define UCG_LINSOL_SPARSE 0
+#define UCG_LINSOL_DENSE 1
+int ucg_set_linear_solver(int handle, int kind);
 
 /* Network sizes of a loaded handle. */
 int ucg_network_info(int handle, int *nspec, int *neq, int *nreac);

@@ -79,9 +79,21 @@ def test_jacobian_directional_derivatives(gpu_net, tables, oracle):
             assert (best / scale).max() <= 1e-6, (q, rep)
 
 
+@pytest.mark.parametrize("kind", ["sparse", "dense"])
 @pytest.mark.parametrize("lu_bits,tol", [(64, 1e-12), (32, 1e-5)])
-def test_newton_solve_backward_error(gpu_net, tables, lu_bits, tol):
-    """(I - gamma J) x = b on states along a real trajectory (config 1 rows)."""
+def test_newton_solve_backward_error(gpu_net, tables, lu_bits, tol, kind):
+    """(I - gamma J) x = b on states along a real trajectory (config 1 rows), J from the
+    dense Jacobian kernel: checks the sparse plan's assembly, bordering, sum-row
+    replacement, factorisation and solve against the plain matrix."""
+    from uclchem_amd import workloads
+    gpu_net.set_linear_solver(gpu_net.SPARSE if kind == "sparse" else gpu_net.DENSE)
+    try:
+        _newton_backward_error(gpu_net, tables, lu_bits, tol)
+    finally:
+        gpu_net.set_linear_solver(gpu_net.SPARSE)
+
+
+def _newton_backward_error(gpu_net, tables, lu_bits, tol):
     from uclchem_amd import workloads
     out = gpu_net.solve(workloads.static_cloud_point(), max_rec=60)
     rows = out["rec"][0, :out["n_rec"][0]]

@@ -44,11 +44,16 @@ def _conservation(tables, y0, y1):
     return np.abs(t1[keep] - t0[keep]) / t0[keep]
 
 
+@pytest.mark.parametrize("kind", ["sparse", "dense"])
 @pytest.mark.parametrize("lu_bits", [64, 32])
-def test_config1_matches_example_output(gpu_net, tables, lu_bits):
+def test_config1_matches_example_output(gpu_net, tables, lu_bits, kind):
     from uclchem_amd import workloads
     g = np.load(os.path.join(GOLD, "static_full_t1e6.npz"))
-    out = gpu_net.solve(workloads.static_cloud_point(), max_rec=60, lu_bits=lu_bits)
+    gpu_net.set_linear_solver(gpu_net.SPARSE if kind == "sparse" else gpu_net.DENSE)
+    try:
+        out = gpu_net.solve(workloads.static_cloud_point(), max_rec=60, lu_bits=lu_bits)
+    finally:
+        gpu_net.set_linear_solver(gpu_net.SPARSE)
     assert out["status"][0] == 0
     n = out["n_rec"][0]
     assert n == len(g["times"])

@@ -37,6 +37,7 @@ struct NetHandle {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     int *counter = nullptr;
+    int linear_solver = UCG_LINSOL_SPARSE;
 };
 std::vector<NetHandle> g_nets;
 
@@ -185,7 +186,8 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     if (n_slots <= 0) n_slots = 2 * g_num_cu;
     if (n_slots > n) n_slots = n;
     const size_t elem = lu_bits == 64 ? sizeof(double) : sizeof(float);
-    size_t stride = (size_t)N.ld * N.neq * elem;
+    const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
+    size_t stride = (sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem;
     stride = (stride + 255) & ~(size_t)255;
     int rc = ensure_ws(h, stride * n_slots);
     if (rc) return rc;
@@ -204,6 +206,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.t_done = t_done;
     A.timers = timers;
     A.equilibrate = getenv("UCG_EQUILIBRATE") ? atoi(getenv("UCG_EQUILIBRATE")) : 0;
+    A.sparse = sparse ? 1 : 0;
     A.ws = h.ws;
     A.ws_stride = stride;
     A.counter = h.counter;
@@ -254,6 +257,59 @@ int ucg_init(int device)
 }
 
 static int load_tables(const ucg_network_desc *d, NetHandle &h);
+
+// Every index of the sparse Newton plan is range-checked here, on the host, so that no
+// table handed in by a caller can make the device address outside its buffers.
+static int check_plan(const ucg_network_desc *d)
+{
+    const int *z = d->nw_sizes;
+    const long NN = z[0], ns = z[1], T = z[2], nsp = z[3], nlev = z[4], ntg = z[5], npair = z[6], nph = z[7];
+    const long nphs = z[8], ngs = z[9];
+    const long nval = nsp + T * T;
+    auto bad = [](const char *what) { return fail(UCG_E_ARG, std::string("sparse Newton plan: ") + what); };
+    if (NN != d->neq + 2 || ns < 0 || ns > NN || T != NN - ns || nsp < 0 || nlev < 0 || ntg < 0 || npair < 0 ||
+        nph < 1 || nphs < 0 || ngs < 0)
+        return bad("inconsistent sizes");
+    if (T > 0 && NN - ns > ucg::NT * ucg::EPT) return bad("tail too large");
+    auto in = [](const int *a, long n, long lo, long hi) {   // every a[i] in [lo, hi)
+        if (n > 0 && !a) return false;
+        for (long i = 0; i < n; i++)
+            if (a[i] < lo || a[i] >= hi) return false;
+        return true;
+    };
+    auto mono = [](const int *a, long n, long first, long last) {   // a[0..n], non-decreasing
+        if (!a || a[0] != first || a[n] != last) return false;
+        for (long i = 0; i < n; i++)
+            if (a[i + 1] < a[i]) return false;
+        return true;
+    };
+    if (!in(d->nw_perm, NN, 0, NN) || !in(d->nw_inv, NN, 0, NN)) return bad("permutation out of range");
+    for (long i = 0; i < NN; i++)
+        if (d->nw_perm[d->nw_inv[i]] != i) return bad("perm and inv are not inverse");
+    if (!mono(d->nw_sp_ptr, NN, 0, nsp) || !in(d->nw_sp_col, nsp, 0, NN)) return bad("sparse rows");
+    for (long r = 0; r < ns; r++)
+        if (d->nw_sp_dpos[r] < d->nw_sp_ptr[r] || d->nw_sp_dpos[r] >= d->nw_sp_ptr[r + 1] ||
+            d->nw_sp_col[d->nw_sp_dpos[r]] != r)
+            return bad("diagonal positions");
+    if (!mono(d->nw_lev_ptr, nlev, 0, ns) || !in(d->nw_lev_rows, ns, 0, ns)) return bad("levels");
+    if (!mono(d->nw_ph_ptr, nph, 0, ntg) || !in(d->nw_tg_slot, ntg, 0, nval) || !mono(d->nw_tg_ptr, ntg, 0, npair) ||
+        !in(d->nw_tg_div, ntg, -1, nval) || !in(d->nw_pr_l, npair, 0, nval) || !in(d->nw_pr_u, npair, 0, nval))
+        return bad("elimination schedule");
+    const long Q = d->neq;
+    if (!in(d->nw_jnz_slot, d->nnz, 0, nval) || !in(d->nw_diag_slot, Q, -1, nval) ||
+        !in(d->nw_ph_slot, nphs, 0, nval) || !in(d->nw_tr_slot, 8L * d->nsurfl, 0, nval) ||
+        !in(d->nw_s1c, Q, -1, nval) || !in(d->nw_s1r, Q, -1, nval) || !in(d->nw_s2c, Q, -1, nval) ||
+        !in(d->nw_s2r, Q, -1, nval) || !mono(d->nw_gs_ptr, Q, 0, ngs) || !in(d->nw_gs_slot, ngs, 0, nval) ||
+        !in(d->nw_misc, 5, 0, nval) || !in(d->nw_nbrow, d->nbulkl, 0, nval) || !in(d->nw_nsrow, d->nsurfl, 0, nval))
+        return bad("assembly slots");
+    long naff_photo = 0;
+    for (int k : {d->special_reactions[6], d->special_reactions[7]}) {
+        if (k < 0 || k >= d->nreac) return bad("photo reactions");
+        naff_photo += d->aff_ptr[k + 1] - d->aff_ptr[k];
+    }
+    if (nphs != 3 * naff_photo) return bad("photo slots");
+    return 0;
+}
 
 int ucg_network_load(const ucg_network_desc *d, int *handle)
 {
@@ -373,6 +429,21 @@ int ucg_network_load_file(const char *path, int *handle)
     memcpy(d.special_species, p, sizeof d.special_species);
     if ((rc = find("special_reactions", 1, 9, &p))) return rc;
     memcpy(d.special_reactions, p, sizeof d.special_reactions);
+    if ((rc = find("nw_sizes", 1, 10, &p))) return rc;
+    memcpy(d.nw_sizes, p, sizeof d.nw_sizes);
+    {
+        const int *z = d.nw_sizes;
+        const int64_t NN = z[0], ns = z[1], nsp = z[3], nlev = z[4], ntg = z[5], npair = z[6], nph = z[7];
+        keep_i.reserve(keep_i.size() + 30);
+#define GI(f, n) if ((rc = geti(#f, (n), &d.f))) return rc
+        GI(nw_perm, NN); GI(nw_inv, NN); GI(nw_sp_ptr, NN + 1); GI(nw_sp_col, nsp); GI(nw_sp_dpos, ns);
+        GI(nw_lev_ptr, nlev + 1); GI(nw_lev_rows, ns); GI(nw_ph_ptr, nph + 1); GI(nw_tg_slot, ntg);
+        GI(nw_tg_ptr, ntg + 1); GI(nw_tg_div, ntg); GI(nw_pr_l, npair); GI(nw_pr_u, npair);
+        GI(nw_jnz_slot, d.nnz); GI(nw_diag_slot, Q); GI(nw_ph_slot, z[8]); GI(nw_tr_slot, 8 * (int64_t)d.nsurfl);
+        GI(nw_s1c, Q); GI(nw_s1r, Q); GI(nw_s2c, Q); GI(nw_s2r, Q); GI(nw_gs_ptr, Q + 1); GI(nw_gs_slot, z[9]);
+        GI(nw_misc, 5); GI(nw_nbrow, d.nbulkl); GI(nw_nsrow, d.nsurfl);
+#undef GI
+    }
     return ucg_network_load(&d, handle);
 }
 
@@ -407,6 +478,20 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     UP(jc_k, (size_t)d->ncontrib); UP(jc_m, (size_t)d->ncontrib); UP(jc_c, (size_t)d->ncontrib);
     UP(aff_ptr, R + 1); UP(aff_sp, (size_t)d->naff); UP(aff_cf, (size_t)d->naff);
     UP(sw_ptr, Q + 1); UP(sw_k, (size_t)d->nsw); UP(sw_c, (size_t)d->nsw);
+    if ((rc = check_plan(d))) return rc;
+    {
+        const int *z = d->nw_sizes;
+        N.nw_N = z[0]; N.nw_ns = z[1]; N.nw_T = z[2]; N.nw_nsp = z[3]; N.nw_nlev = z[4]; N.nw_nph = z[7];
+        N.nw_nval = z[3] + z[2] * z[2];
+        const size_t NN = z[0], ns = z[1], nsp = z[3], nlev = z[4], ntg = z[5], npair = z[6], nph = z[7];
+        UP(nw_perm, NN); UP(nw_inv, NN); UP(nw_sp_ptr, NN + 1); UP(nw_sp_col, nsp); UP(nw_sp_dpos, ns);
+        UP(nw_lev_ptr, nlev + 1); UP(nw_lev_rows, ns); UP(nw_ph_ptr, nph + 1); UP(nw_tg_slot, ntg);
+        UP(nw_tg_ptr, ntg + 1); UP(nw_tg_div, ntg); UP(nw_pr_l, npair); UP(nw_pr_u, npair);
+        UP(nw_jnz_slot, (size_t)d->nnz); UP(nw_diag_slot, Q); UP(nw_ph_slot, (size_t)z[8]);
+        UP(nw_tr_slot, (size_t)8 * d->nsurfl); UP(nw_s1c, Q); UP(nw_s1r, Q); UP(nw_s2c, Q); UP(nw_s2r, Q);
+        UP(nw_gs_ptr, Q + 1); UP(nw_gs_slot, (size_t)z[9]); UP(nw_misc, (size_t)5);
+        UP(nw_nbrow, (size_t)d->nbulkl); UP(nw_nsrow, (size_t)d->nsurfl);
+    }
 #undef UP
     const int *ss = d->special_species;
     N.nh = ss[0]; N.nh2 = ss[1]; N.nhe = ss[2]; N.nc = ss[3]; N.ncx = ss[4]; N.nn = ss[5]; N.no = ss[6];
@@ -448,6 +533,16 @@ int ucg_network_free(int handle)
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->ws) (void)hipFree(h->ws);
     *h = NetHandle();
+    return 0;
+}
+
+int ucg_set_linear_solver(int handle, int kind)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (kind != UCG_LINSOL_SPARSE && kind != UCG_LINSOL_DENSE) return fail(UCG_E_ARG, "unknown linear solver");
+    h->linear_solver = kind;
     return 0;
 }
 
@@ -640,6 +735,7 @@ int ucg_newton_solve(int handle, int n, const double *params, const double *phys
     A.scal_out = (double *)db.p;   // right-hand sides
     A.y_out = (double *)dx.p;
     A.gamma = gamma;
+    A.sparse = h->linear_solver == UCG_LINSOL_SPARSE;
     size_t lds = lds_bytes(N);
     if (lu_bits == 64) {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::newton_solve_kernel<double>,

@@ -63,6 +63,12 @@ struct DevNet {
         n18o, n15n, n13c, nelec, nco, nsurface, nbulk;
     int32_t r_h2form_ct, r_h2form_er, r_h2form_erdes, r_hfreeze, r_efreeze, r_h2freeze, r_h2_hv, r_co_hv,
         r_c_hv;
+    // static-pattern sparse Newton plan (augmented order of size nw_N = neq + 2)
+    int32_t nw_N, nw_ns, nw_T, nw_nsp, nw_nlev, nw_nph, nw_nval;
+    const int32_t *nw_perm, *nw_inv, *nw_sp_ptr, *nw_sp_col, *nw_sp_dpos, *nw_lev_ptr, *nw_lev_rows, *nw_ph_ptr;
+    const int32_t *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
+    const int32_t *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
+    const int32_t *nw_misc, *nw_nbrow, *nw_nsrow;
 };
 
 // static spline tables of the photo-rate functions (photoreactions.f90:14-39),

@@ -773,6 +773,270 @@ __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
     }
 }
 
+// ---------------------------------------------------------------- sparse Newton system
+// The augmented Newton matrix of the build-time plan (uclchem_amd/newton_plan.py): the
+// sparse part M0 = I - gamma J0 of the generated reaction terms, photo-column and
+// transfer S-terms; two bordered unknowns carrying the rank-one totalSwap and transfer
+// couplings (columns u1 and w2, rows -gamma g1 and -gamma gsum0 with c1); the BULK and
+// SURFACE rows replaced by e_BULK - sum_b e_b and e_SURFACE - sum_s e_s.  Values are
+// written in plan slot order; fill slots start at zero.
+template <typename LUT>
+__device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gamma)
+{
+    const int neq = N.neq;
+    double *y = L.y, *rate = L.rate, *sc = L.sc;
+    double *scr = L.flux;   // scratch [>= 2*neq]
+    const double D = y[neq - 1];
+    const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
+    const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
+    const double blr = fmin(1.0, NUM_SITES_PER_GRAIN / (GAS_DUST_DENSITY_RATIO * sbk));
+    const double ism = 1.0 / sm;
+    const double dblr = (blr < 1.0 && ybulk > 1e-30) ? -blr / sbk : 0.0;
+    const double dism = (ysur > 1e-30) ? -ism * ism : 0.0;
+    double ts = 0.0;
+    for (int q = tid(); q < N.nswap; q += NT) ts += term_value(N, N.swap_list[q], rate, y, D, blr, 0.0, sm);
+    const double TS = block_sum(ts, L.red);
+    const double tsm = TS / sm;
+    const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
+    for (int q = tid(); q < N.nw_nval; q += NT) V[q] = (LUT)0;
+    __syncthreads();
+    // 1. generated entries of J0, stored as -gamma * dJ; u1 (totalSwap column factor)
+    for (int n = tid(); n < N.nnz; n += NT) {
+        double acc = 0.0;
+        const int c0 = N.jnz_ptr[n], c1 = N.jnz_ptr[n + 1];
+        for (int q = c0; q < c1; q++) {
+            const int k = N.jc_k[q], m = N.jc_m[q];
+            const int *fc = N.fac + MAXFAC * k;
+            const int nf = N.nfac[k];
+            double p = rate[k];
+            for (int mm = 0; mm < nf; mm++) {
+                const int c = fc[mm];
+                p *= (mm == m) ? fac_deriv(c, dblr, dism, dtsm) : fac_value(c, y, D, blr, ism, tsm);
+            }
+            acc += N.jc_c[q] * p;
+        }
+        V[N.nw_jnz_slot[n]] = (LUT)(-gamma * acc);
+    }
+    for (int i = tid(); i < neq; i += NT) {
+        double u = 0.0;
+        for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {
+            const int k = N.sw_k[q];
+            const int *fc = N.fac + MAXFAC * k;
+            double w = rate[k];
+            for (int mm = 0; mm < N.nfac[k]; mm++) {
+                const int c = fc[mm];
+                w *= (c == OP_SWAP) ? ism : fac_value(c, y, D, blr, ism, tsm);
+            }
+            u += N.sw_c[q] * w;
+        }
+        scr[i] = u;
+        scr[neq + i] = 0.0;
+    }
+    __syncthreads();
+    if (tid() == 0) {   // g1 = d totalSwap / dy
+        for (int q = 0; q < N.nswap; q++) {
+            const int k = N.swap_list[q];
+            scr[neq + N.ops[MAXOPS * k]] += rate[k] * blr;
+        }
+        scr[neq + N.nbulk] = (blr < 1.0 && ybulk > 1e-30) ? -TS / sbk : 0.0;
+    }
+    __syncthreads();
+    // 2. border rows and columns, constant sum rows
+    for (int j = tid(); j < neq; j += NT) {
+        const int r2 = N.nw_s2r[j];
+        if (r2 >= 0) {
+            double acc = 0.0;
+            for (int q = N.nw_gs_ptr[j]; q < N.nw_gs_ptr[j + 1]; q++) acc += (double)V[N.nw_gs_slot[q]];
+            V[r2] = (LUT)acc;
+        }
+        const int r1 = N.nw_s1r[j];
+        if (r1 >= 0) V[r1] = (LUT)(-gamma * scr[neq + j]);
+        const int cc = N.nw_s1c[j];
+        if (cc >= 0) V[cc] = (LUT)scr[j];
+    }
+    for (int q = tid(); q < N.nbulkl; q += NT) V[N.nw_nbrow[q]] = (LUT)(-1.0);
+    for (int q = tid(); q < N.nsurfl; q += NT) V[N.nw_nsrow[q]] = (LUT)(-1.0);
+    double cs = 0.0;
+    for (int q = tid(); q < N.nsurfl; q += NT) cs += scr[N.surface_list[q]];
+    const double c1 = block_sum(cs, L.red);
+    if (tid() == 0) {
+        V[N.nw_misc[0]] = (LUT)(-1.0);
+        V[N.nw_misc[1]] = (LUT)(-1.0);
+        V[N.nw_misc[2]] = (LUT)c1;
+        V[N.nw_misc[3]] = (LUT)1.0;
+        V[N.nw_misc[4]] = (LUT)1.0;
+    }
+    __syncthreads();
+    // 3. identity; transfer column w2 (-t_s on the surface row, +t_s on its bulk partner)
+    for (int i = tid(); i < neq; i += NT) {
+        const int d = N.nw_diag_slot[i];
+        if (d >= 0) V[d] = (LUT)((double)V[d] + 1.0);
+    }
+    const double S = sc[S_SPRE];
+    const bool neg = S < 0;
+    const bool ratio_lt1 = (sbk / sm) < 1.0;
+    for (int q = tid(); q < N.nsurfl; q += NT) {
+        const int s_ = N.surface_list[q], b = N.bulk_partner_of_surface[s_];
+        const double t = neg ? (ratio_lt1 ? y[b] / sm : y[b] / sbk) : SURFACE_COVERAGE0 * y[s_];
+        V[N.nw_s2c[s_]] = (LUT)(-t);
+        V[N.nw_s2c[b]] = (LUT)t;
+    }
+    __syncthreads();
+    // 4. photo-rate columns (H2, CO, density) and the free-fall density row; transfer S-terms
+    {
+        const double cs_ = sc[S_CS], G0 = L.prm[P_RADFIELD], av = sc[S_AV];
+        const double h2col = 0.5 * y[N.nh2] * D * cs_, cocol = 0.5 * y[N.nco] * D * cs_;
+        const double eps = 1e-6;
+        const int t = tid();
+        if (t < 6) {
+            const double h = (t == 1 || t == 4) ? h2col * (1 + eps) : h2col;
+            const double c = (t == 5) ? cocol * (1 + eps) : cocol;
+            scr[t] = (t < 2) ? h2_photodiss_dev(h, G0, sc[S_SCAT]) : (t >= 3 ? co_photodiss_dev(h, c, G0, av) : 0.0);
+        }
+    }
+    __syncthreads();
+    if (tid() == 0) {
+        const double cs_ = sc[S_CS];
+        const double h2col = 0.5 * y[N.nh2] * D * cs_, cocol = 0.5 * y[N.nco] * D * cs_;
+        const double eps = 1e-6;
+        const double kh = scr[0];
+        const double dkh = h2col > 0 ? (scr[1] - kh) / (h2col * eps) : 0.0;
+        const double kc = scr[3];
+        const double dkch = h2col > 0 ? (scr[4] - kc) / (h2col * eps) : 0.0;
+        const double dkcc = cocol > 0 ? (scr[5] - kc) / (cocol * eps) : 0.0;
+        int m3 = 0;
+        for (int which = 0; which < 2; which++) {
+            const int k = which ? N.r_co_hv : N.r_h2_hv;
+            const int *fc = N.fac + MAXFAC * k;
+            double pf = 1.0;
+            for (int mm = 0; mm < N.nfac[k]; mm++) pf *= fac_value(fc[mm], y, D, blr, ism, tsm);
+            const double d_h2 = which ? dkch * 0.5 * D * cs_ : dkh * 0.5 * D * cs_;
+            const double d_co = which ? dkcc * 0.5 * D * cs_ : 0.0;
+            const double d_d = which ? (dkch * 0.5 * y[N.nh2] * cs_ + dkcc * 0.5 * y[N.nco] * cs_)
+                                     : dkh * 0.5 * y[N.nh2] * cs_;
+            for (int q = N.aff_ptr[k]; q < N.aff_ptr[k + 1]; q++, m3 += 3) {
+                const double c = N.aff_cf[q] * pf;
+                const int *ps = N.nw_ph_slot + m3;
+                V[ps[0]] = (LUT)((double)V[ps[0]] - gamma * c * d_h2);
+                V[ps[1]] = (LUT)((double)V[ps[1]] - gamma * c * d_co);
+                V[ps[2]] = (LUT)((double)V[ps[2]] - gamma * c * d_d);
+            }
+        }
+        const double dn = D * 1e-7;
+        const int dd = N.nw_diag_slot[neq - 1];
+        V[dd] = (LUT)((double)V[dd] - gamma * (densdot_dev(L.prm, D + dn) - densdot_dev(L.prm, D)) / dn);
+    }
+    for (int q = tid(); q < N.nsurfl; q += NT) {
+        const int s_ = N.surface_list[q], b = N.bulk_partner_of_surface[s_];
+        double d[4] = {0.0, 0.0, 0.0, 0.0};   // S * dt_s/dy over columns (s, b, SURFACE, BULK)
+        if (neg) {
+            d[1] = S * (ratio_lt1 ? 1.0 / sm : 1.0 / sbk);
+            if (ratio_lt1 && ysur > 1e-30) d[2] = S * (-y[b] / (sm * sm));
+            if (!ratio_lt1 && ybulk > 1e-30) d[3] = S * (-y[b] / (sbk * sbk));
+        } else {
+            d[0] = S * SURFACE_COVERAGE0;
+        }
+        const int *ts_ = N.nw_tr_slot + 8 * q;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            if (d[m] != 0.0) {   // J[s][c] -= d, J[b][c] += d  ->  M = I - gamma J
+                V[ts_[m]] = (LUT)((double)V[ts_[m]] + gamma * d[m]);
+                V[ts_[4 + m]] = (LUT)((double)V[ts_[4 + m]] - gamma * d[m]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Crout elimination of the sparse pivots in plan phases (one thread per target entry,
+// dependencies only on earlier phases), then the blocked dense LU of the trailing block.
+template <typename LUT>
+__device__ int sparse_lu(const DevNet &N, LUT *V, double *red, double *stage)
+{
+    if (tid() == 0) red[9] = 0.0;
+    __syncthreads();
+    for (int p = 0; p < N.nw_nph; p++) {
+        const int t1 = N.nw_ph_ptr[p + 1];
+        for (int t = N.nw_ph_ptr[p] + tid(); t < t1; t += NT) {
+            const int tg = N.nw_tg_slot[t];
+            double acc = (double)V[tg];
+            const int q1 = N.nw_tg_ptr[t + 1];
+            for (int q = N.nw_tg_ptr[t]; q < q1; q++) acc -= (double)V[N.nw_pr_l[q]] * (double)V[N.nw_pr_u[q]];
+            const int dv = N.nw_tg_div[t];
+            if (dv >= 0) {
+                const double piv = (double)V[dv];
+                if (!(piv == piv) || piv == 0.0 || isinf(piv)) red[9] = 1.0;
+                acc = acc / piv;
+            }
+            V[tg] = (LUT)acc;
+        }
+        __syncthreads();
+    }
+    for (int r = tid(); r < N.nw_ns; r += NT) {
+        const double piv = (double)V[N.nw_sp_dpos[r]];
+        if (!(piv == piv) || piv == 0.0 || isinf(piv)) red[9] = 1.0;
+    }
+    __syncthreads();
+    if (red[9] != 0.0) return 1;
+    if (N.nw_T > 0) return block_lu<LUT>(V + N.nw_nsp, N.nw_T, N.nw_T, red, stage);
+    return 0;
+}
+
+// Solve M x = b with the factors of sparse_lu: b (length neq, in LDS) is overwritten by x.
+// w is LDS scratch of at least neq + 2 doubles.
+template <typename LUT>
+__device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *V, double *b, double *w)
+{
+    const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns;
+    double sb = 0.0, ss = 0.0;
+    for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
+    for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
+    block_sum2(sb, ss, L.red);
+    for (int i = tid(); i < NN; i += NT) {
+        double v = 0.0;
+        if (i < neq) {
+            v = b[i];
+            if (i == N.nbulk) v -= sb;
+            else if (i == N.nsurface) v -= ss;
+        }
+        w[N.nw_inv[i]] = v;
+    }
+    __syncthreads();
+    for (int l = 0; l < N.nw_nlev; l++) {   // forward, sparse rows by level
+        const int q1 = N.nw_lev_ptr[l + 1];
+        for (int q = N.nw_lev_ptr[l] + tid(); q < q1; q += NT) {
+            const int r = N.nw_lev_rows[q];
+            double acc = w[r];
+            const int e1 = N.nw_sp_dpos[r];
+            for (int e = N.nw_sp_ptr[r]; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
+            w[r] = acc;
+        }
+        __syncthreads();
+    }
+    for (int r = ns + tid(); r < NN; r += NT) {   // tail rows: L21
+        double acc = w[r];
+        const int e1 = N.nw_sp_ptr[r + 1];
+        for (int e = N.nw_sp_ptr[r]; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
+        w[r] = acc;
+    }
+    __syncthreads();
+    if (N.nw_T > 0) block_lu_solve<LUT>(V + N.nw_nsp, N.nw_T, N.nw_T, w + ns);
+    for (int l = N.nw_nlev - 1; l >= 0; l--) {   // backward, sparse rows by level
+        const int q1 = N.nw_lev_ptr[l + 1];
+        for (int q = N.nw_lev_ptr[l] + tid(); q < q1; q += NT) {
+            const int r = N.nw_lev_rows[q];
+            const int dp = N.nw_sp_dpos[r];
+            double acc = w[r];
+            const int e1 = N.nw_sp_ptr[r + 1];
+            for (int e = dp + 1; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
+            w[r] = acc / (double)V[dp];
+        }
+        __syncthreads();
+    }
+    for (int i = tid(); i < neq; i += NT) b[i] = w[N.nw_inv[i]];
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------- BDF ops policy
 
 template <typename LUT>
@@ -783,7 +1047,8 @@ struct GpuOps {
     int neq;
     double z[6][EPT], acor[EPT], ewt[EPT], atol[EPT], f0[EPT];
     uint64_t t_rhs = 0, t_jac = 0, t_lu = 0, t_sol = 0;   // device wall-clock ticks per phase
-    bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M
+    bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M (dense form only)
+    bool sparse = true;         // static-pattern sparse plan (else dense LU of the full matrix)
 
     __device__ GpuOps(const DevNet &n_, Lds &l_, LUT *m_) : N(n_), L(l_), M(m_), neq(n_.neq) {}
 
@@ -945,7 +1210,8 @@ struct GpuOps {
     __device__ void solve()
     {
         uint64_t t0 = wall_clock64();
-        block_lu_solve<LUT>(M, neq, N.ld, L.b);
+        if (sparse) sparse_lu_solve<LUT>(N, L, M, L.b, L.flux);
+        else block_lu_solve<LUT>(M, neq, N.ld, L.b);
         t_sol += wall_clock64() - t0;
     }
     __device__ double apply_correction(double scale)
@@ -972,9 +1238,17 @@ struct GpuOps {
             if (own(e)) L.g[idx(e)] = equilibrate ? ewt[e] : 1.0;
         __syncthreads();
         uint64_t t0 = wall_clock64();
-        block_newton_matrix<LUT>(N, L, M, gamma, true, equilibrate);
-        uint64_t t1 = wall_clock64();
-        int r = block_lu<LUT>(M, neq, N.ld, L.red, L.flux);
+        int r;
+        uint64_t t1;
+        if (sparse) {
+            sparse_newton_matrix<LUT>(N, L, M, gamma);
+            t1 = wall_clock64();
+            r = sparse_lu<LUT>(N, M, L.red, L.flux);
+        } else {
+            block_newton_matrix<LUT>(N, L, M, gamma, true, equilibrate);
+            t1 = wall_clock64();
+            r = block_lu<LUT>(M, neq, N.ld, L.red, L.flux);
+        }
         uint64_t t2 = wall_clock64();
         t_jac += t1 - t0;
         t_lu += t2 - t1;
@@ -1038,7 +1312,8 @@ struct SolveArgs {
     void *ws;                 // slot workspaces
     size_t ws_stride;         // bytes per slot
     int *counter;             // work queue head
-    int equilibrate;          // factor the error-weight-equilibrated Newton matrix
+    int equilibrate;          // factor the error-weight-equilibrated Newton matrix (dense form)
+    int sparse;               // 1: static-pattern sparse plan, 0: dense LU
 };
 
 __device__ void elemental_start(const DevNet &N, const double *prm, double *y, double density)
@@ -1140,7 +1415,8 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
             elemental_start(N, prm, L.y, L.sc[S_DENS]);
         }
         GpuOps<LUT> o(N, L, M);
-        o.equilibrate = A.equilibrate != 0;
+        o.sparse = A.sparse != 0;
+        o.equilibrate = !o.sparse && A.equilibrate != 0;
         bdf::Stats st;
         int row = 0, nint = 0, nretry = 0, flag = 0;
         record_row(N, L, A, pidx, row++, 0.0);
@@ -1263,6 +1539,7 @@ struct PieceArgs {
     double *scal_out;       // [n][8]
     double *jac_out;        // [n][neq*neq] column-major, J (not M)
     double gamma;
+    int sparse;
     void *ws;
     size_t ws_stride;
 };
@@ -1369,11 +1646,20 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
     for (int i = tid(); i < N.neq; i += NT) L.g[i] = 1.0;
     __syncthreads();
     LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
-    block_newton_matrix<LUT>(N, L, M, A.gamma);
-    int bad = block_lu<LUT>(M, N.neq, N.ld, L.red, L.flux);
+    int bad;
+    if (A.sparse) {
+        sparse_newton_matrix<LUT>(N, L, M, A.gamma);
+        bad = sparse_lu<LUT>(N, M, L.red, L.flux);
+    } else {
+        block_newton_matrix<LUT>(N, L, M, A.gamma);
+        bad = block_lu<LUT>(M, N.neq, N.ld, L.red, L.flux);
+    }
     for (int i = tid(); i < N.neq; i += NT) L.b[i] = A.scal_out[(size_t)p * N.neq + i];
     __syncthreads();
-    if (!bad) block_lu_solve<LUT>(M, N.neq, N.ld, L.b);
+    if (!bad) {
+        if (A.sparse) sparse_lu_solve<LUT>(N, L, M, L.b, L.flux);
+        else block_lu_solve<LUT>(M, N.neq, N.ld, L.b);
+    }
     for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = bad ? NAN : L.b[i];
 }
 

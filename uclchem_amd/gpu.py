@@ -31,7 +31,8 @@ class _Desc(ctypes.Structure):
                   "bulk_partner_of_surface", "surface_partner_of_bulk", "freeze_partners",
                   "is_surface", "is_bulk", "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c",
                   "aff_ptr", "aff_sp", "aff_cf", "sw_ptr", "sw_k", "sw_c"]] +
-                [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 9)])
+                [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 9)] +
+                [("nw_sizes", ctypes.c_int * 10)] + [(n, ctypes.c_void_p) for n in mh.DESC_NW])
 
 
 class GpuError(RuntimeError):
@@ -81,7 +82,11 @@ class Network:
         self._keep = arrays
         d = _Desc()
         for k, v in sizes.items():
-            setattr(d, k, v)
+            if k == "nw_sizes":
+                for i in range(10):
+                    d.nw_sizes[i] = int(v[i])
+            else:
+                setattr(d, k, v)
         for k, v in arrays.items():
             setattr(d, k, _p(v).value)
         for i in range(25):
@@ -91,6 +96,12 @@ class Network:
         h = ctypes.c_int(-1)
         _check(lib().ucg_network_load(ctypes.byref(d), ctypes.byref(h)), "ucg_network_load")
         self.handle = h.value
+
+    SPARSE, DENSE = 0, 1
+
+    def set_linear_solver(self, kind: int) -> None:
+        """Network.SPARSE (static-pattern plan, default) or Network.DENSE (full LU)."""
+        _check(lib().ucg_set_linear_solver(ctypes.c_int(self.handle), ctypes.c_int(kind)), "ucg_set_linear_solver")
 
     # ------------------------------------------------------------ full solves
     def solve(self, params: np.ndarray, y_init: Optional[np.ndarray] = None, max_rec: int = 0,

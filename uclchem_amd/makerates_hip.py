@@ -543,7 +543,10 @@ def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
         scalars[k] = int(v)
     for k, v in spec_idx.items():
         scalars[k] = int(v)
-    return NetworkTables(name, names, nspec, neq, nreac, arrays, scalars)
+    t = NetworkTables(name, names, nspec, neq, nreac, arrays, scalars)
+    from .newton_plan import build_plan
+    t.arrays.update(build_plan(t))      # static-pattern sparse Newton factorisation
+    return t
 
 
 # ---------------------------------------------------------------- static rate constants
@@ -696,6 +699,10 @@ DESC_I32 = ["rtype", "re", "pr", "ice_kind", "n_d", "ops", "fac", "nfac", "type_
             "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c", "aff_ptr", "aff_sp", "aff_cf",
             "sw_ptr", "sw_k", "sw_c"]
 DESC_I8 = ["extrapolate"]
+DESC_NW = ["nw_perm", "nw_inv", "nw_sp_ptr", "nw_sp_col", "nw_sp_dpos", "nw_lev_ptr", "nw_lev_rows", "nw_ph_ptr",
+           "nw_tg_slot", "nw_tg_ptr", "nw_tg_div", "nw_pr_l", "nw_pr_u", "nw_jnz_slot", "nw_diag_slot",
+           "nw_ph_slot", "nw_tr_slot", "nw_s1c", "nw_s1r", "nw_s2c", "nw_s2r", "nw_gs_ptr", "nw_gs_slot",
+           "nw_misc", "nw_nbrow", "nw_nsrow"]
 
 
 def device_descriptor(t: NetworkTables):
@@ -728,12 +735,15 @@ def device_descriptor(t: NetworkTables):
         else:
             arrays[n] = np.ascontiguousarray(a[n], dtype=np.int32)
     arrays["extrapolate"] = np.ascontiguousarray(a["extrapolate"], dtype=np.int8)
+    for n in DESC_NW:
+        arrays[n] = np.ascontiguousarray(a[n], dtype=np.int32)
     sizes = dict(nspec=t.nspec, neq=neq, nreac=t.nreac, nice=len(a["ice_list"]),
                  nsurfl=len(a["surface_list"]), nbulkl=len(a["bulk_list"]), nfp=len(a["freeze_partners"]),
                  nswap=len(a["swap_list"]), nloss=len(a["loss_idx"]), nprod=len(a["prod_idx"]),
                  nnz=len(a["jnz_row"]), ncontrib=len(a["jc_k"]), naff=len(a["aff_sp"]), nsw=len(a["sw_k"]))
     special_species = np.array([t.scalars[k] for k in SPECIAL_SPECIES_ORDER], dtype=np.int32)
     special_reactions = np.array([t.scalars[k] for k in SPECIAL_REACTION_ORDER], dtype=np.int32)
+    sizes["nw_sizes"] = np.ascontiguousarray(a["nw_sizes"], dtype=np.int32)
     return sizes, arrays, special_species, special_reactions
 
 
@@ -750,8 +760,8 @@ def save_network_bin(t: NetworkTables, path: str) -> None:
     and zero padding to a multiple of 8 bytes."""
     sizes, arrays, ssp, srx = device_descriptor(t)
     recs = [("sizes", np.array([sizes[k] for k in DESC_SIZES], dtype=np.int32)),
-            ("special_species", ssp), ("special_reactions", srx)]
-    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8]
+            ("special_species", ssp), ("special_reactions", srx), ("nw_sizes", sizes["nw_sizes"])]
+    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8 + DESC_NW]
     import struct
     with open(path, "wb") as f:
         f.write(BIN_MAGIC)

@@ -64,6 +64,8 @@ extern "C" {
 #define UCG_E_HIP -101
 #define UCG_E_ARG -102
 #define UCG_E_HANDLE -103
+/* per-point status of a point a cancelled batch did not finish (ucg_cancel) */
+#define UCG_E_CANCELLED -10
 
 /* Network tables as emitted by the MakeRates backend (uclchem_amd/makerates_hip.py).
  * All pointers are host pointers, copied to the device by ucg_network_load. */
@@ -123,8 +125,16 @@ int ucg_solve_batch_device(int handle, int n, const double *params, const double
                            int max_rec, double *rec, int *n_rec, int *status, int *stats,
                            double *t_start, double *t_done, int lu_bits, int n_slots, void *stream);
 
-/* ucg_solve_batch with per-point device timers [n][8] in seconds: rates, RHS, Jacobian
- * assembly, LU, triangular solves, total (no output rows).  n_slots as above. */
+/* Stop the running ucg_solve_batch_device call(s) on a handle: workgroups finish their
+ * current integrator step, take no further points, and unfinished points report
+ * UCG_E_CANCELLED.  Callable from another host thread while the solve runs.  No
+ * counterpart in the reference (its grids run until every point ends); for serving
+ * loops and time-bounded batches. */
+int ucg_cancel(int handle);
+
+/* ucg_solve_batch with per-point device timers [n][8]: seconds in rates, RHS, Jacobian
+ * assembly, LU, triangular solves, total; then the mean shader clock over the point's
+ * solve in MHz (no output rows).  n_slots as above. */
 int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,
                              double *timers, int lu_bits, int n_slots);
 

@@ -136,3 +136,27 @@ def test_invalid_parameters_rejected(gpu_net):
     p = make_params(initialdens=-1.0)
     with pytest.raises(GpuError):
         gpu_net.solve(p[None])
+
+
+def test_cancel_stops_a_running_batch(gpu_net):
+    """ucg_cancel from the host while a device solve runs: the kernel drains, every point is
+    either finished (status 0) or reports CANCELLED, and the handle stays usable."""
+    import time
+    import torch
+    from uclchem_amd import workloads
+    params = workloads.static_cloud_grid(64)
+    dev = torch.device("cuda:0")
+    dp = torch.tensor(params, device=dev)
+    dy = torch.zeros((len(params), gpu_net.neq), dtype=torch.float64, device=dev)
+    dst = torch.zeros(len(params), dtype=torch.int32, device=dev)
+    dstats = torch.zeros((len(params), 8), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    gpu_net.solve_device(dp, dy, dst, dstats, stream=stream.cuda_stream)
+    time.sleep(1.0)
+    gpu_net.cancel()
+    torch.cuda.synchronize()
+    st = dst.cpu().numpy()
+    assert set(st.tolist()) <= {0, gpu_net.CANCELLED}
+    assert (st == gpu_net.CANCELLED).any()
+    again = gpu_net.solve(params[:2])
+    assert np.all(again["status"] == 0)

@@ -19,7 +19,8 @@
 // t0 past tout, interpolate back to tout, report DVODE-style states:
 //   2 success, -1 step limit, -2 tolerance too small, -4 repeated error-test
 //   failures, -5 repeated convergence failures, -3 illegal input,
-//   -6 singular/invalid Newton matrix or non-finite state.
+//   -6 singular/invalid Newton matrix or non-finite state, -7 cancelled by the caller
+//   (Ops::cancelled(), checked once per step).
 // On a negative return the state is the last accepted step (t = tn, y = zn[0]).
 #pragma once
 
@@ -448,30 +449,32 @@ BDF_HD int integrate(Ops &o, double &t, double tout, double rtol, int mxstep, St
     s.jcur = false;
     s.etaq = s.etaqm1 = s.etaqp1 = 0.0;
     Stats local;                         // per-call counters drive the step logic
+    int ret = 2;
     for (;;) {
         if (s.nst_call > 0) {
-            if (!o.set_ewt(rtol)) { t = s.tn; o.finish_at_tn(); return -3; }
+            if (!o.set_ewt(rtol)) { ret = -3; break; }
         }
-        if (s.nst_call >= mxstep) { t = s.tn; o.finish_at_tn(); return -1; }
+        if (s.nst_call >= mxstep) { ret = -1; break; }
+        if (o.cancelled()) { ret = -7; break; }   // the caller asked the whole batch to stop
         double tolsf = UROUND * o.wrms(Z0);
-        if (tolsf > 1.0) { t = s.tn; o.finish_at_tn(); return -2; }
+        if (tolsf > 1.0) { ret = -2; break; }
         int k = step(o, s, local);
-        if (k < 0) {
-            st.nst += local.nst; st.nfe += local.nfe; st.nje += local.nje; st.nsetups += local.nsetups;
-            st.nni += local.nni; st.ncfn += local.ncfn; st.netf += local.netf;
-            t = s.tn;
-            o.finish_at_tn();
-            return k;
-        }
+        if (k < 0) { ret = k; break; }
         if (s.tn >= tout) {
             o.interpolate((tout - s.tn) / s.h, s.q);   // Z0 <- y(tout)
-            t = tout;
             break;
         }
     }
+    // every exit accounts the call's work into the caller's counters
     st.nst += local.nst; st.nfe += local.nfe; st.nje += local.nje; st.nsetups += local.nsetups;
     st.nni += local.nni; st.ncfn += local.ncfn; st.netf += local.netf;
-    return 2;
+    if (ret == 2) {
+        t = tout;
+    } else {
+        t = s.tn;
+        o.finish_at_tn();
+    }
+    return ret;
 }
 
 }  // namespace bdf

@@ -3,6 +3,7 @@
 #include "ucg_solver.hip"
 #include "../../include/uclchem_gpu.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -37,6 +38,8 @@ struct NetHandle {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     int *counter = nullptr;
+    int *cancel = nullptr;              // device flag read by running solves
+    hipStream_t cancel_stream = nullptr;
     int linear_solver = UCG_LINSOL_SPARSE;
 };
 std::vector<NetHandle> g_nets;
@@ -125,8 +128,8 @@ NetHandle *get(int handle)
 size_t lds_bytes(const ucg::DevNet &N)
 {
     const int npad = (N.neq + 7) & ~7;
-    const int flux = N.nreac > 2 * npad ? N.nreac : 2 * npad;
-    return sizeof(double) * ((size_t)N.nreac + flux + 4 * npad + 16 + ucg::S_NSCAL + ucg::P_NPARAM);
+    const int region = ucg::flux_region_len(N);
+    return sizeof(double) * ((size_t)region + 4 * npad + 16 + ucg::S_NSCAL + ucg::P_NPARAM);
 }
 
 int ensure_ws(NetHandle &h, size_t bytes)
@@ -187,11 +190,15 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     if (n_slots > n) n_slots = n;
     const size_t elem = lu_bits == 64 ? sizeof(double) : sizeof(float);
     const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
-    size_t stride = (sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem;
-    stride = (stride + 255) & ~(size_t)255;
-    int rc = ensure_ws(h, stride * n_slots);
+    size_t mat = (sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem;
+    mat = (mat + 255) & ~(size_t)255;
+    const size_t stride = mat + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
+    // sized for the default slot count at least, so later calls with more points do not
+    // reallocate between launches
+    int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
     if (rc) return rc;
-    HIPCHK(hipMemsetAsync(h.counter, 0, sizeof(int), stream));
+    hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, h.counter, h.cancel,
+                       status, n);
     ucg::SolveArgs A;
     A.n_points = n;
     A.params = params;
@@ -209,7 +216,9 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.sparse = sparse ? 1 : 0;
     A.ws = h.ws;
     A.ws_stride = stride;
+    A.ws_mat_bytes = mat;
     A.counter = h.counter;
+    A.cancel = h.cancel;
     const size_t lds = lds_bytes(N);
     if (lu_bits == 64) {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
@@ -237,10 +246,12 @@ int piece_common(NetHandle &h, int n, const double *params, const double *phys, 
     A.n = n;
     A.params = (const double *)dp.p;
     A.phys = (const double *)dph.p;
-    size_t stride = ((size_t)N.ld * N.neq * sizeof(double) + 255) & ~(size_t)255;
+    const size_t mat = ((size_t)N.ld * N.neq * sizeof(double) + 255) & ~(size_t)255;
+    const size_t stride = mat + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
     if ((rc = ensure_ws(h, stride * n))) return rc;
     A.ws = h.ws;
     A.ws_stride = stride;
+    A.ws_mat_bytes = mat;
     return 0;
 }
 
@@ -459,6 +470,30 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     UP(alpha, R); UP(beta, R); UP(gama, R); UP(tmin, R); UP(tmax, R); UP(mass, (size_t)d->nspec);
     UP(rstat, 5 * R); UP(rtype, R); UP(re, 3 * R); UP(pr, 4 * R); UP(ice_kind, R); UP(n_d, R);
     UP(ops, ucg::MAXOPS * R); UP(fac, ucg::MAXFAC * R); UP(nfac, R); UP(extrapolate, R);
+    {   // factor records for the term partials: int16 codes x 8, offsets, packed contributions
+        std::vector<int> facp(4 * R, 0), foff(R + 1, 0), jpk((size_t)d->ncontrib);
+        for (size_t k = 0; k < R; k++) {
+            const int nf = d->nfac[k];
+            if (nf < 0 || nf > 8) return fail(UCG_E_ARG, "a reaction term has more than 8 factors");
+            foff[k + 1] = foff[k] + nf;
+            for (int m = 0; m < 8; m++) {
+                const int c = m < nf ? d->fac[ucg::MAXFAC * k + m] : ucg::OP_NONE;
+                if (c < ucg::FAC_D || c >= d->neq) return fail(UCG_E_ARG, "factor code out of range");
+                facp[4 * k + m / 2] |= (int)((unsigned)(c & 0xffff) << (16 * (m & 1)));
+            }
+        }
+        for (int q = 0; q < d->ncontrib; q++) {
+            const int k = d->jc_k[q], m = d->jc_m[q], c = d->jc_c[q];
+            if (k < 0 || k >= d->nreac || m < 0 || m >= d->nfac[k] || c < -8 || c > 7)
+                return fail(UCG_E_ARG, "Jacobian contribution out of range");
+            jpk[q] = ((foff[k] + m) << 4) | (c + 8);
+        }
+        N.npart = foff[R];
+        N.npart_pad = (N.npart + 7) & ~7;
+        if ((rc = upload(h, facp.data(), facp.size(), N.facp))) return rc;
+        if ((rc = upload(h, foff.data(), foff.size(), N.fac_off))) return rc;
+        if ((rc = upload(h, jpk.data(), jpk.size(), N.jc_pk))) return rc;
+    }
     {   // packed term records {n_d, ops[6], nfac}
         std::vector<int> prog(8 * R);
         for (size_t k = 0; k < R; k++) {
@@ -516,12 +551,16 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     }
     HIPCHK(hipMalloc(&h.counter, sizeof(int)));
     h.allocs.push_back(h.counter);
+    HIPCHK(hipMalloc(&h.cancel, sizeof(int)));
+    h.allocs.push_back(h.cancel);
+    HIPCHK(hipMemset(h.cancel, 0, sizeof(int)));
     size_t lds = lds_bytes(N);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, g_device));
     if (lds > (size_t)prop.sharedMemPerBlock)
         return fail(UCG_E_ARG, "network needs " + std::to_string(lds) + " B of LDS per workgroup, device allows " +
                                    std::to_string(prop.sharedMemPerBlock));
+    HIPCHK(hipStreamCreateWithFlags(&h.cancel_stream, hipStreamNonBlocking));   // last: nothing fails after it
     return 0;
 }
 
@@ -532,7 +571,22 @@ int ucg_network_free(int handle)
     if (!h) return fail(UCG_E_HANDLE, "bad network handle");
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->ws) (void)hipFree(h->ws);
+    if (h->cancel_stream) (void)hipStreamDestroy(h->cancel_stream);
     *h = NetHandle();
+    return 0;
+}
+
+int ucg_cancel(int handle)
+{
+    NetHandle *h;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        h = get(handle);
+        if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    }
+    static const int one = 1;
+    HIPCHK(hipMemcpyAsync(h->cancel, &one, sizeof(int), hipMemcpyHostToDevice, h->cancel_stream));
+    HIPCHK(hipStreamSynchronize(h->cancel_stream));
     return 0;
 }
 

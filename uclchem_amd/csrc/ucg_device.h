@@ -51,6 +51,9 @@ struct DevNet {
     const double *alpha, *beta, *gama, *tmin, *tmax, *mass, *rstat;  // rstat [nreac*5]
     const int32_t *rtype, *re, *pr, *ice_kind, *n_d, *ops, *fac, *nfac;
     const int32_t *prog;   // [nreac*8] {n_d, ops[6], nfac}, 32-byte aligned records
+    const int32_t *facp;   // [nreac*4] factor codes as int16 x 8 (density factors first), -1 padded
+    const int32_t *fac_off, *jc_pk;   // partial offsets [nreac+1]; contributions (partial << 4 | coef + 8)
+    int32_t npart, npart_pad;         // number of partials; LDS offset of the scratch after them
     const int8_t *extrapolate;
     const int32_t *type_ranges;                                     // [NTYPES*2]
     const int32_t *loss_ptr, *loss_idx, *prod_ptr, *prod_idx, *swap_list;
@@ -70,6 +73,20 @@ struct DevNet {
     const int32_t *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
     const int32_t *nw_misc, *nw_nbrow, *nw_nsrow;
 };
+
+// length (in doubles) of the LDS region shared by the reaction fluxes (RHS), the term
+// partials plus two NEQ-long scratch vectors (Jacobian assembly), and the staged factors
+// (all of them in FP32, the dense tail in FP64) plus the permuted work vector
+__host__ __device__ inline int flux_region_len(const DevNet &N)
+{
+    const int npad = (N.neq + 7) & ~7, Npad = (N.nw_N + 7) & ~7;
+    const int a = N.npart_pad + 2 * npad;
+    const int b = N.nreac > a ? N.nreac : a;
+    const int t64 = N.nw_T * N.nw_T + Npad;                    // FP64: staged dense tail
+    const int f32 = (((N.nw_nval + 1) / 2 + 7) & ~7) + Npad;   // FP32: all factors + work vector
+    const int c = t64 > f32 ? t64 : f32;
+    return b > c ? b : c;
+}
 
 // static spline tables of the photo-rate functions (photoreactions.f90:14-39),
 // second derivatives precomputed once on the host

@@ -23,6 +23,15 @@ __constant__ PhotoTables c_photo;
 enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI, S_DENS, S_TGAS, S_TDUST,
        S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_NSCAL = 24 };
 
+// Large collective routines can be kept out of line to bound the kernel's register
+// allocation (UCG_BIG=__attribute__((noinline))); inlined by default.
+#ifndef UCG_BIG
+#define UCG_BIG
+#endif
+#ifndef UCG_WAVES_PER_EU
+#define UCG_WAVES_PER_EU 1
+#endif
+
 struct Lds {
     double *rate, *flux, *y, *f, *b, *g, *red, *sc, *prm;
 };
@@ -94,7 +103,7 @@ __device__ inline double freeze_out(const DevNet &N, const double *prm, int j, d
 // Rate coefficients for the current interval attempt (rates.f90:21-343 preceded by the
 // ice bookkeeping of chemistry.f90:167-200).  Reads y (LDS), writes rate (LDS), updates
 // y[BULK], y[SURFACE], the column densities, the ice scalars and phi.
-__device__ void block_rates(const DevNet &N, Lds &L)
+UCG_BIG __device__ void block_rates(const DevNet &N, Lds &L)
 {
     const int neq = N.neq;
     const double *prm = L.prm;
@@ -269,19 +278,30 @@ __device__ void block_rates(const DevNet &N, Lds &L)
 
 // value of one reaction's ODE term, tokens in the order MakeRates emits them.
 // prog[k] packs {n_d, op0..op5, nfac} into two 16-byte words.
-__device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const double *y,
-                                             double D, double blr, double TS, double sm)
+// Factor codes of one reaction from its 16-byte record (int16 x 8, density factors first,
+// then the MakeRates tokens in order, -1 padded).
+__device__ __forceinline__ void decode_codes(const int4 w, int c[8])
 {
-    const int4 *pk = reinterpret_cast<const int4 *>(N.prog) + 2 * k;
-    const int4 w0 = pk[0], w1 = pk[1];
-    const int op[MAXOPS] = {w0.y, w0.z, w0.w, w1.x, w1.y, w1.z};
-    double v = rate[k];
-    for (int q = 0; q < w0.x; q++) v = v * D;
+    const int wd[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int q = 0; q < MAXOPS; q++) {
-        const int o = op[q];
-        if (o == OP_NONE) break;
-        if (o >= 0) v = v * y[o];
+    for (int h = 0; h < 4; h++) {
+        c[2 * h] = (int)(short)(wd[h] & 0xffff);
+        c[2 * h + 1] = wd[h] >> 16;   // arithmetic shift keeps the sign of the high half
+    }
+}
+
+// Value of one reaction's ODE term, factors applied in the order MakeRates emits them
+// (rate * D^b * y... with /safeMantle, *bulkLayersReciprocal, *totalSwap/safeMantle).
+__device__ __forceinline__ double term_from_codes(const int c[8], double r, const double *y, double D, double blr,
+                                                  double TS, double sm)
+{
+    double v = r;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const int o = c[m];
+        if (o == OP_NONE) continue;
+        if (o == FAC_D) v = v * D;
+        else if (o >= 0) v = v * y[o];
         else if (o == OP_BLR) v = v * blr;
         else if (o == OP_SWAP) v = v * TS / sm;
         else v = v / sm;
@@ -289,9 +309,17 @@ __device__ __forceinline__ double term_value(const DevNet &N, int k, const doubl
     return v;
 }
 
+__device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const double *y,
+                                             double D, double blr, double TS, double sm)
+{
+    int c[8];
+    decode_codes(reinterpret_cast<const int4 *>(N.facp)[k], c);
+    return term_from_codes(c, rate[k], y, D, blr, TS, sm);
+}
+
 // RHS f(y) (chemistry.f90:294-352 + the generated GETYDOT, odes.f90).  Input L.y, output
 // L.f; also leaves the pre-transfer surface total in sc[S_SPRE] for the Jacobian.
-__device__ void block_rhs(const DevNet &N, Lds &L)
+UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L)
 {
     const int neq = N.neq;
     double *y = L.y, *f = L.f, *rate = L.rate, *flux = L.flux, *sc = L.sc;
@@ -314,7 +342,31 @@ __device__ void block_rhs(const DevNet &N, Lds &L)
     double ts = 0.0;
     for (int q = tid(); q < N.nswap; q += NT) ts += term_value(N, N.swap_list[q], rate, y, D, blr, 0.0, sm);
     const double TS = block_sum(ts, L.red);
-    for (int k = tid(); k < N.nreac; k += NT) flux[k] = term_value(N, k, rate, y, D, blr, TS, sm);
+    {   // fluxes, four reactions per thread per pass with their records loaded up front
+        const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
+        const int nr = N.nreac;
+        for (int k0 = tid(); k0 < nr; k0 += 4 * NT) {
+            int4 w[4];
+            double r[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u * NT;
+                if (k < nr) {
+                    w[u] = fp[k];
+                    r[u] = rate[k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u * NT;
+                if (k < nr) {
+                    int c[8];
+                    decode_codes(w[u], c);
+                    flux[k] = term_from_codes(c, r[u], y, D, blr, TS, sm);
+                }
+            }
+        }
+    }
     __syncthreads();
     double pre[EPT];
     double spre = 0.0;
@@ -394,17 +446,59 @@ __device__ __forceinline__ double fac_deriv(int c, double dblr, double dism, dou
     return dtsm;
 }
 
+// Partial derivatives of every reaction term with respect to each of its factors:
+// P[fac_off[k] + m] = rate_k * d(factor m) * prod_{m' != m} factor m'.  One 16-byte
+// record per reaction holds its (up to 8) factor codes as int16, -1 padded.
+__device__ void block_term_partials(const DevNet &N, const double *rate, const double *y, double *P, double D,
+                                    double blr, double ism, double tsm, double dblr, double dism, double dtsm)
+{
+    const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
+    for (int k = tid(); k < N.nreac; k += NT) {
+        int c[8];
+        decode_codes(fp[k], c);
+        double v[8], pre[8];
+        double acc = 1.0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            v[m] = (c[m] == OP_NONE) ? 1.0 : fac_value(c[m], y, D, blr, ism, tsm);
+            pre[m] = acc;
+            acc *= v[m];
+        }
+        const double r = rate[k];
+        const int off = N.fac_off[k];
+        double suf = 1.0;
+#pragma unroll
+        for (int m = 7; m >= 0; m--) {
+            if (c[m] != OP_NONE) P[off + m] = r * fac_deriv(c[m], dblr, dism, dtsm) * pre[m] * suf;
+            suf *= v[m];
+        }
+    }
+}
+
+// Value of structural nonzero n of dJ (generated part): sum of coefficient * partial.
+__device__ __forceinline__ double jac_entry(const DevNet &N, const double *P, int n)
+{
+    double acc = 0.0;
+    const int q1 = N.jnz_ptr[n + 1];
+#pragma unroll 4
+    for (int q = N.jnz_ptr[n]; q < q1; q++) {
+        const int pk = N.jc_pk[q];
+        acc += (double)((pk & 15) - 8) * P[pk >> 4];
+    }
+    return acc;
+}
+
 // Newton matrix M = I - gamma J(y) in column-major storage, J from the generated
 // structure plus the structured couplings (photo rates, totalSwap, ice transfer and the
 // BULK/SURFACE totals).  With scale != 0 the matrix is equilibrated by the error weights:
 // M' = W M W^-1, W = diag(ewt) (held in L.g).
 template <typename LUT>
-__device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma, bool convert = true,
+UCG_BIG __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma, bool convert = true,
                                     bool scaled = false)
 {
     const int neq = N.neq, ld = N.ld;
     double *y = L.y, *rate = L.rate, *sc = L.sc, *g = L.g;
-    double *scr = L.flux;   // scratch [>= 2*neq]
+    double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
     const double D = y[neq - 1];
     const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
     const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
@@ -418,26 +512,14 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
     const double TS = block_sum(ts, L.red);
     const double tsm = TS / sm;
     const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
-    // 1. zero J
+    // 1. zero J; partial derivatives of every term
     for (int idx = tid(); idx < neq * ld; idx += NT) M[idx] = (LUT)0;
+    double *P = L.flux;
+    block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 2. generated sparse part
-    for (int n = tid(); n < N.nnz; n += NT) {
-        double acc = 0.0;
-        const int c0 = N.jnz_ptr[n], c1 = N.jnz_ptr[n + 1];
-        for (int q = c0; q < c1; q++) {
-            const int k = N.jc_k[q], m = N.jc_m[q];
-            const int *fc = N.fac + MAXFAC * k;
-            const int nf = N.nfac[k];
-            double p = rate[k];
-            for (int mm = 0; mm < nf; mm++) {
-                const int c = fc[mm];
-                p *= (mm == m) ? fac_deriv(c, dblr, dism, dtsm) : fac_value(c, y, D, blr, ism, tsm);
-            }
-            acc += N.jc_c[q] * p;
-        }
-        M[(size_t)N.jnz_col[n] * ld + N.jnz_row[n]] = (LUT)acc;
-    }
+    for (int n = tid(); n < N.nnz; n += NT) M[(size_t)N.jnz_col[n] * ld + N.jnz_row[n]] = (LUT)jac_entry(N, P, n);
+    __syncthreads();
     // 3a. totalSwap dependence of SURFSWAP terms: J[i][b] += u_i * dTS/dy_b
     for (int i = tid(); i < neq; i += NT) {
         double u = 0.0;
@@ -579,7 +661,10 @@ __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamm
 
 // ---------------------------------------------------------------- dense LU (no pivoting)
 
-constexpr int LU_NB = 16;
+#ifndef UCG_LU_NB
+#define UCG_LU_NB 16
+#endif
+constexpr int LU_NB = UCG_LU_NB;
 
 // Right-looking blocked LU of the column-major n x n matrix M (unit lower L, upper U
 // stored in place).  Per block column: the 16x16 diagonal block is factored by wave 0 in
@@ -589,7 +674,7 @@ constexpr int LU_NB = 16;
 // `stage` is LDS scratch of >= 2*16*16 doubles.  Returns nonzero (uniformly) on a zero
 // or non-finite pivot.
 template <typename LUT>
-__device__ int block_lu(LUT *M, int n, int ld, double *red, double *stage)
+UCG_BIG __device__ int block_lu(LUT *M, int n, int ld, double *red, double *stage)
 {
     double *U11 = stage;              // U11[p*16+q] = U(kb+p, kb+q)
     double *L11 = stage + LU_NB * LU_NB;
@@ -711,11 +796,14 @@ __device__ int block_lu(LUT *M, int n, int ld, double *red, double *stage)
     return 0;
 }
 
-constexpr int SB = 32;   // triangular-solve block: wave 0 holds the diagonal block rows
+#ifndef UCG_SB
+#define UCG_SB 32
+#endif
+constexpr int SB = UCG_SB;   // triangular-solve block: wave 0 holds the diagonal block rows
 
 // Solve (LU) x = b in place on the LDS vector b (FP64 accumulation).
 template <typename LUT>
-__device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
+UCG_BIG __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
 {
     const int lane = tid() & 63, wave = tid() >> 6;
     // forward, unit lower
@@ -781,11 +869,11 @@ __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
 // SURFACE rows replaced by e_BULK - sum_b e_b and e_SURFACE - sum_s e_s.  Values are
 // written in plan slot order; fill slots start at zero.
 template <typename LUT>
-__device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gamma)
+UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gamma)
 {
     const int neq = N.neq;
     double *y = L.y, *rate = L.rate, *sc = L.sc;
-    double *scr = L.flux;   // scratch [>= 2*neq]
+    double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
     const double D = y[neq - 1];
     const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
     const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
@@ -799,24 +887,11 @@ __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gam
     const double tsm = TS / sm;
     const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
     for (int q = tid(); q < N.nw_nval; q += NT) V[q] = (LUT)0;
+    double *P = L.flux;
+    block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 1. generated entries of J0, stored as -gamma * dJ; u1 (totalSwap column factor)
-    for (int n = tid(); n < N.nnz; n += NT) {
-        double acc = 0.0;
-        const int c0 = N.jnz_ptr[n], c1 = N.jnz_ptr[n + 1];
-        for (int q = c0; q < c1; q++) {
-            const int k = N.jc_k[q], m = N.jc_m[q];
-            const int *fc = N.fac + MAXFAC * k;
-            const int nf = N.nfac[k];
-            double p = rate[k];
-            for (int mm = 0; mm < nf; mm++) {
-                const int c = fc[mm];
-                p *= (mm == m) ? fac_deriv(c, dblr, dism, dtsm) : fac_value(c, y, D, blr, ism, tsm);
-            }
-            acc += N.jc_c[q] * p;
-        }
-        V[N.nw_jnz_slot[n]] = (LUT)(-gamma * acc);
-    }
+    for (int n = tid(); n < N.nnz; n += NT) V[N.nw_jnz_slot[n]] = (LUT)(-gamma * jac_entry(N, P, n));
     for (int i = tid(); i < neq; i += NT) {
         double u = 0.0;
         for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {
@@ -950,44 +1025,80 @@ __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gam
 
 // Crout elimination of the sparse pivots in plan phases (one thread per target entry,
 // dependencies only on earlier phases), then the blocked dense LU of the trailing block.
+// FP32 factors are staged in LDS (region) for the whole factorisation; FP64 factors stay
+// in the workspace and only the dense tail is staged for its LU.  The factors end in V.
+// stage: 512 doubles of LDS.
 template <typename LUT>
-__device__ int sparse_lu(const DevNet &N, LUT *V, double *red, double *stage)
+__device__ __forceinline__ void crout_phases(const DevNet &N, LUT *W, double *red)
 {
-    if (tid() == 0) red[9] = 0.0;
-    __syncthreads();
     for (int p = 0; p < N.nw_nph; p++) {
         const int t1 = N.nw_ph_ptr[p + 1];
         for (int t = N.nw_ph_ptr[p] + tid(); t < t1; t += NT) {
             const int tg = N.nw_tg_slot[t];
-            double acc = (double)V[tg];
+            double acc = (double)W[tg];
             const int q1 = N.nw_tg_ptr[t + 1];
-            for (int q = N.nw_tg_ptr[t]; q < q1; q++) acc -= (double)V[N.nw_pr_l[q]] * (double)V[N.nw_pr_u[q]];
+#pragma unroll 4
+            for (int q = N.nw_tg_ptr[t]; q < q1; q++) acc -= (double)W[N.nw_pr_l[q]] * (double)W[N.nw_pr_u[q]];
             const int dv = N.nw_tg_div[t];
             if (dv >= 0) {
-                const double piv = (double)V[dv];
+                const double piv = (double)W[dv];
                 if (!(piv == piv) || piv == 0.0 || isinf(piv)) red[9] = 1.0;
                 acc = acc / piv;
             }
-            V[tg] = (LUT)acc;
+            W[tg] = (LUT)acc;
         }
         __syncthreads();
     }
     for (int r = tid(); r < N.nw_ns; r += NT) {
-        const double piv = (double)V[N.nw_sp_dpos[r]];
+        const double piv = (double)W[N.nw_sp_dpos[r]];
         if (!(piv == piv) || piv == 0.0 || isinf(piv)) red[9] = 1.0;
     }
     __syncthreads();
-    if (red[9] != 0.0) return 1;
-    if (N.nw_T > 0) return block_lu<LUT>(V + N.nw_nsp, N.nw_T, N.nw_T, red, stage);
-    return 0;
+}
+
+template <typename LUT>
+UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, double *region, double *red, double *stage)
+{
+    const int nval = N.nw_nval, nsp = N.nw_nsp, T = N.nw_T;
+    if (tid() == 0) red[9] = 0.0;
+    int r = 0;
+    if constexpr (sizeof(LUT) == 4) {
+        LUT *Vl = (LUT *)region;
+        for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
+        __syncthreads();
+        crout_phases<LUT>(N, Vl, red);
+        if (red[9] != 0.0) return 1;
+        if (T > 0) r = block_lu<LUT>(Vl + nsp, T, T, red, stage);
+        for (int q = tid(); q < nval; q += NT) V[q] = Vl[q];
+    } else {
+        __syncthreads();
+        crout_phases<LUT>(N, V, red);
+        if (red[9] != 0.0) return 1;
+        LUT *tl = (LUT *)region;
+        for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
+        __syncthreads();
+        if (T > 0) r = block_lu<LUT>(tl, T, T, red, stage);
+        for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
+    }
+    __syncthreads();
+    return r;
 }
 
 // Solve M x = b with the factors of sparse_lu: b (length neq, in LDS) is overwritten by x.
-// w is LDS scratch of at least neq + 2 doubles.
+// region: LDS scratch; FP32 factors are staged there (read-only copy) and the permuted
+// work vector follows them, FP64 factors are read from the workspace.
 template <typename LUT>
-__device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *V, double *b, double *w)
+UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, double *b, double *region)
 {
     const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns;
+    const LUT *V = Vg;
+    double *w = region;
+    if constexpr (sizeof(LUT) == 4) {   // FP32 factors: read-only copy in LDS, work vector after it
+        LUT *Vs = (LUT *)region;
+        for (int q = tid(); q < N.nw_nval; q += NT) Vs[q] = Vg[q];
+        V = Vs;
+        w = region + (((N.nw_nval + 1) / 2 + 7) & ~7);
+    }
     double sb = 0.0, ss = 0.0;
     for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
     for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
@@ -1049,6 +1160,11 @@ struct GpuOps {
     uint64_t t_rhs = 0, t_jac = 0, t_lu = 0, t_sol = 0;   // device wall-clock ticks per phase
     bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M (dense form only)
     bool sparse = true;         // static-pattern sparse plan (else dense LU of the full matrix)
+    const int *cancel = nullptr;  // device flag the host may raise while the kernel runs
+    __device__ bool cancelled() const
+    {
+        return cancel && __hip_atomic_load(cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+    }
 
     __device__ GpuOps(const DevNet &n_, Lds &l_, LUT *m_) : N(n_), L(l_), M(m_), neq(n_.neq) {}
 
@@ -1243,7 +1359,7 @@ struct GpuOps {
         if (sparse) {
             sparse_newton_matrix<LUT>(N, L, M, gamma);
             t1 = wall_clock64();
-            r = sparse_lu<LUT>(N, M, L.red, L.flux);
+            r = sparse_lu<LUT>(N, M, L.flux, L.red, L.b);
         } else {
             block_newton_matrix<LUT>(N, L, M, gamma, true, equilibrate);
             t1 = wall_clock64();
@@ -1294,6 +1410,7 @@ struct GpuOps {
 
 // ---------------------------------------------------------------- persistent solve kernel
 
+constexpr int STATUS_CANCELLED = -10;   // UCG_E_CANCELLED
 constexpr int MAX_INTERVALS = 100000;
 
 struct SolveArgs {
@@ -1310,8 +1427,11 @@ struct SolveArgs {
     double *t_start;          // [n] device wall clock when the point was taken, or null
     double *t_done;           // [n] device wall clock at retirement, or null
     void *ws;                 // slot workspaces
-    size_t ws_stride;         // bytes per slot
+    size_t ws_stride;         // bytes per slot: Newton matrix (ws_mat_bytes) then rate[nreac]
+    size_t ws_mat_bytes;
     int *counter;             // work queue head
+    const int *cancel;        // nullable device flag: nonzero stops the batch (points not done get
+                              // STATUS_CANCELLED)
     int equilibrate;          // factor the error-weight-equilibrated Newton matrix (dense form)
     int sparse;               // 1: static-pattern sparse plan, 0: dense LU
 };
@@ -1362,27 +1482,50 @@ __device__ void record_row(const DevNet &N, Lds &L, const SolveArgs &A, int pidx
     for (int i = tid(); i < ns; i += NT) r[4 + i] = L.y[i];
 }
 
-template <typename LUT>
-__global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
+// LDS layout shared by all kernels; the rate coefficients live in the workspace slot
+// (read coalesced, written once per interval attempt).
+__device__ void lds_layout(const DevNet &N, Lds &L, double *lds_raw, double *rate)
 {
-    extern __shared__ double lds_raw[];
-    const int neq = N.neq, nr = N.nreac;
-    const int npad = (neq + 7) & ~7;
-    Lds L;
-    L.rate = lds_raw;
-    L.flux = L.rate + nr;
-    L.y = L.flux + (nr > 2 * npad ? nr : 2 * npad);
+    const int npad = (N.neq + 7) & ~7;
+    L.rate = rate;
+    L.flux = lds_raw;
+    L.y = L.flux + flux_region_len(N);
     L.f = L.y + npad;
     L.b = L.f + npad;
     L.g = L.b + npad;
     L.red = L.g + npad;
     L.sc = L.red + 16;
     L.prm = L.sc + S_NSCAL;
+}
+
+// Resets the work counter and the cancel flag and marks every point "not run" before a
+// solve; a kernel rather than memsets so that the launch sequence can be graph-captured.
+__global__ void solve_init_kernel(int *counter, int *cancel, int *status, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *counter = 0;
+        if (cancel) *cancel = 0;
+    }
+    if (i < n) status[i] = STATUS_CANCELLED;
+}
+
+template <typename LUT>
+__global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevNet N, SolveArgs A)
+{
+    extern __shared__ double lds_raw[];
+    const int neq = N.neq;
+    Lds L;
+    char *slot = (char *)A.ws + (size_t)blockIdx.x * A.ws_stride;
+    lds_layout(N, L, lds_raw, (double *)(slot + A.ws_mat_bytes));
     __shared__ int s_point;
-    LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
+    LUT *M = (LUT *)slot;
 
     for (;;) {
-        if (tid() == 0) s_point = atomicAdd(A.counter, 1);
+        if (tid() == 0) {
+            const bool stop = A.cancel && __hip_atomic_load(A.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+            s_point = stop ? A.n_points : atomicAdd(A.counter, 1);
+        }
         __syncthreads();
         const int pidx = s_point;
         __syncthreads();
@@ -1416,6 +1559,7 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
         }
         GpuOps<LUT> o(N, L, M);
         o.sparse = A.sparse != 0;
+        o.cancel = A.cancel;
         o.equilibrate = !o.sparse && A.equilibrate != 0;
         bdf::Stats st;
         int row = 0, nint = 0, nretry = 0, flag = 0;
@@ -1431,6 +1575,7 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
         int stuck = 0;
         uint64_t t_rates = 0;
         const uint64_t t_point0 = wall_clock64();
+        const uint64_t c_point0 = clock64();   // shader clock, for the effective frequency
         while ((end_dens && L.sc[S_DENS] < prm[P_FINALDENS]) || (!end_dens && tyears < prm[P_FINALTIME])) {
             if (nint >= MAX_INTERVALS || stuck > 10) {
                 flag = -5;
@@ -1457,10 +1602,11 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
                 __syncthreads();
                 int is = bdf::integrate(o, t, target, reltol, mxstep, st);
                 if (is == -2 && tid() == 0) L.sc[S_ABSFAC] = absfac * 10.0;
-                if (is < 0) nretry++;
+                if (is < 0 && is != -7) nretry++;
                 if (is == -3) {
                     flag = -4;   // INT_UNRECOVERABLE_ERROR (constants.f90:25)
                 }
+                if (is == -7) flag = STATUS_CANCELLED;
                 // clamp (chemistry.f90:215) and density bookkeeping (:216)
 #pragma unroll
                 for (int e = 0; e < EPT; e++)
@@ -1519,7 +1665,10 @@ __global__ __launch_bounds__(NT) void solve_points_kernel(DevNet N, SolveArgs A)
                 const double tick = 1.0e-8;   // wall_clock64 runs at 100 MHz
                 double *tm = A.timers + (size_t)pidx * 8;
                 tm[0] = t_rates * tick; tm[1] = o.t_rhs * tick; tm[2] = o.t_jac * tick; tm[3] = o.t_lu * tick;
-                tm[4] = o.t_sol * tick; tm[5] = (wall_clock64() - t_point0) * tick; tm[6] = 0; tm[7] = 0;
+                const uint64_t dw = wall_clock64() - t_point0, dc = clock64() - c_point0;
+                tm[4] = o.t_sol * tick; tm[5] = dw * tick;
+                tm[6] = dw ? (double)dc / ((double)dw * tick) * 1e-6 : 0.0;   // mean shader clock, MHz
+                tm[7] = 0;
             }
         }
         __syncthreads();
@@ -1541,7 +1690,8 @@ struct PieceArgs {
     double gamma;
     int sparse;
     void *ws;
-    size_t ws_stride;
+    size_t ws_stride;      // bytes per point: Newton matrix (ws_mat_bytes) then rate[nreac]
+    size_t ws_mat_bytes;
 };
 
 __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
@@ -1565,26 +1715,12 @@ __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
     __syncthreads();
 }
 
-__device__ void piece_lds(const DevNet &N, Lds &L, double *lds_raw)
-{
-    const int neq = N.neq, nr = N.nreac;
-    const int npad = (neq + 7) & ~7;
-    L.rate = lds_raw;
-    L.flux = L.rate + nr;
-    L.y = L.flux + (nr > 2 * npad ? nr : 2 * npad);
-    L.f = L.y + npad;
-    L.b = L.f + npad;
-    L.g = L.b + npad;
-    L.red = L.g + npad;
-    L.sc = L.red + 16;
-    L.prm = L.sc + S_NSCAL;
-}
 
 __global__ __launch_bounds__(NT) void rates_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    piece_lds(N, L, lds_raw);
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1602,7 +1738,7 @@ __global__ __launch_bounds__(NT) void rhs_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    piece_lds(N, L, lds_raw);
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1616,7 +1752,7 @@ __global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    piece_lds(N, L, lds_raw);
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1636,7 +1772,7 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    piece_lds(N, L, lds_raw);
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     // y holds the state for J; the right-hand side is rate_in reinterpreted as [n][neq]
@@ -1649,7 +1785,7 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
     int bad;
     if (A.sparse) {
         sparse_newton_matrix<LUT>(N, L, M, A.gamma);
-        bad = sparse_lu<LUT>(N, M, L.red, L.flux);
+        bad = sparse_lu<LUT>(N, M, L.flux, L.red, L.b);
     } else {
         block_newton_matrix<LUT>(N, L, M, A.gamma);
         bad = block_lu<LUT>(M, N.neq, N.ld, L.red, L.flux);

@@ -42,10 +42,24 @@ class GpuError(RuntimeError):
 _lib = None
 
 
+def _init_torch_runtime_first():
+    """torch ships its own HIP runtime; when both are used in one process, torch's must
+    initialise the device before this library's (/opt/rocm) runtime does, or torch finds
+    no GPU afterwards.  Device buffers are shared between the two (one GPU address space
+    per process)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     """Load the HIP library once; raise loudly when it is not built."""
     global _lib
     if _lib is None:
+        _init_torch_runtime_first()
         if not os.path.exists(LIB_PATH):
             raise GpuError(f"{LIB_PATH} is not built; run `python -m uclchem_amd.build` "
                            "(or __graft_entry__.build())")
@@ -98,6 +112,11 @@ class Network:
         self.handle = h.value
 
     SPARSE, DENSE = 0, 1
+    CANCELLED = -10
+
+    def cancel(self) -> None:
+        """Stop running device solves on this network (unfinished points: CANCELLED)."""
+        _check(lib().ucg_cancel(ctypes.c_int(self.handle)), "ucg_cancel")
 
     def set_linear_solver(self, kind: int) -> None:
         """Network.SPARSE (static-pattern plan, default) or Network.DENSE (full LU)."""

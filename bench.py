@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-cap", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump", default="", help="write per-point counters and times to <prefix>_fp<bits>.npz")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; grid points are independent and would shard "
@@ -177,6 +178,8 @@ def main():
         solve = (t1 - t0) * 1e-8              # one point's own solve time on its workgroup
         results[lu_bits] = dict(ms=ms, wall=wall, stats=st, status=status, lat=lat, solve=solve,
                                 y=d_y.cpu().numpy().copy())
+        if a.dump:
+            np.savez_compressed(a.dump + f"_fp{lu_bits}.npz", stats=st, status=status, solve=solve, lat=lat)
     main_r = results[a.lu]
     value = n / (main_r["ms"] / 1e3)
     lu_flop, solve_flop = newton_flop_counts(tables)

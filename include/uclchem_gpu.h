@@ -132,9 +132,11 @@ int ucg_solve_batch_device(int handle, int n, const double *params, const double
  * loops and time-bounded batches. */
 int ucg_cancel(int handle);
 
-/* ucg_solve_batch with per-point device timers [n][8]: seconds in rates, RHS, Jacobian
- * assembly, LU, triangular solves, total; then the mean shader clock over the point's
- * solve in MHz (no output rows).  n_slots as above. */
+/* ucg_solve_batch with per-point device timers [n][16]: seconds in rates, RHS, Jacobian
+ * assembly, LU, triangular solves, total; the mean shader clock over the point's solve
+ * in MHz; then LU sparse part, LU dense tail, solve sparse part, solve dense tail
+ * (sparse plan only); RHS photo rates, totalSwap, fluxes, gather and totals; 1 reserved
+ * (no output rows).  n_slots as above. */
 int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,
                              double *timers, int lu_bits, int n_slots);
 

@@ -14,6 +14,10 @@
 // the GPU every thread of the workgroup calls each method (each owns a slice of
 // the state vectors; block reductions and the RHS/LU/solve synchronise inside).
 //
+// Ops::setup(gamma, new_jacobian) forms and factors M = I - gamma J, evaluating J first
+// when new_jacobian (else from the J saved at the last evaluation, as DVODE's JSV=1 path
+// does); Ops::can_reuse_jacobian() false forces an evaluation at every setup.
+//
 // Call contract of one integrate() call mirrors DVODE_F90 with ITASK=1, ISTATE=1
 // (a cold start per call, src/fortran_src/chemistry.f90:241-292): integrate from
 // t0 past tout, interpolate back to tout, report DVODE-style states:
@@ -44,6 +48,11 @@ constexpr double ETAMX1 = 1.0e4, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETA
 constexpr double ETACF = 0.25, CRDOWN = 0.3, DGMAX = BDF_DGMAX, RDIV = 2.0, NLSCOEF = 0.1;
 constexpr int SMALL_NST = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
 constexpr int MSBP = 20, MAXCOR = 3, MXNCF = 10, HIN_ITERS = 4;
+// saved-Jacobian reuse (DVODE's JSV=1 / CVODE's lsetup): re-evaluate J at most every
+// MSBJ steps, after a convergence failure, or when a stale-J failure is not explained by
+// a change of gamma of more than DGMAX_J
+constexpr int MSBJ = 50;
+constexpr double DGMAX_J = 0.2;
 constexpr double HLB_FACTOR = 100.0, HUB_FACTOR = 0.1, H_BIAS = 0.5;
 
 // vector slots the Ops policy provides
@@ -59,7 +68,7 @@ struct State {
     double l[QMAX + 1];
     double tq[6];
     double gamma, gammap, gamrat, rl1, crate, acnrm, saved_tq5;
-    int q, qprime, qwait, L, nstlp, nst_call;
+    int q, qprime, qwait, L, nstlp, nstlj, nst_call;
     bool jcur;
 };
 
@@ -191,20 +200,33 @@ BDF_HD int nls(Ops &o, State &s, Stats &st, int nflag)
 {
     bool call_setup = (nflag == PREV_CONV_FAIL) || (nflag == PREV_ERR_FAIL) || (st.nst == 0) ||
                       (st.nst >= s.nstlp + MSBP) || (dabs(s.gamrat - 1.0) > DGMAX);
+    bool stale_retry = false;
     for (;;) {
         o.start_corrector();          // acor = 0, y = zn[0]
         o.rhs();                      // F = f(tn, y)
         st.nfe++;
         if (call_setup) {
-            int ierr = o.setup(s.gamma);   // J at y = zn[0]; M = I - gamma J; factor
+            const bool jbad = (st.nst == 0) || (st.nst >= s.nstlj + MSBJ) || (nflag == PREV_CONV_FAIL) ||
+                              (stale_retry && dabs(s.gamrat - 1.0) < DGMAX_J) || !o.can_reuse_jacobian();
+            int ierr = o.setup(s.gamma, jbad);   // new J at y = zn[0] if jbad; M = I - gamma J; factor
             st.nsetups++;
-            st.nje++;
-            s.jcur = true;
+            if (jbad) {
+                st.nje++;
+                s.nstlj = st.nst;
+            }
+            s.jcur = jbad;
             s.gamrat = 1.0;
             s.gammap = s.gamma;
             s.crate = 1.0;
             s.nstlp = st.nst;
-            if (ierr) return 1;       // singular Newton matrix: retry with a smaller step
+            if (ierr) {
+                if (!jbad) {          // singular with a saved J: evaluate J and try once more
+                    stale_retry = false;
+                    s.nstlj = -MSBJ;
+                    continue;
+                }
+                return 1;             // singular Newton matrix: retry with a smaller step
+            }
         }
         double del = 0.0, delp = 0.0;
         int m = 0;
@@ -237,6 +259,7 @@ BDF_HD int nls(Ops &o, State &s, Stats &st, int nflag)
         }
         if (retry) {
             call_setup = true;        // stale Jacobian: refresh and start the iteration again
+            stale_retry = true;
             continue;
         }
     }
@@ -445,6 +468,7 @@ BDF_HD int integrate(Ops &o, double &t, double tout, double rtol, int mxstep, St
     s.acnrm = 0.0;
     s.saved_tq5 = 1.0;
     s.nstlp = 0;
+    s.nstlj = 0;
     s.nst_call = 0;
     s.jcur = false;
     s.etaq = s.etaqm1 = s.etaqp1 = 0.0;

@@ -192,7 +192,8 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
     size_t mat = (sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem;
     mat = (mat + 255) & ~(size_t)255;
-    const size_t stride = mat + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
+    const size_t jac = sparse ? (((size_t)N.nw_nval * sizeof(double) + 255) & ~(size_t)255) : 0;
+    const size_t stride = mat + jac + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
     // sized for the default slot count at least, so later calls with more points do not
     // reallocate between launches
     int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
@@ -217,6 +218,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.ws = h.ws;
     A.ws_stride = stride;
     A.ws_mat_bytes = mat;
+    A.ws_jac_bytes = jac;
     A.counter = h.counter;
     A.cancel = h.cancel;
     const size_t lds = lds_bytes(N);
@@ -247,11 +249,13 @@ int piece_common(NetHandle &h, int n, const double *params, const double *phys, 
     A.params = (const double *)dp.p;
     A.phys = (const double *)dph.p;
     const size_t mat = ((size_t)N.ld * N.neq * sizeof(double) + 255) & ~(size_t)255;
-    const size_t stride = mat + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
+    const size_t jac = ((size_t)N.nw_nval * sizeof(double) + 255) & ~(size_t)255;
+    const size_t stride = mat + jac + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
     if ((rc = ensure_ws(h, stride * n))) return rc;
     A.ws = h.ws;
     A.ws_stride = stride;
     A.ws_mat_bytes = mat;
+    A.ws_jac_bytes = jac;
     return 0;
 }
 
@@ -494,6 +498,51 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         if ((rc = upload(h, foff.data(), foff.size(), N.fac_off))) return rc;
         if ((rc = upload(h, jpk.data(), jpk.size(), N.jc_pk))) return rc;
     }
+    {   // production/loss lists index fluxes: checked here, the device trusts them
+        for (int kind = 0; kind < 2; kind++) {
+            const int *ptr = kind ? d->loss_ptr : d->prod_ptr, *idx = kind ? d->loss_idx : d->prod_idx;
+            const int len = kind ? d->nloss : d->nprod;
+            if (ptr[0] != 0 || ptr[d->nspec] != len) return fail(UCG_E_ARG, "production/loss list bounds");
+            for (int i = 0; i < d->nspec; i++)
+                if (ptr[i + 1] < ptr[i]) return fail(UCG_E_ARG, "production/loss lists not ordered");
+            for (int q = 0; q < len; q++)
+                if (idx[q] < 0 || idx[q] >= d->nreac) return fail(UCG_E_ARG, "production/loss reaction index");
+        }
+    }
+    {   // balanced production/loss sum tasks (chunks of at most GT_CHUNK flux indices)
+        const int GT_CHUNK = 24;
+        struct Task { int start, end, species, kind; };
+        std::vector<Task> tasks;
+        for (int i = 0; i < d->nspec; i++)
+            for (int kind = 0; kind < 2; kind++) {
+                const int *ptr = kind ? d->loss_ptr : d->prod_ptr;
+                for (int a = ptr[i]; a < ptr[i + 1]; a += GT_CHUNK)
+                    tasks.push_back({a, std::min(a + GT_CHUNK, ptr[i + 1]), i, kind});
+            }
+        std::vector<int> order(tasks.size());
+        for (size_t t = 0; t < order.size(); t++) order[t] = (int)t;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return tasks[x].end - tasks[x].start > tasks[y].end - tasks[y].start;
+        });
+        std::vector<int> rec(4 * tasks.size()), where(tasks.size());
+        for (size_t t = 0; t < order.size(); t++) {
+            const Task &k = tasks[order[t]];
+            rec[4 * t] = k.start; rec[4 * t + 1] = k.end; rec[4 * t + 2] = k.species; rec[4 * t + 3] = k.kind;
+            where[order[t]] = (int)t;
+        }
+        std::vector<int> sptr(2 * (size_t)d->nspec + 1, 0), stask;
+        for (size_t t = 0; t < tasks.size(); t++) sptr[2 * tasks[t].species + tasks[t].kind + 1]++;
+        for (size_t q = 1; q < sptr.size(); q++) sptr[q] += sptr[q - 1];
+        stask.resize(tasks.size());
+        std::vector<int> fill(sptr.begin(), sptr.end() - 1);
+        for (size_t t = 0; t < tasks.size(); t++)   // tasks were generated in chunk order
+            stask[fill[2 * tasks[t].species + tasks[t].kind]++] = where[t];
+        N.ngt = (int)tasks.size();
+        N.gt_off = (d->nreac + 7) & ~7;
+        if ((rc = upload(h, rec.data(), rec.size(), N.gt_rec))) return rc;
+        if ((rc = upload(h, sptr.data(), sptr.size(), N.gt_sp_ptr))) return rc;
+        if ((rc = upload(h, stask.data(), stask.size(), N.gt_sp_task))) return rc;
+    }
     {   // packed term records {n_d, ops[6], nfac}
         std::vector<int> prog(8 * R);
         for (size_t k = 0; k < R; k++) {
@@ -526,6 +575,31 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         UP(nw_tr_slot, (size_t)8 * d->nsurfl); UP(nw_s1c, Q); UP(nw_s1r, Q); UP(nw_s2c, Q); UP(nw_s2r, Q);
         UP(nw_gs_ptr, Q + 1); UP(nw_gs_slot, (size_t)z[9]); UP(nw_misc, (size_t)5);
         UP(nw_nbrow, (size_t)d->nbulkl); UP(nw_nsrow, (size_t)d->nsurfl);
+        // solve row records: sparse rows in level order, then the tail rows
+        std::vector<int> sv(4 * NN);
+        for (size_t q = 0; q < NN; q++) {
+            const int r = q < ns ? d->nw_lev_rows[q] : (int)q;
+            sv[4 * q] = r;
+            sv[4 * q + 1] = d->nw_sp_ptr[r];
+            sv[4 * q + 2] = r < (int)ns ? d->nw_sp_dpos[r] : d->nw_sp_ptr[r + 1];
+            sv[4 * q + 3] = d->nw_sp_ptr[r + 1];
+        }
+        if ((rc = upload(h, sv.data(), sv.size(), N.sv_rec))) return rc;
+        std::vector<int> pr(2 * npair);
+        for (size_t q = 0; q < npair; q++) {
+            pr[2 * q] = d->nw_pr_l[q];
+            pr[2 * q + 1] = d->nw_pr_u[q];
+        }
+        if ((rc = upload(h, pr.data(), pr.size(), N.nw_pr))) return rc;
+        std::vector<signed char> uns((size_t)N.nw_nval, 0);
+        for (size_t i = 0; i < Q; i++) {
+            if (d->nw_s1c[i] >= 0) uns[d->nw_s1c[i]] = 1;
+            if (d->nw_s2c[i] >= 0) uns[d->nw_s2c[i]] = 1;
+        }
+        uns[d->nw_misc[2]] = 1;
+        const int8_t *du = nullptr;
+        if ((rc = upload(h, (const int8_t *)uns.data(), uns.size(), du))) return rc;
+        N.nw_unscaled = du;
     }
 #undef UP
     const int *ss = d->special_species;
@@ -637,7 +711,7 @@ int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_
     if ((rc = alloc_dev<double>(dy, (size_t)n * neq))) return rc;
     if ((rc = alloc_dev<int>(dst, (size_t)n))) return rc;
     if ((rc = alloc_dev<int>(dstat, (size_t)n * 8))) return rc;
-    if ((rc = alloc_dev<double>(dtm, (size_t)n * 8))) return rc;
+    if ((rc = alloc_dev<double>(dtm, (size_t)n * 16))) return rc;
     rc = launch_solve(*h, n, (const double *)dp.p, nullptr, (double *)dy.p, 0, nullptr, nullptr, (int *)dst.p,
                       (int *)dstat.p, nullptr, nullptr, lu_bits, n_slots, nullptr, (double *)dtm.p);
     if (rc) return rc;
@@ -645,7 +719,7 @@ int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_
     HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(timers, dtm.p, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(timers, dtm.p, (size_t)n * 16 * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

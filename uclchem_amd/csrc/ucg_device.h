@@ -11,6 +11,10 @@
 
 namespace ucg {
 
+// pointer into workgroup-local memory (LDS); keeps the compiler from falling back to
+// flat addressing, whose loads wait on both the LDS and the vector-memory counters
+#define LDS __attribute__((address_space(3)))
+
 constexpr int NT = 256;          // threads per workgroup (one workgroup solves one grid point)
 constexpr int EPT = 2;           // state elements owned per thread => NEQ <= 512
 constexpr int MAXOPS = 6;
@@ -54,6 +58,10 @@ struct DevNet {
     const int32_t *facp;   // [nreac*4] factor codes as int16 x 8 (density factors first), -1 padded
     const int32_t *fac_off, *jc_pk;   // partial offsets [nreac+1]; contributions (partial << 4 | coef + 8)
     int32_t npart, npart_pad;         // number of partials; LDS offset of the scratch after them
+    // balanced production/loss sums: tasks {start, end, species, 0 prod | 1 loss} (longest
+    // first), per species the task lists [2i, 2i+1) prod and [2i+1, 2i+2) loss
+    const int32_t *gt_rec, *gt_sp_ptr, *gt_sp_task;
+    int32_t ngt, gt_off;              // task count; LDS offset (after the fluxes) of the partial sums
     const int8_t *extrapolate;
     const int32_t *type_ranges;                                     // [NTYPES*2]
     const int32_t *loss_ptr, *loss_idx, *prod_ptr, *prod_idx, *swap_list;
@@ -72,7 +80,19 @@ struct DevNet {
     const int32_t *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
     const int32_t *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
     const int32_t *nw_misc, *nw_nbrow, *nw_nsrow;
+    const int32_t *sv_rec;   // solve rows {row, first, diagonal, end}: sparse rows in level order, then tail rows
+    const int32_t *nw_pr;    // Crout pairs interleaved {L slot, U slot}
+    const int8_t *nw_unscaled;   // [nval] 1 where the Newton matrix takes the J-form value without gamma
 };
+
+// LDS bytes the sparse solve stages: factors (lut bytes each; FP32 all of them, FP64 the
+// sparse part), column indices, row records, permutation, work vector (16-byte aligned parts)
+__host__ __device__ inline int solve_stage_bytes(const DevNet &N, int lut)
+{
+    const int nst = lut == 4 ? N.nw_nval : N.nw_nsp;
+    auto al = [](int x) { return (x + 15) & ~15; };
+    return al(nst * lut) + al(N.nw_nsp * 4) + N.nw_N * 16 + al((N.nw_N + N.nw_nlev + 1) * 4) + N.nw_N * 8;
+}
 
 // length (in doubles) of the LDS region shared by the reaction fluxes (RHS), the term
 // partials plus two NEQ-long scratch vectors (Jacobian assembly), and the staged factors
@@ -81,10 +101,13 @@ __host__ __device__ inline int flux_region_len(const DevNet &N)
 {
     const int npad = (N.neq + 7) & ~7, Npad = (N.nw_N + 7) & ~7;
     const int a = N.npart_pad + 2 * npad;
-    const int b = N.nreac > a ? N.nreac : a;
-    const int t64 = N.nw_T * N.nw_T + Npad;                    // FP64: staged dense tail
-    const int f32 = (((N.nw_nval + 1) / 2 + 7) & ~7) + Npad;   // FP32: all factors + work vector
-    const int c = t64 > f32 ? t64 : f32;
+    const int g = N.gt_off + N.ngt;                  // fluxes + chunk partial sums
+    const int b0 = N.nreac > a ? N.nreac : a;
+    const int b = b0 > g ? b0 : g;
+    const int t64 = N.nw_T * N.nw_T + Npad;                    // FP64 factorisation: staged dense tail
+    const int s32 = (solve_stage_bytes(N, 4) + 7) / 8, s64 = (solve_stage_bytes(N, 8) + 7) / 8;
+    const int f = s32 > s64 ? s32 : s64;
+    const int c = t64 > f ? t64 : f;
     return b > c ? b : c;
 }
 
@@ -179,22 +202,47 @@ __device__ inline double h2_photodiss_dev(double nh2col, double radfield, double
     return 5.18e-11 * (radfield / (double)1.7f) * scat1000 * h2_self_shielding_dev(nh2col);
 }
 
-__device__ inline double co_photodiss_dev(double nh2col, double ncocol, double radfield, double av)
-{   // photoreactions.f90:57-72, 238-287
-    double lognco = log10(ncocol + 1.0), lognh2 = log10(nh2col + 1.0);
+// CO photodissociation (photoreactions.f90:57-72, 238-287) in parts, so that a wave can
+// evaluate the seven row splines and the scattering term in parallel lanes.
+__device__ __forceinline__ void co_log_columns(double nh2col, double ncocol, double &lognh2, double &lognco)
+{
+    lognco = log10(ncocol + 1.0);
+    lognh2 = log10(nh2col + 1.0);
     if (lognco < c_photo.nco[0]) lognco = c_photo.nco[0];
     if (lognh2 < c_photo.nh2[0]) lognh2 = c_photo.nh2[0];
     if (lognco > c_photo.nco[6]) lognco = c_photo.nco[6];
     if (lognh2 > c_photo.nh2[5]) lognh2 = c_photo.nh2[5];
-    double yy[7], y2[7];
-    for (int j = 0; j < 7; j++) yy[j] = splint_dev(c_photo.nh2, c_photo.ya[j], c_photo.y2a[j], 6, lognh2);
+}
+__device__ __forceinline__ double co_row_spline(int j, double lognh2)
+{   // row j of the 7x6 shielding table interpolated in log N(H2) (splin2's first pass)
+    return splint_dev(c_photo.nh2, c_photo.ya[j], c_photo.y2a[j], 6, lognh2);
+}
+__device__ __forceinline__ double co_shielding(const double yy[7], double lognco)
+{   // splin2's second pass along log N(CO)
+    double y2[7];
     spline_dev(c_photo.nco, yy, 7, y2);
-    double ssf = pow(10.0, splint_dev(c_photo.nco, yy, y2, 7, lognco));
+    return pow(10.0, splint_dev(c_photo.nco, yy, y2, 7, lognco));
+}
+__device__ __forceinline__ double co_scatter(double nh2col, double ncocol, double av)
+{   // dust scattering at the mean wavelength lbar (photoreactions.f90:289-300)
     double lu = log10(fabs(ncocol) + 1.0), lw = log10(fabs(nh2col) + 1.0);
     double lb = (5675.0 - 200.6 * lw) - (571.6 - 24.09 * lw) * lu + (18.22 - 0.7664 * lw) * (lu * lu);
     if (lb > 1076.1) lb = 1076.1;
     if (lb < 913.6) lb = 913.6;
-    return 2.0e-10 * (radfield / (double)1.7f) * ssf * scatter_dev(lb, av);
+    return scatter_dev(lb, av);
+}
+__device__ __forceinline__ double co_rate_from_parts(double radfield, double ssf, double scat)
+{
+    return 2.0e-10 * (radfield / (double)1.7f) * ssf * scat;
+}
+__device__ inline double co_photodiss_dev(double nh2col, double ncocol, double radfield, double av)
+{
+    double lognh2, lognco;
+    co_log_columns(nh2col, ncocol, lognh2, lognco);
+    double yy[7];
+    for (int j = 0; j < 7; j++) yy[j] = co_row_spline(j, lognh2);
+    const double ssf = co_shielding(yy, lognco);
+    return co_rate_from_parts(radfield, ssf, co_scatter(nh2col, ncocol, av));
 }
 
 __device__ inline double c_ionization_dev(double alpha, double gamma, double t, double ncol, double nh2col,
@@ -229,7 +277,8 @@ __device__ __forceinline__ double sticking_dev(double s0, double tcrit, double t
     return s0 * (1.0 + 2.5 * r) / pow(1.0 + r, 2.5);
 }
 
-__device__ inline double densdot_dev(const double *p, double d)
+template <typename PP>
+__device__ inline double densdot_dev(const PP p, double d)
 {   // physics-core.f90:90-103
     if (d < p[P_FINALDENS] && p[P_FREEFALL] != 0.0) {
         double n0 = p[P_DENS];

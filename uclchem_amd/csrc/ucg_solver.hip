@@ -33,12 +33,13 @@ enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI
 #endif
 
 struct Lds {
-    double *rate, *flux, *y, *f, *b, *g, *red, *sc, *prm;
+    double *rate;                                   // workspace (global memory)
+    LDS double *flux, *y, *f, *b, *g, *red, *sc, *prm;
 };
 
 __device__ __forceinline__ int tid() { return (int)threadIdx.x; }
 
-__device__ inline double block_sum(double v, double *red)
+__device__ inline double block_sum(double v, LDS double *red)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     __syncthreads();
@@ -50,7 +51,7 @@ __device__ inline double block_sum(double v, double *red)
     return r;
 }
 
-__device__ inline void block_sum2(double &a, double &b, double *red)
+__device__ inline void block_sum2(double &a, double &b, LDS double *red)
 {
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off);
@@ -72,7 +73,7 @@ __device__ inline void block_sum2(double &a, double &b, double *red)
     b = rb;
 }
 
-__device__ inline double block_max(double v, double *red)
+__device__ inline double block_max(double v, LDS double *red)
 {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
     __syncthreads();
@@ -91,7 +92,7 @@ __device__ __forceinline__ bool type_active(const DevNet &N, int t)
     return a >= 0 && a != b;   // rates.f90 range blocks are skipped when idx1 == idx2
 }
 
-__device__ inline double freeze_out(const DevNet &N, const double *prm, int j, double T, double Td)
+__device__ inline double freeze_out(const DevNet &N, const LDS double *prm, int j, double T, double Td)
 {   // rates.f90:354-367
     double f = 1.0 + N.beta[j] * 16.71e-4 / (GRAIN_RADIUS * T);
     if (prm[P_FREEZEFAC] == 0.0 || Td > MAX_GRAIN_TEMP) return 0.0;
@@ -106,8 +107,9 @@ __device__ inline double freeze_out(const DevNet &N, const double *prm, int j, d
 UCG_BIG __device__ void block_rates(const DevNet &N, Lds &L)
 {
     const int neq = N.neq;
-    const double *prm = L.prm;
-    double *y = L.y, *rate = L.rate, *sc = L.sc;
+    const LDS double *prm = L.prm;
+    LDS double *y = L.y, *sc = L.sc;
+    double *rate = L.rate;
     const double T = sc[S_TGAS], Td = sc[S_TDUST], av = sc[S_AV], dens = sc[S_DENS];
     const double zeta = prm[P_ZETA], radfield = prm[P_RADFIELD];
     if (tid() == 0 && prm[P_FREEFALL] == 0.0) y[neq - 1] = dens;
@@ -292,7 +294,7 @@ __device__ __forceinline__ void decode_codes(const int4 w, int c[8])
 
 // Value of one reaction's ODE term, factors applied in the order MakeRates emits them
 // (rate * D^b * y... with /safeMantle, *bulkLayersReciprocal, *totalSwap/safeMantle).
-__device__ __forceinline__ double term_from_codes(const int c[8], double r, const double *y, double D, double blr,
+__device__ __forceinline__ double term_from_codes(const int c[8], double r, const LDS double *y, double D, double blr,
                                                   double TS, double sm)
 {
     double v = r;
@@ -309,7 +311,7 @@ __device__ __forceinline__ double term_from_codes(const int c[8], double r, cons
     return v;
 }
 
-__device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const double *y,
+__device__ __forceinline__ double term_value(const DevNet &N, int k, const double *rate, const LDS double *y,
                                              double D, double blr, double TS, double sm)
 {
     int c[8];
@@ -319,55 +321,90 @@ __device__ __forceinline__ double term_value(const DevNet &N, int k, const doubl
 
 // RHS f(y) (chemistry.f90:294-352 + the generated GETYDOT, odes.f90).  Input L.y, output
 // L.f; also leaves the pre-transfer surface total in sc[S_SPRE] for the Jacobian.
-UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L)
+UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L, uint64_t *split = nullptr)
 {
+    const uint64_t c0 = wall_clock64();
     const int neq = N.neq;
-    double *y = L.y, *f = L.f, *rate = L.rate, *flux = L.flux, *sc = L.sc;
-    const double *prm = L.prm;
+    LDS double *y = L.y, *f = L.f, *flux = L.flux, *sc = L.sc;
+    double *rate = L.rate;
+    const LDS double *prm = L.prm;
     const double D = y[neq - 1];
     const double radfield = prm[P_RADFIELD];
-    if (tid() == 0) {
-        double h2col = 0.5 * y[N.nh2] * D * (sc[S_CS] / 1.0);
-        rate[N.r_h2_hv] = h2_photodiss_dev(h2col, radfield, sc[S_SCAT]);
-    }
-    if (tid() == 64) {
-        double h2col = 0.5 * y[N.nh2] * D * (sc[S_CS] / 1.0);
-        double cocol = 0.5 * y[N.nco] * D * (sc[S_CS] / 1.0);
-        rate[N.r_co_hv] = co_photodiss_dev(h2col, cocol, radfield, sc[S_AV]);
-    }
     const double sm = fmax(1e-30, y[N.nsurface]);
     const double sbk = fmax(1e-30, y[N.nbulk]);
     const double blr = fmin(1.0, NUM_SITES_PER_GRAIN / (GAS_DUST_DENSITY_RATIO * sbk));
-    __syncthreads();
     double ts = 0.0;
     for (int q = tid(); q < N.nswap; q += NT) ts += term_value(N, N.swap_list[q], rate, y, D, blr, 0.0, sm);
     const double TS = block_sum(ts, L.red);
-    {   // fluxes, four reactions per thread per pass with their records loaded up front
+    const uint64_t c1 = wall_clock64();
+    // H2 and CO photodissociation depend on the current column densities (chemistry.f90:311-319):
+    // the last wave evaluates them (the seven CO row splines, the scattering term and the H2
+    // rate in separate lanes) and their two fluxes while the other waves take every other flux.
+    if (tid() >= NT - 64) {
+        const int ln = tid() - (NT - 64);
+        const double h2col = 0.5 * y[N.nh2] * D * (sc[S_CS] / 1.0);
+        const double cocol = 0.5 * y[N.nco] * D * (sc[S_CS] / 1.0);
+        double lognh2, lognco;
+        co_log_columns(h2col, cocol, lognh2, lognco);
+        double v = 0.0;
+        if (ln < 7) v = co_row_spline(ln, lognh2);
+        else if (ln == 7) v = co_scatter(h2col, cocol, sc[S_AV]);
+        else if (ln == 8) v = h2_photodiss_dev(h2col, radfield, sc[S_SCAT]);
+        double yy[7];
+#pragma unroll
+        for (int j = 0; j < 7; j++) yy[j] = __shfl(v, j);
+        const double scat = __shfl(v, 7), kh2 = __shfl(v, 8);
+        if (ln == 0) {
+            rate[N.r_co_hv] = co_rate_from_parts(radfield, co_shielding(yy, lognco), scat);
+            rate[N.r_h2_hv] = kh2;
+            flux[N.r_co_hv] = term_value(N, N.r_co_hv, rate, y, D, blr, TS, sm);
+            flux[N.r_h2_hv] = term_value(N, N.r_h2_hv, rate, y, D, blr, TS, sm);
+        }
+    } else {
         const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
         const int nr = N.nreac;
-        for (int k0 = tid(); k0 < nr; k0 += 4 * NT) {
-            int4 w[4];
+        const int w = tid(), nw = NT - 64;
+        const int kh = N.r_h2_hv, kc = N.r_co_hv;
+        for (int k0 = w; k0 < nr; k0 += 4 * nw) {   // four reactions per pass, records loaded up front
+            int4 wr[4];
             double r[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int k = k0 + u * NT;
+                const int k = k0 + u * nw;
                 if (k < nr) {
-                    w[u] = fp[k];
+                    wr[u] = fp[k];
                     r[u] = rate[k];
                 }
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int k = k0 + u * NT;
-                if (k < nr) {
+                const int k = k0 + u * nw;
+                if (k < nr && k != kh && k != kc) {
                     int c[8];
-                    decode_codes(w[u], c);
+                    decode_codes(wr[u], c);
                     flux[k] = term_from_codes(c, r[u], y, D, blr, TS, sm);
                 }
             }
         }
     }
     __syncthreads();
+    const uint64_t c2 = wall_clock64();
+    // production and loss sums in balanced chunks (partial sums in LDS after the fluxes),
+    // then per species in chunk order
+    LDS double *part = flux + N.gt_off;
+    {
+        const int4 *gr = reinterpret_cast<const int4 *>(N.gt_rec);
+        for (int t = tid(); t < N.ngt; t += NT) {
+            const int4 rc = gr[t];
+            const int *src = rc.w ? N.loss_idx : N.prod_idx;
+            double acc = 0.0;
+#pragma unroll 8
+            for (int q = rc.x; q < rc.y; q++) acc += flux[src[q]];
+            part[t] = acc;
+        }
+    }
+    __syncthreads();
+    const uint64_t c3 = wall_clock64();
     double pre[EPT];
     double spre = 0.0;
 #pragma unroll
@@ -375,13 +412,10 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L)
         const int i = tid() + e * NT;
         pre[e] = 0.0;
         if (i < N.nspec) {
-            double loss = 0.0, prod = 0.0;
-            const int l0 = N.loss_ptr[i], l1 = N.loss_ptr[i + 1];
-            const int p0 = N.prod_ptr[i], p1 = N.prod_ptr[i + 1];
-#pragma unroll 4
-            for (int q = l0; q < l1; q++) loss += flux[N.loss_idx[q]];
-#pragma unroll 4
-            for (int q = p0; q < p1; q++) prod += flux[N.prod_idx[q]];
+            double prod = 0.0, loss = 0.0;
+            const int q0 = N.gt_sp_ptr[2 * i], q1 = N.gt_sp_ptr[2 * i + 1], q2 = N.gt_sp_ptr[2 * i + 2];
+            for (int q = q0; q < q1; q++) prod += part[N.gt_sp_task[q]];
+            for (int q = q1; q < q2; q++) loss += part[N.gt_sp_task[q]];
             pre[e] = prod - loss;
             if (N.is_surface[i]) spre += pre[e];
         }
@@ -424,11 +458,17 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L)
     }
     if (tid() == 0) sc[S_SPRE] = S;
     __syncthreads();
+    if (split) {   // totalSwap, photo rates + fluxes, chunked sums, species totals + transfer
+        split[0] += c1 - c0;
+        split[1] += c2 - c1;
+        split[2] += c3 - c2;
+        split[3] += wall_clock64() - c3;
+    }
 }
 
 // ---------------------------------------------------------------- analytic Jacobian
 
-__device__ __forceinline__ double fac_value(int c, const double *y, double D, double blr, double ism, double tsm)
+__device__ __forceinline__ double fac_value(int c, const LDS double *y, double D, double blr, double ism, double tsm)
 {
     if (c >= 0) return y[c];
     if (c == FAC_D) return D;
@@ -449,7 +489,7 @@ __device__ __forceinline__ double fac_deriv(int c, double dblr, double dism, dou
 // Partial derivatives of every reaction term with respect to each of its factors:
 // P[fac_off[k] + m] = rate_k * d(factor m) * prod_{m' != m} factor m'.  One 16-byte
 // record per reaction holds its (up to 8) factor codes as int16, -1 padded.
-__device__ void block_term_partials(const DevNet &N, const double *rate, const double *y, double *P, double D,
+__device__ void block_term_partials(const DevNet &N, const double *rate, const LDS double *y, LDS double *P, double D,
                                     double blr, double ism, double tsm, double dblr, double dism, double dtsm)
 {
     const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
@@ -476,7 +516,7 @@ __device__ void block_term_partials(const DevNet &N, const double *rate, const d
 }
 
 // Value of structural nonzero n of dJ (generated part): sum of coefficient * partial.
-__device__ __forceinline__ double jac_entry(const DevNet &N, const double *P, int n)
+__device__ __forceinline__ double jac_entry(const DevNet &N, const LDS double *P, int n)
 {
     double acc = 0.0;
     const int q1 = N.jnz_ptr[n + 1];
@@ -497,8 +537,9 @@ UCG_BIG __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, dou
                                     bool scaled = false)
 {
     const int neq = N.neq, ld = N.ld;
-    double *y = L.y, *rate = L.rate, *sc = L.sc, *g = L.g;
-    double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
+    LDS double *y = L.y, *sc = L.sc, *g = L.g;
+    double *rate = L.rate;
+    LDS double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
     const double D = y[neq - 1];
     const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
     const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
@@ -514,7 +555,7 @@ UCG_BIG __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, dou
     const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
     // 1. zero J; partial derivatives of every term
     for (int idx = tid(); idx < neq * ld; idx += NT) M[idx] = (LUT)0;
-    double *P = L.flux;
+    LDS double *P = L.flux;
     block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 2. generated sparse part
@@ -673,11 +714,11 @@ constexpr int LU_NB = UCG_LU_NB;
 // the trailing update A22 -= L21 U12 (threads own rows, four columns per pass).
 // `stage` is LDS scratch of >= 2*16*16 doubles.  Returns nonzero (uniformly) on a zero
 // or non-finite pivot.
-template <typename LUT>
-UCG_BIG __device__ int block_lu(LUT *M, int n, int ld, double *red, double *stage)
+template <typename LUT, typename MP>
+UCG_BIG __device__ int block_lu(MP M, int n, int ld, LDS double *red, LDS double *stage)
 {
-    double *U11 = stage;              // U11[p*16+q] = U(kb+p, kb+q)
-    double *L11 = stage + LU_NB * LU_NB;
+    LDS double *U11 = stage;          // U11[p*16+q] = U(kb+p, kb+q)
+    LDS double *L11 = stage + LU_NB * LU_NB;
     for (int kb = 0; kb < n; kb += LU_NB) {
         const int nb = min(LU_NB, n - kb);
         __syncthreads();
@@ -796,14 +837,144 @@ UCG_BIG __device__ int block_lu(LUT *M, int n, int ld, double *red, double *stag
     return 0;
 }
 
+// Broadcast of lane k's double to the whole wave without an LDS round trip.
+__device__ __forceinline__ double wave_bcast(double v, int k)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+
+// Blocked right-looking LU without pivoting for a dense block (the trailing block of the
+// sparse plan, in LDS).  Panels of LU_NB columns: the diagonal block is factored in the
+// registers of wave 0 (row r in lane r, pivots broadcast lane to lane), the panel's L21
+// rows and U12 columns are solved one per thread against reciprocal pivots, and the
+// rank-LU_NB trailing update runs over every thread in 4x4 output tiles.  ld should be
+// odd for column accesses to spread over the LDS banks.
+template <typename LUT, typename MP>
+UCG_BIG __device__ int block_lu_tiled(MP M, int n, int ld, LDS double *red, LDS double *stage)
+{
+    LDS double *U11 = stage;                  // U11[p*16+q] = U(kb+p, kb+q), 1/U(p,p) on the diagonal
+    LDS double *L11 = stage + LU_NB * LU_NB;
+    for (int kb = 0; kb < n; kb += LU_NB) {
+        const int nb = min(LU_NB, n - kb);
+        __syncthreads();
+        if (tid() < 64) {
+            const int r = tid();
+            double a[LU_NB];
+#pragma unroll
+            for (int q = 0; q < LU_NB; q++) a[q] = (r < nb && q < nb) ? (double)M[(size_t)(kb + q) * ld + kb + r] : 0.0;
+            double bad = 0.0;
+#pragma unroll
+            for (int k = 0; k < LU_NB; k++) {
+                if (k < nb) {
+                    const double piv = wave_bcast(a[k], k);
+                    if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = 1.0;
+                    const double l = (r > k && r < nb) ? a[k] / piv : 0.0;
+                    if (r > k && r < nb) a[k] = l;
+#pragma unroll
+                    for (int q = k + 1; q < LU_NB; q++) {
+                        const double uq = wave_bcast(a[q], k);
+                        if (r > k && r < nb) a[q] -= l * uq;
+                    }
+                }
+            }
+            if (r < nb) {
+#pragma unroll
+                for (int q = 0; q < LU_NB; q++) {
+                    if (q < nb) {
+                        M[(size_t)(kb + q) * ld + kb + r] = (LUT)a[q];
+                        U11[r * LU_NB + q] = (q > r) ? a[q] : (q == r ? 1.0 / a[q] : 0.0);
+                        L11[r * LU_NB + q] = (q < r) ? a[q] : 0.0;
+                    }
+                }
+            }
+            if (r == 0) red[8] = bad;
+        }
+        __syncthreads();
+        if (red[8] != 0.0) return 1;
+        const int c0 = kb + nb;
+        if (c0 >= n) break;
+        for (int w = tid(); w < 2 * (n - c0); w += NT) {   // L21 rows and U12 columns
+            double a[LU_NB];
+            if (w < n - c0) {
+                const int i = c0 + w;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++) a[p] = (p < nb) ? (double)M[(size_t)(kb + p) * ld + i] : 0.0;
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++) {
+                    if (p < nb) {
+                        double acc = a[p];
+#pragma unroll
+                        for (int q = 0; q < p; q++) acc -= a[q] * U11[q * LU_NB + p];
+                        a[p] = acc * U11[p * LU_NB + p];
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < LU_NB; p++)
+                    if (p < nb) M[(size_t)(kb + p) * ld + i] = (LUT)a[p];
+            } else {
+                const int c = c0 + (w - (n - c0));
+#pragma unroll
+                for (int r = 0; r < LU_NB; r++) a[r] = (r < nb) ? (double)M[(size_t)c * ld + kb + r] : 0.0;
+#pragma unroll
+                for (int r = 1; r < LU_NB; r++) {
+                    if (r < nb) {
+                        double acc = a[r];
+#pragma unroll
+                        for (int p = 0; p < r; p++) acc -= L11[r * LU_NB + p] * a[p];
+                        a[r] = acc;
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < LU_NB; r++)
+                    if (r < nb) M[(size_t)c * ld + kb + r] = (LUT)a[r];
+            }
+        }
+        __syncthreads();
+        const int m = n - c0;
+        const int tr = (m + 3) >> 2;
+        for (int tt = tid(); tt < tr * tr; tt += NT) {   // A22 -= L21 U12, 4x4 tiles
+            const int ti = tt % tr, tj = tt / tr;
+            const int i0 = c0 + 4 * ti, j0 = c0 + 4 * tj;
+            LUT acc[4][4];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] = (LUT)0;
+#pragma unroll
+            for (int p = 0; p < LU_NB; p++) {
+                if (p < nb) {
+                    LUT a[4], b[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) a[r] = (i0 + r < n) ? M[(size_t)(kb + p) * ld + i0 + r] : (LUT)0;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) b[c] = (j0 + c < n) ? M[(size_t)(j0 + c) * ld + kb + p] : (LUT)0;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] += a[r] * b[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (i0 + r < n && j0 + c < n) M[(size_t)(j0 + c) * ld + i0 + r] -= acc[r][c];
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
 #ifndef UCG_SB
 #define UCG_SB 32
 #endif
 constexpr int SB = UCG_SB;   // triangular-solve block: wave 0 holds the diagonal block rows
 
 // Solve (LU) x = b in place on the LDS vector b (FP64 accumulation).
-template <typename LUT>
-UCG_BIG __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
+template <typename LUT, typename MP>
+UCG_BIG __device__ void block_lu_solve(MP M, int n, int ld, LDS double *b)
 {
     const int lane = tid() & 63, wave = tid() >> 6;
     // forward, unit lower
@@ -862,18 +1033,18 @@ UCG_BIG __device__ void block_lu_solve(const LUT *M, int n, int ld, double *b)
 }
 
 // ---------------------------------------------------------------- sparse Newton system
-// The augmented Newton matrix of the build-time plan (uclchem_amd/newton_plan.py): the
-// sparse part M0 = I - gamma J0 of the generated reaction terms, photo-column and
-// transfer S-terms; two bordered unknowns carrying the rank-one totalSwap and transfer
-// couplings (columns u1 and w2, rows -gamma g1 and -gamma gsum0 with c1); the BULK and
-// SURFACE rows replaced by e_BULK - sum_b e_b and e_SURFACE - sum_s e_s.  Values are
-// written in plan slot order; fill slots start at zero.
-template <typename LUT>
-UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, double gamma)
+// J-form values (gamma = 1, no identity) of the augmented Newton matrix of the build-time
+// plan (newton_plan.py): the sparse part -J0 of the generated reaction terms, photo
+// columns and transfer S-terms; two bordered unknowns carrying the rank-one totalSwap and
+// transfer couplings (columns u1 and w2, rows -g1 and -gsum0, and c1); the BULK and
+// SURFACE rows are replaced by constant rows when the Newton matrix is formed
+// (sparse_form_newton).  Values are written in plan slot order; fill slots start at zero.
+UCG_BIG __device__ void sparse_jacobian(const DevNet &N, Lds &L, double *V)
 {
     const int neq = N.neq;
-    double *y = L.y, *rate = L.rate, *sc = L.sc;
-    double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
+    LDS double *y = L.y, *sc = L.sc;
+    double *rate = L.rate;
+    LDS double *scr = L.flux + N.npart_pad;   // scratch [>= 2*neq] beside the partials
     const double D = y[neq - 1];
     const double ysur = y[N.nsurface], ybulk = y[N.nbulk];
     const double sm = fmax(1e-30, ysur), sbk = fmax(1e-30, ybulk);
@@ -886,12 +1057,12 @@ UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, do
     const double TS = block_sum(ts, L.red);
     const double tsm = TS / sm;
     const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
-    for (int q = tid(); q < N.nw_nval; q += NT) V[q] = (LUT)0;
-    double *P = L.flux;
+    for (int q = tid(); q < N.nw_nval; q += NT) V[q] = 0;
+    LDS double *P = L.flux;
     block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
-    // 1. generated entries of J0, stored as -gamma * dJ; u1 (totalSwap column factor)
-    for (int n = tid(); n < N.nnz; n += NT) V[N.nw_jnz_slot[n]] = (LUT)(-gamma * jac_entry(N, P, n));
+    // 1. generated entries of J0, stored as -dJ; u1 (totalSwap column factor)
+    for (int n = tid(); n < N.nnz; n += NT) V[N.nw_jnz_slot[n]] = (-jac_entry(N, P, n));
     for (int i = tid(); i < neq; i += NT) {
         double u = 0.0;
         for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {
@@ -922,39 +1093,29 @@ UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, do
         if (r2 >= 0) {
             double acc = 0.0;
             for (int q = N.nw_gs_ptr[j]; q < N.nw_gs_ptr[j + 1]; q++) acc += (double)V[N.nw_gs_slot[q]];
-            V[r2] = (LUT)acc;
+            V[r2] = acc;
         }
         const int r1 = N.nw_s1r[j];
-        if (r1 >= 0) V[r1] = (LUT)(-gamma * scr[neq + j]);
+        if (r1 >= 0) V[r1] = -scr[neq + j];
         const int cc = N.nw_s1c[j];
-        if (cc >= 0) V[cc] = (LUT)scr[j];
+        if (cc >= 0) V[cc] = scr[j];
     }
-    for (int q = tid(); q < N.nbulkl; q += NT) V[N.nw_nbrow[q]] = (LUT)(-1.0);
-    for (int q = tid(); q < N.nsurfl; q += NT) V[N.nw_nsrow[q]] = (LUT)(-1.0);
     double cs = 0.0;
     for (int q = tid(); q < N.nsurfl; q += NT) cs += scr[N.surface_list[q]];
     const double c1 = block_sum(cs, L.red);
     if (tid() == 0) {
-        V[N.nw_misc[0]] = (LUT)(-1.0);
-        V[N.nw_misc[1]] = (LUT)(-1.0);
-        V[N.nw_misc[2]] = (LUT)c1;
-        V[N.nw_misc[3]] = (LUT)1.0;
-        V[N.nw_misc[4]] = (LUT)1.0;
+        V[N.nw_misc[2]] = c1;
     }
     __syncthreads();
     // 3. identity; transfer column w2 (-t_s on the surface row, +t_s on its bulk partner)
-    for (int i = tid(); i < neq; i += NT) {
-        const int d = N.nw_diag_slot[i];
-        if (d >= 0) V[d] = (LUT)((double)V[d] + 1.0);
-    }
     const double S = sc[S_SPRE];
     const bool neg = S < 0;
     const bool ratio_lt1 = (sbk / sm) < 1.0;
     for (int q = tid(); q < N.nsurfl; q += NT) {
         const int s_ = N.surface_list[q], b = N.bulk_partner_of_surface[s_];
         const double t = neg ? (ratio_lt1 ? y[b] / sm : y[b] / sbk) : SURFACE_COVERAGE0 * y[s_];
-        V[N.nw_s2c[s_]] = (LUT)(-t);
-        V[N.nw_s2c[b]] = (LUT)t;
+        V[N.nw_s2c[s_]] = (-t);
+        V[N.nw_s2c[b]] = t;
     }
     __syncthreads();
     // 4. photo-rate columns (H2, CO, density) and the free-fall density row; transfer S-terms
@@ -992,14 +1153,14 @@ UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, do
             for (int q = N.aff_ptr[k]; q < N.aff_ptr[k + 1]; q++, m3 += 3) {
                 const double c = N.aff_cf[q] * pf;
                 const int *ps = N.nw_ph_slot + m3;
-                V[ps[0]] = (LUT)((double)V[ps[0]] - gamma * c * d_h2);
-                V[ps[1]] = (LUT)((double)V[ps[1]] - gamma * c * d_co);
-                V[ps[2]] = (LUT)((double)V[ps[2]] - gamma * c * d_d);
+                V[ps[0]] = ((double)V[ps[0]] - c * d_h2);
+                V[ps[1]] = ((double)V[ps[1]] - c * d_co);
+                V[ps[2]] = ((double)V[ps[2]] - c * d_d);
             }
         }
         const double dn = D * 1e-7;
         const int dd = N.nw_diag_slot[neq - 1];
-        V[dd] = (LUT)((double)V[dd] - gamma * (densdot_dev(L.prm, D + dn) - densdot_dev(L.prm, D)) / dn);
+        V[dd] = ((double)V[dd] - (densdot_dev(L.prm, D + dn) - densdot_dev(L.prm, D)) / dn);
     }
     for (int q = tid(); q < N.nsurfl; q += NT) {
         const int s_ = N.surface_list[q], b = N.bulk_partner_of_surface[s_];
@@ -1014,11 +1175,34 @@ UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, do
         const int *ts_ = N.nw_tr_slot + 8 * q;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-            if (d[m] != 0.0) {   // J[s][c] -= d, J[b][c] += d  ->  M = I - gamma J
-                V[ts_[m]] = (LUT)((double)V[ts_[m]] + gamma * d[m]);
-                V[ts_[4 + m]] = (LUT)((double)V[ts_[4 + m]] - gamma * d[m]);
+            if (d[m] != 0.0) {   // J[s][c] -= d, J[b][c] += d, stored as -dJ
+                V[ts_[m]] = ((double)V[ts_[m]] + d[m]);
+                V[ts_[4 + m]] = ((double)V[ts_[4 + m]] - d[m]);
             }
         }
+    }
+    __syncthreads();
+}
+
+// M = I - gamma J in plan slots from the saved J-form values (sparse_jacobian): entries
+// that carry gamma are scaled, the rank-one column factors and c1 are not; then the
+// identity, the border diagonals and the replaced BULK/SURFACE rows.
+template <typename LUT>
+UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT *V, double gamma)
+{
+    for (int q = tid(); q < N.nw_nval; q += NT) V[q] = (LUT)(N.nw_unscaled[q] ? J[q] : gamma * J[q]);
+    __syncthreads();
+    for (int i = tid(); i < N.neq; i += NT) {
+        const int d = N.nw_diag_slot[i];
+        if (d >= 0) V[d] = (LUT)(gamma * J[d] + 1.0);
+    }
+    for (int q = tid(); q < N.nbulkl; q += NT) V[N.nw_nbrow[q]] = (LUT)(-1.0);
+    for (int q = tid(); q < N.nsurfl; q += NT) V[N.nw_nsrow[q]] = (LUT)(-1.0);
+    if (tid() == 0) {
+        V[N.nw_misc[0]] = (LUT)(-1.0);
+        V[N.nw_misc[1]] = (LUT)(-1.0);
+        V[N.nw_misc[3]] = (LUT)1.0;
+        V[N.nw_misc[4]] = (LUT)1.0;
     }
     __syncthreads();
 }
@@ -1028,17 +1212,29 @@ UCG_BIG __device__ void sparse_newton_matrix(const DevNet &N, Lds &L, LUT *V, do
 // FP32 factors are staged in LDS (region) for the whole factorisation; FP64 factors stay
 // in the workspace and only the dense tail is staged for its LU.  The factors end in V.
 // stage: 512 doubles of LDS.
-template <typename LUT>
-__device__ __forceinline__ void crout_phases(const DevNet &N, LUT *W, double *red)
+template <typename LUT, typename WP>
+__device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red)
 {
     for (int p = 0; p < N.nw_nph; p++) {
         const int t1 = N.nw_ph_ptr[p + 1];
         for (int t = N.nw_ph_ptr[p] + tid(); t < t1; t += NT) {
             const int tg = N.nw_tg_slot[t];
-            double acc = (double)W[tg];
+            double a0 = (double)W[tg], a1 = 0.0, a2 = 0.0, a3 = 0.0;
             const int q1 = N.nw_tg_ptr[t + 1];
-#pragma unroll 4
-            for (int q = N.nw_tg_ptr[t]; q < q1; q++) acc -= (double)W[N.nw_pr_l[q]] * (double)W[N.nw_pr_u[q]];
+            const int2 *pr = reinterpret_cast<const int2 *>(N.nw_pr);
+            int q = N.nw_tg_ptr[t];
+            for (; q + 4 <= q1; q += 4) {
+                const int2 p0 = pr[q], p1 = pr[q + 1], p2 = pr[q + 2], p3 = pr[q + 3];
+                a0 -= (double)W[p0.x] * (double)W[p0.y];
+                a1 -= (double)W[p1.x] * (double)W[p1.y];
+                a2 -= (double)W[p2.x] * (double)W[p2.y];
+                a3 -= (double)W[p3.x] * (double)W[p3.y];
+            }
+            for (; q < q1; q++) {
+                const int2 p0 = pr[q];
+                a0 -= (double)W[p0.x] * (double)W[p0.y];
+            }
+            double acc = (a0 + a1) + (a2 + a3);
             const int dv = N.nw_tg_div[t];
             if (dv >= 0) {
                 const double piv = (double)W[dv];
@@ -1057,52 +1253,89 @@ __device__ __forceinline__ void crout_phases(const DevNet &N, LUT *W, double *re
 }
 
 template <typename LUT>
-UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, double *region, double *red, double *stage)
+UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LDS double *red, LDS double *stage,
+                                 uint64_t *split = nullptr)
 {
     const int nval = N.nw_nval, nsp = N.nw_nsp, T = N.nw_T;
     if (tid() == 0) red[9] = 0.0;
     int r = 0;
+    const uint64_t c0 = wall_clock64();
+    uint64_t c1;
     if constexpr (sizeof(LUT) == 4) {
         LUT *Vl = (LUT *)region;
         for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
         __syncthreads();
         crout_phases<LUT>(N, Vl, red);
+        c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
-        if (T > 0) r = block_lu<LUT>(Vl + nsp, T, T, red, stage);
+        if (T > 0) r = block_lu_tiled<LUT>(Vl + nsp, T, T, red, stage);
         for (int q = tid(); q < nval; q += NT) V[q] = Vl[q];
     } else {
         __syncthreads();
         crout_phases<LUT>(N, V, red);
+        c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
         LUT *tl = (LUT *)region;
         for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
         __syncthreads();
-        if (T > 0) r = block_lu<LUT>(tl, T, T, red, stage);
+        if (T > 0) r = block_lu_tiled<LUT>(tl, T, T, red, stage);
         for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
     }
     __syncthreads();
+    if (split) {
+        split[0] += c1 - c0;
+        split[1] += wall_clock64() - c1;
+    }
     return r;
 }
 
 // Solve M x = b with the factors of sparse_lu: b (length neq, in LDS) is overwritten by x.
-// region: LDS scratch; FP32 factors are staged there (read-only copy) and the permuted
-// work vector follows them, FP64 factors are read from the workspace.
+// The solve program (row records in level order, column indices, permutation) and the
+// sparse factors are staged in LDS (region) so that every dependent access of the
+// level-scheduled substitutions is an LDS access; FP32 factors bring their dense tail
+// too, FP64 tails are read from the workspace by the dense triangular solves.
 template <typename LUT>
-UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, double *b, double *region)
+UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, LDS double *b, LDS double *region,
+                                       uint64_t *split = nullptr)
 {
-    const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns;
-    const LUT *V = Vg;
-    double *w = region;
-    if constexpr (sizeof(LUT) == 4) {   // FP32 factors: read-only copy in LDS, work vector after it
-        LUT *Vs = (LUT *)region;
-        for (int q = tid(); q < N.nw_nval; q += NT) Vs[q] = Vg[q];
-        V = Vs;
-        w = region + (((N.nw_nval + 1) / 2 + 7) & ~7);
+    const uint64_t c0 = wall_clock64();
+    uint64_t c1 = c0, c2 = c0;
+    const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns, nsp = N.nw_nsp, T = N.nw_T;
+    constexpr bool STAGE_TAIL = sizeof(LUT) == 4;
+    const int nst = STAGE_TAIL ? N.nw_nval : nsp;                       // staged factor entries
+    // layout in bytes, 16-byte aligned parts (solve_stage_bytes sizes the same)
+    LDS char *base = (LDS char *)region;
+    int off = 0;
+    LDS LUT *Vs = (LDS LUT *)base;
+    off = (nst * (int)sizeof(LUT) + 15) & ~15;
+    LDS int *cols = (LDS int *)(base + off);
+    off += (nsp * 4 + 15) & ~15;
+    LDS int *recs = (LDS int *)(base + off);   // 4 ints per row: {row, first, diagonal, end}
+    off += NN * 16;
+    LDS int *inv = (LDS int *)(base + off);
+    off += ((NN + N.nw_nlev + 1) * 4 + 15) & ~15;   // permutation, then the level pointers
+    LDS double *w = (LDS double *)(base + off);
+    for (int q = tid(); q < nst; q += NT) Vs[q] = Vg[q];
+    for (int q = tid(); q < nsp; q += NT) cols[q] = N.nw_sp_col[q];
+    {
+        const int4 *gr = reinterpret_cast<const int4 *>(N.sv_rec);
+        for (int q = tid(); q < NN; q += NT) {
+            const int4 g = gr[q];
+            recs[4 * q] = g.x;
+            recs[4 * q + 1] = g.y;
+            recs[4 * q + 2] = g.z;
+            recs[4 * q + 3] = g.w;
+        }
     }
+    for (int q = tid(); q < NN; q += NT) inv[q] = N.nw_inv[q];
+    LDS int *lev = inv + NN;   // level pointers (nlev + 1), after the permutation
+    for (int q = tid(); q <= N.nw_nlev; q += NT) lev[q] = N.nw_lev_ptr[q];
+    const LDS LUT *V = Vs;
     double sb = 0.0, ss = 0.0;
     for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
     for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
-    block_sum2(sb, ss, L.red);
+    block_sum2(sb, ss, L.red);   // its barriers also publish the staged program
+    const uint64_t cs = wall_clock64();
     for (int i = tid(); i < NN; i += NT) {
         double v = 0.0;
         if (i < neq) {
@@ -1110,42 +1343,72 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, 
             if (i == N.nbulk) v -= sb;
             else if (i == N.nsurface) v -= ss;
         }
-        w[N.nw_inv[i]] = v;
+        w[inv[i]] = v;
     }
     __syncthreads();
+    const uint64_t cf0 = wall_clock64();
     for (int l = 0; l < N.nw_nlev; l++) {   // forward, sparse rows by level
-        const int q1 = N.nw_lev_ptr[l + 1];
-        for (int q = N.nw_lev_ptr[l] + tid(); q < q1; q += NT) {
-            const int r = N.nw_lev_rows[q];
-            double acc = w[r];
-            const int e1 = N.nw_sp_dpos[r];
-            for (int e = N.nw_sp_ptr[r]; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
-            w[r] = acc;
+        const int q1 = lev[l + 1];
+        for (int q = lev[l] + tid(); q < q1; q += NT) {
+            const int r = recs[4 * q], e0 = recs[4 * q + 1], ed = recs[4 * q + 2];
+            double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;   // four chains hide the FMA latency
+            int e = e0;
+            for (; e + 4 <= ed; e += 4) {
+                a0 -= (double)V[e] * w[cols[e]];
+                a1 -= (double)V[e + 1] * w[cols[e + 1]];
+                a2 -= (double)V[e + 2] * w[cols[e + 2]];
+                a3 -= (double)V[e + 3] * w[cols[e + 3]];
+            }
+            for (; e < ed; e++) a0 -= (double)V[e] * w[cols[e]];
+            w[r] = (a0 + a1) + (a2 + a3);
         }
         __syncthreads();
     }
-    for (int r = ns + tid(); r < NN; r += NT) {   // tail rows: L21
-        double acc = w[r];
-        const int e1 = N.nw_sp_ptr[r + 1];
-        for (int e = N.nw_sp_ptr[r]; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
-        w[r] = acc;
+    for (int q = ns + tid(); q < NN; q += NT) {   // tail rows: L21
+        const int r = recs[4 * q], e0 = recs[4 * q + 1], e1 = recs[4 * q + 3];
+        double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int e = e0;
+        for (; e + 4 <= e1; e += 4) {
+            a0 -= (double)V[e] * w[cols[e]];
+            a1 -= (double)V[e + 1] * w[cols[e + 1]];
+            a2 -= (double)V[e + 2] * w[cols[e + 2]];
+            a3 -= (double)V[e + 3] * w[cols[e + 3]];
+        }
+        for (; e < e1; e++) a0 -= (double)V[e] * w[cols[e]];
+        w[r] = (a0 + a1) + (a2 + a3);
     }
     __syncthreads();
-    if (N.nw_T > 0) block_lu_solve<LUT>(V + N.nw_nsp, N.nw_T, N.nw_T, w + ns);
+    c1 = wall_clock64();
+    if (T > 0) {
+        if constexpr (STAGE_TAIL) block_lu_solve<LUT>(Vs + nsp, T, T, w + ns);
+        else block_lu_solve<LUT>(Vg + nsp, T, T, w + ns);
+    }
+    c2 = wall_clock64();
     for (int l = N.nw_nlev - 1; l >= 0; l--) {   // backward, sparse rows by level
-        const int q1 = N.nw_lev_ptr[l + 1];
-        for (int q = N.nw_lev_ptr[l] + tid(); q < q1; q += NT) {
-            const int r = N.nw_lev_rows[q];
-            const int dp = N.nw_sp_dpos[r];
-            double acc = w[r];
-            const int e1 = N.nw_sp_ptr[r + 1];
-            for (int e = dp + 1; e < e1; e++) acc -= (double)V[e] * w[N.nw_sp_col[e]];
-            w[r] = acc / (double)V[dp];
+        const int q1 = lev[l + 1];
+        for (int q = lev[l] + tid(); q < q1; q += NT) {
+            const int r = recs[4 * q], ed = recs[4 * q + 2], e1 = recs[4 * q + 3];
+            double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int e = ed + 1;
+            for (; e + 4 <= e1; e += 4) {
+                a0 -= (double)V[e] * w[cols[e]];
+                a1 -= (double)V[e + 1] * w[cols[e + 1]];
+                a2 -= (double)V[e + 2] * w[cols[e + 2]];
+                a3 -= (double)V[e + 3] * w[cols[e + 3]];
+            }
+            for (; e < e1; e++) a0 -= (double)V[e] * w[cols[e]];
+            w[r] = ((a0 + a1) + (a2 + a3)) / (double)V[ed];
         }
         __syncthreads();
     }
-    for (int i = tid(); i < neq; i += NT) b[i] = w[N.nw_inv[i]];
+    for (int i = tid(); i < neq; i += NT) b[i] = w[inv[i]];
     __syncthreads();
+    if (split) {
+        split[0] += (c1 - c0) + (wall_clock64() - c2);   // staging, sparse rows, permutations
+        split[1] += c2 - c1;                             // dense tail
+        split[2] += cs - c0;                             // staging + right-hand-side sums
+        split[3] += c1 - cf0;                            // forward levels + tail rows
+    }
 }
 
 // ---------------------------------------------------------------- BDF ops policy
@@ -1158,8 +1421,11 @@ struct GpuOps {
     int neq;
     double z[6][EPT], acor[EPT], ewt[EPT], atol[EPT], f0[EPT];
     uint64_t t_rhs = 0, t_jac = 0, t_lu = 0, t_sol = 0;   // device wall-clock ticks per phase
+    uint64_t t_lu_split[2] = {0, 0}, t_sol_split[4] = {0, 0, 0, 0};   // sparse part, dense tail, ...
+    uint64_t t_rhs_split[4] = {0, 0, 0, 0};
     bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M (dense form only)
     bool sparse = true;         // static-pattern sparse plan (else dense LU of the full matrix)
+    double *Jsave = nullptr;    // saved J-form values of the sparse plan (workspace)
     const int *cancel = nullptr;  // device flag the host may raise while the kernel runs
     __device__ bool cancelled() const
     {
@@ -1242,7 +1508,7 @@ struct GpuOps {
     __device__ void rhs()
     {
         uint64_t t0 = wall_clock64();
-        block_rhs(N, L);
+        block_rhs(N, L, t_rhs_split);
         t_rhs += wall_clock64() - t0;
     }
     __device__ void save_f0()
@@ -1326,7 +1592,7 @@ struct GpuOps {
     __device__ void solve()
     {
         uint64_t t0 = wall_clock64();
-        if (sparse) sparse_lu_solve<LUT>(N, L, M, L.b, L.flux);
+        if (sparse) sparse_lu_solve<LUT>(N, L, M, L.b, L.flux, t_sol_split);
         else block_lu_solve<LUT>(M, neq, N.ld, L.b);
         t_sol += wall_clock64() - t0;
     }
@@ -1347,7 +1613,12 @@ struct GpuOps {
         double r = sqrt(block_sum(acc, L.red) / neq);
         return r;
     }
-    __device__ int setup(double gamma)
+    // Reusing a saved J between Newton-matrix setups (DVODE's JSV=1) was measured on the
+    // seeded grid: 31% more steps (slower Newton convergence) for 7x fewer J evaluations,
+    // a net loss since an analytic J costs under half a factorisation here; kept off.
+    bool reuse_jacobian = false;
+    __device__ bool can_reuse_jacobian() const { return sparse && reuse_jacobian; }
+    __device__ int setup(double gamma, bool new_jacobian)
     {
 #pragma unroll
         for (int e = 0; e < EPT; e++)
@@ -1357,9 +1628,10 @@ struct GpuOps {
         int r;
         uint64_t t1;
         if (sparse) {
-            sparse_newton_matrix<LUT>(N, L, M, gamma);
+            if (new_jacobian) sparse_jacobian(N, L, Jsave);
+            sparse_form_newton<LUT>(N, Jsave, M, gamma);
             t1 = wall_clock64();
-            r = sparse_lu<LUT>(N, M, L.flux, L.red, L.b);
+            r = sparse_lu<LUT>(N, M, L.flux, L.red, L.b, t_lu_split);
         } else {
             block_newton_matrix<LUT>(N, L, M, gamma, true, equilibrate);
             t1 = wall_clock64();
@@ -1427,8 +1699,9 @@ struct SolveArgs {
     double *t_start;          // [n] device wall clock when the point was taken, or null
     double *t_done;           // [n] device wall clock at retirement, or null
     void *ws;                 // slot workspaces
-    size_t ws_stride;         // bytes per slot: Newton matrix (ws_mat_bytes) then rate[nreac]
-    size_t ws_mat_bytes;
+    size_t ws_stride;         // bytes per slot: Newton matrix (ws_mat_bytes), saved J-form values
+    size_t ws_mat_bytes;      // (ws_jac_bytes, sparse plan only), then rate[nreac]
+    size_t ws_jac_bytes;
     int *counter;             // work queue head
     const int *cancel;        // nullable device flag: nonzero stops the batch (points not done get
                               // STATUS_CANCELLED)
@@ -1436,7 +1709,7 @@ struct SolveArgs {
     int sparse;               // 1: static-pattern sparse plan, 0: dense LU
 };
 
-__device__ void elemental_start(const DevNet &N, const double *prm, double *y, double density)
+__device__ void elemental_start(const DevNet &N, const LDS double *prm, LDS double *y, double density)
 {   // chemistry.f90:63-105 with the default elemental abundances (defaultparameters.f90:68-90)
     const int neq = N.neq;
     for (int i = tid(); i < neq; i += NT) y[i] = MIN_ABUND;
@@ -1488,7 +1761,7 @@ __device__ void lds_layout(const DevNet &N, Lds &L, double *lds_raw, double *rat
 {
     const int npad = (N.neq + 7) & ~7;
     L.rate = rate;
-    L.flux = lds_raw;
+    L.flux = (LDS double *)lds_raw;
     L.y = L.flux + flux_region_len(N);
     L.f = L.y + npad;
     L.b = L.f + npad;
@@ -1517,7 +1790,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
     const int neq = N.neq;
     Lds L;
     char *slot = (char *)A.ws + (size_t)blockIdx.x * A.ws_stride;
-    lds_layout(N, L, lds_raw, (double *)(slot + A.ws_mat_bytes));
+    lds_layout(N, L, lds_raw, (double *)(slot + A.ws_mat_bytes + A.ws_jac_bytes));
     __shared__ int s_point;
     LUT *M = (LUT *)slot;
 
@@ -1536,7 +1809,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
         const double *pin = A.params + (size_t)pidx * P_NPARAM;
         for (int i = tid(); i < P_NPARAM; i += NT) L.prm[i] = pin[i];
         __syncthreads();
-        const double *prm = L.prm;
+        const LDS double *prm = L.prm;
         if (tid() == 0) {
             double cs = (prm[P_ROUT] - prm[P_RIN]) * PC;
             L.sc[S_CS] = cs;
@@ -1559,6 +1832,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
         }
         GpuOps<LUT> o(N, L, M);
         o.sparse = A.sparse != 0;
+        o.Jsave = (double *)(slot + A.ws_mat_bytes);
         o.cancel = A.cancel;
         o.equilibrate = !o.sparse && A.equilibrate != 0;
         bdf::Stats st;
@@ -1663,12 +1937,16 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
             if (A.t_done) A.t_done[pidx] = (double)wall_clock64();
             if (A.timers) {
                 const double tick = 1.0e-8;   // wall_clock64 runs at 100 MHz
-                double *tm = A.timers + (size_t)pidx * 8;
+                double *tm = A.timers + (size_t)pidx * 16;
                 tm[0] = t_rates * tick; tm[1] = o.t_rhs * tick; tm[2] = o.t_jac * tick; tm[3] = o.t_lu * tick;
                 const uint64_t dw = wall_clock64() - t_point0, dc = clock64() - c_point0;
                 tm[4] = o.t_sol * tick; tm[5] = dw * tick;
                 tm[6] = dw ? (double)dc / ((double)dw * tick) * 1e-6 : 0.0;   // mean shader clock, MHz
-                tm[7] = 0;
+                tm[7] = o.t_lu_split[0] * tick; tm[8] = o.t_lu_split[1] * tick;
+                tm[9] = o.t_sol_split[0] * tick; tm[10] = o.t_sol_split[1] * tick;
+                tm[11] = o.t_sol_split[2] * tick;   // solve: staging + right-hand-side sums
+                for (int k = 1; k < 4; k++) tm[11 + k] = o.t_rhs_split[k] * tick;
+                tm[15] = o.t_sol_split[3] * tick;   // solve: forward levels (staging in [9])
             }
         }
         __syncthreads();
@@ -1690,8 +1968,9 @@ struct PieceArgs {
     double gamma;
     int sparse;
     void *ws;
-    size_t ws_stride;      // bytes per point: Newton matrix (ws_mat_bytes) then rate[nreac]
-    size_t ws_mat_bytes;
+    size_t ws_stride;      // bytes per point: Newton matrix (ws_mat_bytes), J-form values
+    size_t ws_mat_bytes;   // (ws_jac_bytes), then rate[nreac]
+    size_t ws_jac_bytes;
 };
 
 __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
@@ -1720,7 +1999,7 @@ __global__ __launch_bounds__(NT) void rates_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1738,7 +2017,7 @@ __global__ __launch_bounds__(NT) void rhs_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1752,7 +2031,7 @@ __global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     piece_setup(N, L, A, p);
@@ -1772,7 +2051,7 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
 {
     extern __shared__ double lds_raw[];
     Lds L;
-    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes));
+    lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
     const int p = blockIdx.x;
     if (p >= A.n) return;
     // y holds the state for J; the right-hand side is rate_in reinterpreted as [n][neq]
@@ -1784,7 +2063,9 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
     LUT *M = (LUT *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride);
     int bad;
     if (A.sparse) {
-        sparse_newton_matrix<LUT>(N, L, M, A.gamma);
+        double *J = (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes);
+        sparse_jacobian(N, L, J);
+        sparse_form_newton<LUT>(N, J, M, A.gamma);
         bad = sparse_lu<LUT>(N, M, L.flux, L.red, L.b);
     } else {
         block_newton_matrix<LUT>(N, L, M, A.gamma);

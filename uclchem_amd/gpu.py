@@ -151,7 +151,7 @@ class Network:
         y_out = np.zeros((n, self.neq))
         status = np.zeros(n, dtype=np.int32)
         stats = np.zeros((n, 8), dtype=np.int32)
-        timers = np.zeros((n, 8))
+        timers = np.zeros((n, 16))
         _check(lib().ucg_solve_batch_profiled(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(y_out),
                                               _p(status), _p(stats), _p(timers), ctypes.c_int(lu_bits),
                                               ctypes.c_int(n_slots)), "ucg_solve_batch_profiled")

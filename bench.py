@@ -114,18 +114,35 @@ def main():
     ap.add_argument("--cpu-cap", type=float, default=25.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump", default="", help="write per-point counters and times to <prefix>_fp<bits>.npz")
+    ap.add_argument("--workload", default="grid", choices=["grid", "collapse500", "hotcore", "serving"],
+                    help="grid: 4096-point static cloud grid (default); collapse500: free-fall grid on the "
+                         "500-species synthetic network; hotcore: hot-core warm-up grid; serving: Poisson "
+                         "request stream over three networks")
+    ap.add_argument("--requests", type=int, default=192, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson arrival rate, requests/s")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; grid points are independent and would shard "
                           "by contiguous point ranges with no collective (see DESIGN.md)", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "serving":
+        return bench_serving(a)
     import torch
     from uclchem_amd import gpu, workloads
     from uclchem_amd import makerates_hip as mh
 
-    tables = mh.load_network(mh.network_path("default"))
+    if a.workload == "collapse500":
+        from uclchem_amd import synthetic as sy
+        tables = sy.load_or_build("synthetic_500_6000")
+        if a.points == 4096:
+            a.points = 8192
+        params = workloads.collapse_grid(a.points)
+        a.no_cpu = True              # the reference is compiled for its default network only
+    else:
+        tables = mh.load_network(mh.network_path("default"))
+        params = (workloads.hot_core_grid(a.points) if a.workload == "hotcore"
+                  else workloads.static_cloud_grid(a.points))
     net = gpu.Network(tables)
-    params = workloads.static_cloud_grid(a.points)
     dev = torch.device("cuda:0")
     n, neq = len(params), tables.neq
     d_params = torch.tensor(params, dtype=torch.float64, device=dev)
@@ -185,14 +202,19 @@ def main():
     lu_flop, solve_flop = newton_flop_counts(tables)
     flops = algorithmic_flops(main_r["stats"], lu_flop, solve_flop, rhs_flop_count(tables))
     achieved = flops / (main_r["ms"] / 1e3)
+    wl = {"grid": (f"static cloud grid, default network ({tables.nspec} species, {tables.nreac} reactions), "
+                   f"{n} points to 1e6 yr, FP{a.lu} factorisation" + (" (plain batched FP64)" if a.lu == 64 else "")),
+          "collapse500": (f"free-fall collapse grid (n_H,0 log-uniform 1e2..1e6 towards 1e6 cm^-3, 1 Myr), "
+                          f"seeded synthetic network ({tables.nspec} species, {tables.nreac} reactions), {n} points, "
+                          f"FP{a.lu} factorisation"),
+          "hotcore": (f"hot-core warm-up grid (temp_indx 4, 10->300 K, log n_H U[5,8]), default network, {n} "
+                      f"points to 1e6 yr, FP{a.lu} factorisation")}[a.workload]
     line = {
         "metric": "grid-point solves/s to 1 Myr (p50/p99 solve latency, 1% abundance agreement)",
         "value": round(value, 3), "unit": "grid-point solves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(main_r["ms"], 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded 4096-point static-cloud grid)",
-        "config": {"workload": f"static cloud grid, default network ({tables.nspec} species, {tables.nreac} "
-                               f"reactions), {n} points to 1e6 yr, FP64 (plain batched)",
-                   "points": n, "network": "default", "lu": f"fp{a.lu}"},
+        "vs_baseline": None, "dtype": "f64", "data": f"synthetic (seeded {n}-point {a.workload} workload)",
+        "config": {"workload": wl, "points": n, "network": tables.name, "lu": f"fp{a.lu}"},
         "latency_s": {"p50": float(np.percentile(main_r["lat"], 50)), "p99": float(np.percentile(main_r["lat"], 99)),
                       "solve_p50": float(np.percentile(main_r["solve"], 50)),
                       "solve_p99": float(np.percentile(main_r["solve"], 99))},
@@ -222,6 +244,54 @@ def main():
         line["cpu_baseline"] = cpu_baseline(params, cores, a.cpu_cap, a.cpu_sample)
     else:
         line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(a):
+    """Poisson stream of single-point requests over three resident networks (default,
+    150/1500 and 500/6000 synthetic), t_end log-uniform in [1e3, 1e7] yr, request queues
+    with continuous admission; throughput = requests / wall, latency = arrival -> result."""
+    from uclchem_amd import gpu, serving
+    from uclchem_amd import makerates_hip as mh
+    from uclchem_amd import synthetic as sy
+    names = ["synthetic_150_1500", "default", "synthetic_500_6000"]
+    nets = {"default": gpu.Network(mh.load_network(mh.network_path("default")))}
+    for nm in names:
+        if nm != "default":
+            nets[nm] = gpu.Network(sy.load_or_build(nm))
+    import torch
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    budget = 64e9                                   # HBM the serving state may take
+    slots = serving.plan_slots(nets, n_cu, budget, lu_bits=a.lu)
+    reqs = serving.poisson_requests(a.requests, a.rate, names)
+    srv = serving.Server(nets, slots, capacity=a.requests + 8, lu_bits=a.lu)
+    try:
+        warm = [(0.0, nm, serving.make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e3)) for nm in names]
+        serving.run_stream(srv, warm, timeout_s=120.0)
+        t0 = time.time()
+        done = serving.run_stream(srv, reqs, timeout_s=1100.0,
+                                  progress=lambda d, s, t: print(f"[bench] serving: {d}/{s} done at {t:.0f} s",
+                                                                 file=sys.stderr, flush=True))
+        wall = time.time() - t0
+    finally:
+        srv.close()
+    lat = np.array([t.t_done - t.t_arrival for t in done])
+    by_net = {nm: np.array([t.t_done - t.t_arrival for t in done if t.network == nm]) for nm in names}
+    line = {
+        "metric": "serving: completed single-point solves/s with p50/p99 latency (arrival -> result)",
+        "value": round(len(done) / wall, 3), "unit": "requests/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(wall * 1e3, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"synthetic (seeded Poisson stream, lambda={a.rate}/s, {a.requests} requests)",
+        "config": {"workload": "serving loop: Poisson single-point requests over the default network and the "
+                               "150/1500 and 500/6000 synthetic networks, t_end log-uniform 1e3..1e7 yr, "
+                               "request queues with continuous admission, slots by HBM budget",
+                   "slots": slots, "hbm_budget_gb": budget / 1e9, "lu": f"fp{a.lu}"},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "per_network_p50": {k: float(np.percentile(v, 50)) for k, v in by_net.items() if len(v)},
+                      "per_network_p99": {k: float(np.percentile(v, 99)) for k, v in by_net.items() if len(v)}},
+        "status_nonzero": int(sum(t.status != 0 for t in done)),
+        "roofline": None, "cpu_baseline": None,
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -164,6 +164,30 @@ int ucg_set_linear_solver(int handle, int kind);
 /* Network sizes of a loaded handle. */
 int ucg_network_info(int handle, int *nspec, int *neq, int *nreac);
 
+/* Request queues (serving).  The reference solves one model per call
+ * (uclchemwrap.cloud, wrap.f90:15-80, driven from uclchem.model.cloud, model.py:277-289);
+ * a queue keeps a persistent solve kernel resident on one network and admits single-point
+ * requests while it runs: a workgroup that finishes a request takes the next posted one,
+ * so requests arriving while others are on the GPU start as soon as a slot frees up.
+ * Each request is a full model to params[UCG_P_FINALTIME] with the batch semantics
+ * (status codes, counters, results bit-identical to ucg_solve_batch of the same point).
+ *   ucg_queue_slot_bytes: workspace of one slot (for admission by HBM budget);
+ *   ucg_queue_open: capacity = requests the queue will ever take, n_slots resident
+ *     workgroups (<= CUs; queues open together should not exceed the CU count in total),
+ *     idle_timeout_s: a workgroup waiting this long for a request leaves (bounds the kernel
+ *     if the caller never closes); the queue's workspace and pinned buffers are its own;
+ *   ucg_queue_submit: posts one request (params [UCG_NPARAM]), returns its index;
+ *   ucg_queue_test: *done = 1 once the request's results are available (error if the
+ *     kernel left before running it);
+ *   ucg_queue_result: y [neq], status, stats [8] of a finished request;
+ *   ucg_queue_close: admits nothing more, waits for posted requests, frees the queue. */
+int ucg_queue_slot_bytes(int handle, int lu_bits, size_t *bytes);
+int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double idle_timeout_s, int *queue);
+int ucg_queue_submit(int queue, const double *params, int *request);
+int ucg_queue_test(int queue, int request, int *done);
+int ucg_queue_result(int queue, int request, double *y, int *status, int *stats);
+int ucg_queue_close(int queue);
+
 #ifdef __cplusplus
 }
 #endif

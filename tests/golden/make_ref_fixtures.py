@@ -8,6 +8,8 @@ Outputs (small .npz files next to this script):
   ref_cloud_point.npz    the reference model for the default static cloud point
   ref_cloud_grid.npz     the reference model for the first N seeded grid points
   ref_hot_core.npz       the reference model for seeded hot-core points
+  example_collapse_hotcore.npz  examples/example-output/phase1-full.dat, startcollapse.dat,
+                         phase2-full.dat (the collapse and hot-core example models)
 """
 from __future__ import annotations
 
@@ -48,11 +50,36 @@ def run_models(params, procs):
     return rc, final, stats, wall, nrows
 
 
+def _read_output(path):
+    """Header names and rows of a reference full-output file (io.f90:85-88 format)."""
+    lines = open(path).read().splitlines()
+    hdr = [h.strip() for h in lines[0].split(",")]
+    rows = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+    return hdr, rows
+
+
+def example_models():
+    """The collapse and hot-core example outputs the reference ships
+    (scripts/run_uclchem_tests.py:28-58): phase 1 (free-fall 1e2 -> 1e5 cm^-3, stop at the
+    final density), its saved final abundances, phase 2 (hot core, temp_indx 3, 300 K, from
+    those abundances).  Data only; no reference code runs."""
+    d = os.path.dirname(REF_EXAMPLE)
+    hdr, r1 = _read_output(os.path.join(d, "phase1-full.dat"))
+    start = np.array([float(x) for x in open(os.path.join(d, "startcollapse.dat")).read().replace(",", " ").split()])
+    hdr2, r2 = _read_output(os.path.join(d, "phase2-full.dat"))
+    assert hdr == hdr2 and len(start) == len(hdr) - 8
+    np.savez_compressed(os.path.join(HERE, "example_collapse_hotcore.npz"), species=np.array(hdr[8:], dtype="U16"),
+                        phase1_times=r1[:, 0], phase1_physics=r1[:, :8], phase1_final=r1[-1, 8:],
+                        start_abundances=start, phase2_times=r2[:, 0], phase2_physics=r2[:, :8],
+                        phase2_final=r2[-1, 8:])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=24)
     ap.add_argument("--hot", type=int, default=4)
     ap.add_argument("--procs", type=int, default=8)
+    ap.add_argument("--examples-only", action="store_true", help="only convert the reference's example outputs")
     a = ap.parse_args()
     # 1. the reference's own example output at t = 1e6 yr
     lines = open(REF_EXAMPLE).read().splitlines()
@@ -61,6 +88,9 @@ def main():
     k = int(np.where(np.isclose(rows[:, 0], 1e6))[0][0])
     np.savez_compressed(os.path.join(HERE, "static_full_t1e6.npz"), species=np.array(hdr[8:], dtype="U16"),
                         abundances=rows[k, 8:], physics=rows[k, :8], times=rows[:k + 1, 0])
+    example_models()
+    if a.examples_only:
+        return
     # 2. reference models
     for name, params in [("ref_cloud_point", workloads.static_cloud_point()),
                          ("ref_cloud_grid", workloads.static_cloud_grid(a.grid)),

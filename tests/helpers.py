@@ -72,3 +72,20 @@ def term_magnitude(tables, rate, y):
 def element_totals(tables, y):
     """Total abundance of each element (rows of elem_counts) over the species."""
     return tables.elem_counts[:tables.nspec].T.astype(np.float64) @ y[:tables.nspec]
+
+
+def conserved_elements(tables):
+    """Elements every reaction conserves.  Four freeze-outs of the default network
+    hydrogenate on the grain (e.g. C2N -> #CH3CNH), so H is not an invariant of the
+    reference equations; electrons follow charge, which ionic freeze-out breaks."""
+    from uclchem_amd.makerates_hip import ELEMENTS
+    a = tables.arrays
+    ok = np.ones(len(ELEMENTS), dtype=bool)
+    for k in range(tables.nreac):
+        bal = np.zeros(len(ELEMENTS))
+        for i, c in zip(a["aff_sp"][a["aff_ptr"][k]:a["aff_ptr"][k + 1]], a["aff_cf"][a["aff_ptr"][k]:a["aff_ptr"][k + 1]]):
+            if i < tables.nspec:
+                bal += c * tables.elem_counts[i]
+        ok &= bal == 0
+    ok[ELEMENTS.index("E-")] = False
+    return np.where(ok)[0]

@@ -1,0 +1,82 @@
+"""GPU tests of the batched model API (uclchem_amd/model.py) in the reference's layouts.
+
+* cloud_grid(return_array=True) on the reference's example model reproduces
+  examples/example-output/static-full.dat (times column, physics row and abundances at
+  1 Myr), in the (timepoints+1, points, ...) arrays of io.f90:74-82;
+* the out_species form is [success_flag] + abundances, as model.py's _format_output;
+* hot_core_grid runs the hot-core schedule with the reference's temp_indx check;
+* get_rates_and_odes: the rate coefficients equal the oracle's at the given abundances and
+  F after the first update equals the oracle's F at the same updated state.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import term_magnitude
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_cloud_grid_arrays_match_example_output():
+    from uclchem_amd import model
+    g = np.load(os.path.join(GOLD, "static_full_t1e6.npz"))
+    cfg1 = {"initialDens": 1e4, "initialTemp": 10.0, "finalTime": 1e6}
+    other = {"initialDens": 1e5, "initialTemp": 20.0, "finalTime": 1e5}
+    phys, chem, rates, start, flags = model.cloud_grid([cfg1, other], return_array=True)
+    assert rates is None and list(flags) == [0, 0]
+    n = len(g["times"])
+    assert phys.shape[1:] == (2, 8) and chem.shape[1:] == (2, 335) and phys.shape[0] >= n
+    assert np.allclose(phys[:n, 0, 0], g["times"], rtol=1e-12)
+    assert np.allclose(phys[n - 1, 0, 1:6], g["physics"][1:6], rtol=1e-4)
+    m = g["abundances"] > 1e-14
+    rel = np.abs(chem[n - 1, 0, m] - g["abundances"][m]) / g["abundances"][m]
+    assert rel.max() < 1e-3
+    assert np.array_equal(start[0], chem[n - 1, 0])
+    assert phys[0, 1, 1] == 1e5
+
+
+def test_out_species_format():
+    from uclchem_amd import model
+    res = model.cloud_grid([{"initialDens": 1e4, "finalTime": 1e4}], out_species=["CO", "H2O", "#H2O"])
+    assert len(res) == 1 and len(res[0]) == 4 and res[0][0] == 0
+    phys, chem, _, _, _ = model.cloud_grid([{"initialDens": 1e4, "finalTime": 1e4}], return_array=True)
+    from uclchem_amd import makerates_hip as mh
+    names = mh.load_network(mh.network_path("default")).species_names
+    assert res[0][1] == chem[-1, 0, names.index("CO")]
+
+
+def test_hot_core_grid_runs():
+    from uclchem_amd import model
+    with pytest.raises(ValueError):
+        model.hot_core_grid(7, 300.0, [{}])
+    phys, chem, _, _, flags = model.hot_core_grid(3, 300.0, [{"initialDens": 1e6, "finalTime": 2e4}],
+                                                  return_array=True)
+    assert flags[0] == 0
+    assert phys[-1, 0, 2] > 10.0          # the gas warmed up along the hot-core schedule
+
+
+def test_get_rates_and_odes_match_oracle(tables, oracle):
+    from oracle import make_params as omake
+    from uclchem_amd import model
+    rng = np.random.default_rng(9)
+    dicts, ab = [], []
+    for _ in range(3):
+        d = {"initialDens": float(10 ** rng.uniform(3, 6)), "initialTemp": float(rng.uniform(10, 60))}
+        dicts.append(d)
+        y = 10 ** rng.uniform(-20, -4, tables.nspec)
+        y[tables.nh2] = 0.4
+        ab.append(y)
+    rate, odes = model.get_rates_and_odes(dicts, np.array(ab))
+    for q, d in enumerate(dicts):
+        pt, _ = oracle.init_point(omake(dens=d["initialDens"], temp=d["initialTemp"], finaltime=1e-7))
+        y0 = np.append(ab[q], d["initialDens"])
+        ro, y0r, _ = oracle.rates(pt, y0)
+        rel = np.abs(rate[q] - ro) / np.maximum(np.abs(ro), 1e-300)
+        rel[(rate[q] == 0) & (ro == 0)] = 0
+        assert rel.max() <= 1e-12
+        # F at the updated state: same rates, the GPU solve's y after the 1e-7-yr interval
+        out = model.network().solve(np.stack([model.make_params(dict(d), finaltime=1e-7)]), y_init=y0[None])
+        fo, r2 = oracle.rhs(pt, ro, out["y"][0])
+        assert np.all(np.abs(odes[q] - fo) <= 1e-12 * term_magnitude(tables, r2, out["y"][0]))

@@ -1,0 +1,82 @@
+"""GPU tests of the serving loop (request queues, continuous batching over three networks).
+
+Each result is the same alone and in the mix: a request answered by a queue while other
+requests of three networks arrive (Poisson stream) and run beside it is bit-identical to
+the same point solved alone by ucg_solve_batch (the per-point computation never depends
+on which workgroup runs it or on what else is resident).  Error paths are forced from the
+host (a closed queue, a full queue, an idle limit the host lets expire)."""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nets():
+    from uclchem_amd import gpu
+    from uclchem_amd import makerates_hip as mh
+    from uclchem_amd import synthetic as sy
+    out = {"default": gpu.Network(mh.load_network(mh.network_path("default")))}
+    for name in ("synthetic_150_1500", "synthetic_500_6000"):
+        out[name] = gpu.Network(sy.load_or_build(name))
+    return out
+
+
+def test_results_same_alone_and_in_mix(nets):
+    from uclchem_amd import serving
+    reqs = serving.poisson_requests(18, rate_per_s=40.0, networks=list(nets), seed=3, t_end_range=(1e3, 3e4))
+    slots = serving.plan_slots(nets, n_cu=64, hbm_budget_bytes=8e9)
+    srv = serving.Server(nets, slots, capacity=64)
+    try:
+        print(f"[serving] slots {slots}", flush=True)
+        done = serving.run_stream(srv, reqs, timeout_s=300.0,
+                                  progress=lambda d, s, t: print(f"[serving] {d}/{s} done at {t:.0f}s", flush=True))
+    finally:
+        srv.close()
+    assert len(done) == len(reqs)
+    by_arrival = {round(t.t_arrival - min(x.t_arrival for x in done), 9): t for t in done}
+    assert len(by_arrival) == len(reqs)
+    for (arr, name, p), t in zip(reqs, sorted(done, key=lambda x: x.t_arrival)):
+        assert t.network == name
+        alone = nets[name].solve(p[None])
+        assert t.status == 0 and alone["status"][0] == 0
+        assert np.array_equal(t.y, alone["y"][0]), name
+        assert np.array_equal(t.stats, alone["stats"][0]), name
+        assert t.t_done >= t.t_arrival
+
+
+def test_queue_errors_are_reported(nets):
+    from uclchem_amd import serving
+    from uclchem_amd.gpu import GpuError
+    from uclchem_amd.params import make_params
+    net = nets["synthetic_150_1500"]
+    p = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e3)
+    q = serving.RequestQueue(net, capacity=2, n_slots=2)
+    try:
+        r0 = q.submit(p)
+        r1 = q.submit(p)
+        with pytest.raises(GpuError):
+            q.submit(p)                       # capacity reached
+        while not (q.done(r0) and q.done(r1)):
+            time.sleep(0.01)
+        y0, s0, _ = q.result(r0)
+        y1, s1, _ = q.result(r1)
+        assert s0 == 0 and np.array_equal(y0, y1)
+        with pytest.raises(GpuError):
+            q.submit(make_params(initialdens=-1.0))   # invalid parameters never reach the device
+    finally:
+        q.close()
+    # a queue whose workgroups hit their idle limit reports requests it can no longer run
+    q = serving.RequestQueue(net, capacity=4, n_slots=1, idle_timeout_s=0.2)
+    try:
+        time.sleep(1.5)
+        r = q.submit(p)
+        with pytest.raises(GpuError):
+            for _ in range(200):
+                if q.done(r):
+                    break
+                time.sleep(0.01)
+    finally:
+        q.close()

@@ -202,3 +202,21 @@ def test_element_totals_helper(tables):
     tot = element_totals(tables, y)
     from uclchem_amd.makerates_hip import ELEMENTS
     assert tot[ELEMENTS.index("H")] == 2 and tot[ELEMENTS.index("O")] == 1
+
+
+def test_reference_reproduces_collapse_example(reflib):
+    """oracle/_ref runs the reference's phase-1 test model (scripts/run_uclchem_tests.py:28-40:
+    free-fall from 1e2 to 1e5 cm^-3, stop at the final density) and lands on
+    examples/example-output/phase1-full.dat: the same 90 output times and every abundance
+    above 1e-14 within 1% (the shipped file comes from another compiler build; the largest
+    difference is 0.54%)."""
+    from oracle import make_params
+    g = np.load(os.path.join(GOLD, "example_collapse_hotcore.npz"))
+    p = make_params(dens=1e2, temp=10.0, freefall=1, endatfinaldensity=1, finaldens=1e5, finaltime=5e6)
+    rc, rows, stats = reflib.model(p, maxrec=400)
+    assert rc == 0 and len(rows) == len(g["phase1_times"])
+    assert np.allclose(rows[:, 0], g["phase1_times"], rtol=1e-3)
+    assert abs(rows[-1, 1] - 1e5) < 1e-3 * 1e5
+    m = g["phase1_final"] > 1e-14
+    rel = np.abs(rows[-1, 4:][m] - g["phase1_final"][m]) / g["phase1_final"][m]
+    assert rel.max() < 0.01

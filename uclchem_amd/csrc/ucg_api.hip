@@ -44,6 +44,40 @@ struct NetHandle {
 };
 std::vector<NetHandle> g_nets;
 
+// A request queue: one persistent solve kernel bound to a network that takes requests the
+// host posts while it runs.  Requests, the control words and the results live in pinned,
+// host-coherent memory the kernel reads and writes directly; the workspace is the queue's
+// own, so queues and batch solves on one network never share slots.
+struct Queue {
+    bool used = false;
+    bool drained = false;         // closed and its kernel finished; resources freed when no queue runs
+    int handle = -1;
+    int capacity = 0, n_slots = 0, lu_bits = 64, next = 0;
+    hipStream_t stream = nullptr;
+    double *h_params = nullptr;   // [capacity][UCG_NPARAM]
+    int *h_ctrl = nullptr;        // [0] requests posted, [1] closed
+    int *h_done = nullptr;        // [capacity]
+    double *h_y = nullptr;        // [capacity][neq]
+    int *h_status = nullptr;      // [capacity]
+    int *h_stats = nullptr;       // [capacity][8]
+    void *ws = nullptr;
+    int *counter = nullptr;
+    int *cancel = nullptr;
+    size_t ws_bytes = 0;
+};
+std::vector<Queue> g_queues;
+
+void queue_release(Queue &q)
+{
+    for (void *p : {(void *)q.h_params, (void *)q.h_ctrl, (void *)q.h_done, (void *)q.h_y, (void *)q.h_status,
+                    (void *)q.h_stats})
+        if (p) (void)hipHostFree(p);
+    for (void *p : {q.ws, (void *)q.counter, (void *)q.cancel})
+        if (p) (void)hipFree(p);
+    if (q.stream) (void)hipStreamDestroy(q.stream);
+    q = Queue();
+}
+
 // natural cubic spline second derivatives on the host (photoreactions.f90:340-418)
 void spline_host(const double *x, const double *y, int n, double *y2)
 {
@@ -179,6 +213,40 @@ int check_params(int n, const double *params)
     return 0;
 }
 
+// Bytes of one workgroup slot's workspace: the Newton matrix (FP64 or FP32), the saved
+// J-form values of the sparse plan, and the rate coefficients, each 256-byte aligned.
+struct SlotLayout {
+    size_t mat, jac, stride;
+};
+
+SlotLayout slot_layout(const NetHandle &h, int lu_bits)
+{
+    const ucg::DevNet &N = h.dev;
+    const size_t elem = lu_bits == 64 ? sizeof(double) : sizeof(float);
+    const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
+    SlotLayout s;
+    s.mat = (((sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem) + 255) & ~(size_t)255;
+    s.jac = sparse ? (((size_t)N.nw_nval * sizeof(double) + 255) & ~(size_t)255) : 0;
+    s.stride = s.mat + s.jac + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
+    return s;
+}
+
+int launch_kernel(const ucg::DevNet &N, const ucg::SolveArgs &A, int n_slots, int lu_bits, hipStream_t stream)
+{
+    const size_t lds = lds_bytes(N);
+    if (lu_bits == 64) {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::solve_points_kernel<double>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+    } else {
+        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(ucg::solve_points_kernel<float>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int launch_solve(NetHandle &h, int n, const double *params, const double *y_init, double *y_out, int max_rec,
                  double *rec, int *n_rec, int *status, int *stats, double *t_start, double *t_done, int lu_bits,
                  int n_slots, hipStream_t stream, double *timers = nullptr)
@@ -188,19 +256,16 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     if (n <= 0) return 0;
     if (n_slots <= 0) n_slots = 2 * g_num_cu;
     if (n_slots > n) n_slots = n;
-    const size_t elem = lu_bits == 64 ? sizeof(double) : sizeof(float);
     const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
-    size_t mat = (sparse ? (size_t)N.nw_nval : (size_t)N.ld * N.neq) * elem;
-    mat = (mat + 255) & ~(size_t)255;
-    const size_t jac = sparse ? (((size_t)N.nw_nval * sizeof(double) + 255) & ~(size_t)255) : 0;
-    const size_t stride = mat + jac + (((size_t)N.nreac * sizeof(double) + 255) & ~(size_t)255);
+    const SlotLayout sl = slot_layout(h, lu_bits);
+    const size_t mat = sl.mat, jac = sl.jac, stride = sl.stride;
     // sized for the default slot count at least, so later calls with more points do not
     // reallocate between launches
     int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
     if (rc) return rc;
     hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, h.counter, h.cancel,
                        status, n);
-    ucg::SolveArgs A;
+    ucg::SolveArgs A = {};
     A.n_points = n;
     A.params = params;
     A.y_init = y_init;
@@ -221,18 +286,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.ws_jac_bytes = jac;
     A.counter = h.counter;
     A.cancel = h.cancel;
-    const size_t lds = lds_bytes(N);
-    if (lu_bits == 64) {
-        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::solve_points_kernel<double>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
-    } else {
-        HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<float>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::solve_points_kernel<float>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
-    }
-    HIPCHK(hipGetLastError());
-    return 0;
+    return launch_kernel(N, A, n_slots, lu_bits, stream);
 }
 
 int piece_common(NetHandle &h, int n, const double *params, const double *phys, ucg::PieceArgs &A,
@@ -567,6 +621,7 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         const int *z = d->nw_sizes;
         N.nw_N = z[0]; N.nw_ns = z[1]; N.nw_T = z[2]; N.nw_nsp = z[3]; N.nw_nlev = z[4]; N.nw_nph = z[7];
         N.nw_nval = z[3] + z[2] * z[2];
+        N.nw_stage = 1;   // cleared below when the staged layout exceeds the device's LDS
         const size_t NN = z[0], ns = z[1], nsp = z[3], nlev = z[4], ntg = z[5], npair = z[6], nph = z[7];
         UP(nw_perm, NN); UP(nw_inv, NN); UP(nw_sp_ptr, NN + 1); UP(nw_sp_col, nsp); UP(nw_sp_dpos, ns);
         UP(nw_lev_ptr, nlev + 1); UP(nw_lev_rows, ns); UP(nw_ph_ptr, nph + 1); UP(nw_tg_slot, ntg);
@@ -628,9 +683,10 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     HIPCHK(hipMalloc(&h.cancel, sizeof(int)));
     h.allocs.push_back(h.cancel);
     HIPCHK(hipMemset(h.cancel, 0, sizeof(int)));
-    size_t lds = lds_bytes(N);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, g_device));
+    if (lds_bytes(N) > (size_t)prop.sharedMemPerBlock) N.nw_stage = 0;   // factors stay in the workspace
+    size_t lds = lds_bytes(N);
     if (lds > (size_t)prop.sharedMemPerBlock)
         return fail(UCG_E_ARG, "network needs " + std::to_string(lds) + " B of LDS per workgroup, device allows " +
                                    std::to_string(prop.sharedMemPerBlock));
@@ -643,6 +699,8 @@ int ucg_network_free(int handle)
     std::lock_guard<std::mutex> lk(g_mu);
     NetHandle *h = get(handle);
     if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    for (const Queue &q : g_queues)
+        if (q.used && !q.drained && q.handle == handle) return fail(UCG_E_ARG, "close the network's request queues first");
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->ws) (void)hipFree(h->ws);
     if (h->cancel_stream) (void)hipStreamDestroy(h->cancel_stream);
@@ -877,6 +935,183 @@ int ucg_newton_solve(int handle, int n, const double *params, const double *phys
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(x, dx.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------- request queues
+
+int ucg_queue_slot_bytes(int handle, int lu_bits, size_t *bytes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
+    if (!bytes) return fail(UCG_E_ARG, "null output");
+    *bytes = slot_layout(*h, lu_bits).stride;
+    return 0;
+}
+
+int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double idle_timeout_s, int *queue)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!queue) return fail(UCG_E_ARG, "null output");
+    if (capacity <= 0 || n_slots <= 0 || n_slots > g_num_cu)
+        return fail(UCG_E_ARG, "capacity must be positive and n_slots in [1, number of CUs]");
+    if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
+    if (!(idle_timeout_s > 0.0) || idle_timeout_s > 3600.0) return fail(UCG_E_ARG, "idle timeout must be in (0, 3600] s");
+    int id = -1;
+    for (int i = 0; i < (int)g_queues.size(); i++)
+        if (!g_queues[i].used) {   // drained queues keep their slot until released
+            id = i;
+            break;
+        }
+    if (id < 0) {
+        g_queues.emplace_back();
+        id = (int)g_queues.size() - 1;
+    }
+    Queue &q = g_queues[id];
+    q = Queue();
+    q.used = true;
+    q.handle = handle;
+    q.capacity = capacity;
+    q.n_slots = n_slots;
+    q.lu_bits = lu_bits;
+    const ucg::DevNet &N = h->dev;
+    const SlotLayout sl = slot_layout(*h, lu_bits);
+    const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+    hipError_t e = hipSuccess;
+    auto hm = [&](void **p, size_t b) {
+        if (e == hipSuccess) e = hipHostMalloc(p, b, fl);
+        if (e == hipSuccess) memset(*p, 0, b);
+    };
+    hm((void **)&q.h_params, (size_t)capacity * UCG_NPARAM * sizeof(double));
+    hm((void **)&q.h_ctrl, 2 * sizeof(int));
+    hm((void **)&q.h_done, (size_t)capacity * sizeof(int));
+    hm((void **)&q.h_y, (size_t)capacity * N.neq * sizeof(double));
+    hm((void **)&q.h_status, (size_t)capacity * sizeof(int));
+    hm((void **)&q.h_stats, (size_t)capacity * 8 * sizeof(int));
+    q.ws_bytes = sl.stride * (size_t)n_slots;
+    if (e == hipSuccess) e = hipMalloc(&q.ws, q.ws_bytes);
+    if (e == hipSuccess) e = hipMalloc(&q.counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&q.cancel, sizeof(int));
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking);
+    // stream-ordered clears: other queues' kernels may be running, and nothing here may wait
+    // for them (a device-wide synchronisation would wait for kernels that only end on close)
+    if (e == hipSuccess) e = hipMemsetAsync(q.counter, 0, sizeof(int), q.stream);
+    if (e == hipSuccess) e = hipMemsetAsync(q.cancel, 0, sizeof(int), q.stream);
+    ucg::SolveArgs A = {};
+    void *d_params = nullptr, *d_ctrl = nullptr, *d_done = nullptr, *d_y = nullptr, *d_status = nullptr,
+         *d_stats = nullptr;
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_params, q.h_params, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_ctrl, q.h_ctrl, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_done, q.h_done, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_y, q.h_y, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_status, q.h_status, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_stats, q.h_stats, 0);
+    if (e != hipSuccess) {
+        std::string msg = std::string("queue set-up: ") + hipGetErrorString(e);
+        queue_release(q);
+        return fail(UCG_E_HIP, msg);
+    }
+    A.n_points = capacity;
+    A.params = (const double *)d_params;
+    A.y_out = (double *)d_y;
+    A.status = (int *)d_status;
+    A.stats = (int *)d_stats;
+    A.sparse = h->linear_solver == UCG_LINSOL_SPARSE ? 1 : 0;
+    A.ws = q.ws;
+    A.ws_stride = sl.stride;
+    A.ws_mat_bytes = sl.mat;
+    A.ws_jac_bytes = sl.jac;
+    A.counter = q.counter;
+    A.cancel = q.cancel;
+    A.q_avail = (const int *)d_ctrl;
+    A.q_closed = (const int *)d_ctrl + 1;
+    A.q_done = (int *)d_done;
+    A.q_idle_ticks = (long long)(idle_timeout_s * 1.0e8);
+    int rc = launch_kernel(N, A, n_slots, lu_bits, q.stream);
+    if (rc) {
+        queue_release(q);
+        return rc;
+    }
+    *queue = id;
+    return 0;
+}
+
+static Queue *get_queue(int queue)
+{
+    if (queue < 0 || queue >= (int)g_queues.size() || !g_queues[queue].used || g_queues[queue].drained) return nullptr;
+    return &g_queues[queue];
+}
+
+int ucg_queue_submit(int queue, const double *params, int *request)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "bad queue");
+    if (!params || !request) return fail(UCG_E_ARG, "null buffer");
+    int rc = check_params(1, params);
+    if (rc) return rc;
+    if (q->next >= q->capacity) return fail(UCG_E_ARG, "queue capacity reached");
+    if (__atomic_load_n(&q->h_ctrl[1], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "queue is closed");
+    const int i = q->next++;
+    memcpy(q->h_params + (size_t)i * UCG_NPARAM, params, UCG_NPARAM * sizeof(double));
+    __atomic_store_n(&q->h_ctrl[0], i + 1, __ATOMIC_RELEASE);   // the parameters are visible first
+    *request = i;
+    return 0;
+}
+
+int ucg_queue_test(int queue, int request, int *done)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "bad queue");
+    if (!done || request < 0 || request >= q->next) return fail(UCG_E_ARG, "bad request");
+    *done = __atomic_load_n(&q->h_done[request], __ATOMIC_ACQUIRE);
+    if (!*done && hipStreamQuery(q->stream) == hipSuccess)
+        return fail(UCG_E_ARG, "the queue's kernel has exited (idle limit or cancel) before this request ran");
+    return 0;
+}
+
+int ucg_queue_result(int queue, int request, double *y, int *status, int *stats)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "bad queue");
+    if (request < 0 || request >= q->next) return fail(UCG_E_ARG, "bad request");
+    if (!__atomic_load_n(&q->h_done[request], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "request not finished");
+    const int neq = g_nets[q->handle].dev.neq;
+    if (y) memcpy(y, q->h_y + (size_t)request * neq, (size_t)neq * sizeof(double));
+    if (status) *status = q->h_status[request];
+    if (stats) memcpy(stats, q->h_stats + (size_t)request * 8, 8 * sizeof(int));
+    return 0;
+}
+
+int ucg_queue_close(int queue)
+{
+    hipStream_t st;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        Queue *q = get_queue(queue);
+        if (!q) return fail(UCG_E_HANDLE, "bad queue");
+        __atomic_store_n(&q->h_ctrl[1], 1, __ATOMIC_RELEASE);
+        st = q->stream;
+    }
+    hipError_t e = hipStreamSynchronize(st);   // posted requests finish, idle workgroups leave
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    q->drained = true;
+    // Freeing device or pinned memory may wait for every kernel on the device, including
+    // other queues' kernels that only end when those queues close: release drained queues
+    // once no queue is running.
+    bool running = false;
+    for (const Queue &o : g_queues) running |= o.used && !o.drained;
+    if (!running)
+        for (Queue &o : g_queues)
+            if (o.used) queue_release(o);
+    if (e != hipSuccess) return fail(UCG_E_HIP, std::string("queue drain: ") + hipGetErrorString(e));
     return 0;
 }
 

@@ -76,6 +76,7 @@ struct DevNet {
         r_c_hv;
     // static-pattern sparse Newton plan (augmented order of size nw_N = neq + 2)
     int32_t nw_N, nw_ns, nw_T, nw_nsp, nw_nlev, nw_nph, nw_nval;
+    int32_t nw_stage;            // 1: factors and solve program are staged in LDS; 0: they stay in the workspace
     const int32_t *nw_perm, *nw_inv, *nw_sp_ptr, *nw_sp_col, *nw_sp_dpos, *nw_lev_ptr, *nw_lev_rows, *nw_ph_ptr;
     const int32_t *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
     const int32_t *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
@@ -86,17 +87,21 @@ struct DevNet {
 };
 
 // LDS bytes the sparse solve stages: factors (lut bytes each; FP32 all of them, FP64 the
-// sparse part), column indices, row records, permutation, work vector (16-byte aligned parts)
+// sparse part), column indices, row records, permutation, work vector and the dense-tail
+// product's scratch (16-byte aligned parts)
 __host__ __device__ inline int solve_stage_bytes(const DevNet &N, int lut)
 {
     const int nst = lut == 4 ? N.nw_nval : N.nw_nsp;
     auto al = [](int x) { return (x + 15) & ~15; };
-    return al(nst * lut) + al(N.nw_nsp * 4) + N.nw_N * 16 + al((N.nw_N + N.nw_nlev + 1) * 4) + N.nw_N * 8;
+    return al(nst * lut) + al(N.nw_nsp * 4) + N.nw_N * 16 + al((N.nw_N + N.nw_nlev + 1) * 4) + N.nw_N * 8 +
+           N.nw_T * 8;
 }
 
 // length (in doubles) of the LDS region shared by the reaction fluxes (RHS), the term
 // partials plus two NEQ-long scratch vectors (Jacobian assembly), and the staged factors
-// (all of them in FP32, the dense tail in FP64) plus the permuted work vector
+// (all of them in FP32, the dense tail in FP64) plus the permuted work vector.  Plans too
+// large to stage (strongly coupled networks with a wide dense tail) keep their factors in
+// the workspace and need only the work vector here.
 __host__ __device__ inline int flux_region_len(const DevNet &N)
 {
     const int npad = (N.neq + 7) & ~7, Npad = (N.nw_N + 7) & ~7;
@@ -104,6 +109,7 @@ __host__ __device__ inline int flux_region_len(const DevNet &N)
     const int g = N.gt_off + N.ngt;                  // fluxes + chunk partial sums
     const int b0 = N.nreac > a ? N.nreac : a;
     const int b = b0 > g ? b0 : g;
+    if (!N.nw_stage) return b > Npad ? b : Npad;               // unstaged plan: the work vector only
     const int t64 = N.nw_T * N.nw_T + Npad;                    // FP64 factorisation: staged dense tail
     const int s32 = (solve_stage_bytes(N, 4) + 7) / 8, s64 = (solve_stage_bytes(N, 8) + 7) / 8;
     const int f = s32 > s64 ? s32 : s64;

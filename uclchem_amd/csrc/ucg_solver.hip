@@ -967,6 +967,173 @@ UCG_BIG __device__ int block_lu_tiled(MP M, int n, int ld, LDS double *red, LDS 
     return 0;
 }
 
+// In-place inverse of the dense n x n block M (column-major, leading dimension ld) by
+// blocked Gauss-Jordan elimination without pivoting, four pivots per sweep: the 4x4 pivot
+// block is inverted in thread 0's registers, its row panel is multiplied by that inverse
+// (one column per thread), every entry outside the pivot rows and columns takes the
+// rank-4 update in 4x4 tiles, then the pivot column panel becomes -A_IK inv(A_KK).  The
+// pivots are those of the unpivoted LU and are checked the same way; it is used for the
+// trailing block of the sparse plan so that each Newton solve applies the block as one
+// matrix-vector product instead of two dependent triangular sweeps.  stage: >= 16
+// doubles of LDS.  Returns nonzero (uniformly) on a zero or non-finite pivot.
+template <typename LUT, typename MP>
+UCG_BIG __device__ int block_gj_inverse(MP M, int n, int ld, LDS double *red, LDS double *stage)
+{
+    constexpr int B = 4;
+    LDS double *P = stage;   // P[l*4+m] = inv(A_KK)(l, m)
+    const int nt = (n + 3) >> 2;
+    for (int kb = 0; kb < n; kb += B) {
+        const int nb = min(B, n - kb);
+        __syncthreads();
+        if (tid() == 0) {
+            double a[B][B];
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++)
+                    a[r][c] = (r < nb && c < nb) ? (double)M[(size_t)(kb + c) * ld + kb + r] : (r == c ? 1.0 : 0.0);
+            double bad = 0.0;
+#pragma unroll
+            for (int k = 0; k < B; k++) {
+                const double piv = a[k][k];
+                if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = 1.0;
+                const double q = 1.0 / piv;
+#pragma unroll
+                for (int c = 0; c < B; c++)
+                    if (c != k) a[k][c] *= q;
+                a[k][k] = q;
+#pragma unroll
+                for (int r = 0; r < B; r++) {
+                    if (r == k) continue;
+                    const double f = a[r][k];
+#pragma unroll
+                    for (int c = 0; c < B; c++)
+                        if (c != k) a[r][c] -= f * a[k][c];
+                    a[r][k] = -f * q;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < B; r++)
+#pragma unroll
+                for (int c = 0; c < B; c++) {
+                    P[r * B + c] = a[r][c];
+                    if (r < nb && c < nb) M[(size_t)(kb + c) * ld + kb + r] = (LUT)a[r][c];
+                }
+            red[8] = bad;
+        }
+        __syncthreads();
+        if (red[8] != 0.0) return 1;
+        for (int j = tid(); j < n; j += NT) {   // row panel: A_KJ <- inv(A_KK) A_KJ
+            if (j >= kb && j < kb + nb) continue;
+            double v[B];
+#pragma unroll
+            for (int m = 0; m < B; m++) v[m] = m < nb ? (double)M[(size_t)j * ld + kb + m] : 0.0;
+#pragma unroll
+            for (int l = 0; l < B; l++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < B; m++) acc += P[l * B + m] * v[m];
+                if (l < nb) M[(size_t)j * ld + kb + l] = (LUT)acc;
+            }
+        }
+        __syncthreads();
+        const int kt = kb >> 2;
+        for (int tt = tid(); tt < nt * nt; tt += NT) {   // A_IJ -= A_IK A_KJ outside the pivot block
+            const int ti = tt % nt, tj = tt / nt;
+            if (ti == kt || tj == kt) continue;
+            const int i0 = 4 * ti, j0 = 4 * tj;
+            double acc[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    acc[r][c] = (i0 + r < n && j0 + c < n) ? (double)M[(size_t)(j0 + c) * ld + i0 + r] : 0.0;
+#pragma unroll
+            for (int m = 0; m < B; m++) {
+                if (m < nb) {
+                    double a[4], b[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) a[r] = (i0 + r < n) ? (double)M[(size_t)(kb + m) * ld + i0 + r] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) b[c] = (j0 + c < n) ? (double)M[(size_t)(j0 + c) * ld + kb + m] : 0.0;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] -= a[r] * b[c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (i0 + r < n && j0 + c < n) M[(size_t)(j0 + c) * ld + i0 + r] = (LUT)acc[r][c];
+        }
+        __syncthreads();
+        for (int i = tid(); i < n; i += NT) {   // column panel: A_IK <- -A_IK inv(A_KK)
+            if (i >= kb && i < kb + nb) continue;
+            double c[B];
+#pragma unroll
+            for (int l = 0; l < B; l++) c[l] = l < nb ? (double)M[(size_t)(kb + l) * ld + i] : 0.0;
+#pragma unroll
+            for (int m = 0; m < B; m++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < B; l++) acc += c[l] * P[l * B + m];
+                if (m < nb) M[(size_t)(kb + m) * ld + i] = (LUT)(-acc);
+            }
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
+// x <- Ainv x for the n x n inverse Ainv (column-major, leading dimension n) and the LDS
+// vector x: two threads per row (column halves) when 2n fits the workgroup, the second
+// half's partial passed through `scratch` (n doubles of LDS); x is overwritten only after
+// every thread has read it.
+template <typename TP>
+__device__ __forceinline__ void tail_matvec(TP Ainv, int n, LDS double *x, LDS double *scratch)
+{
+    const int t = tid();
+    if (2 * n <= NT) {
+        const int half = (n + 1) >> 1;
+        double acc = 0.0;
+        int r = -1;
+        if (t < 2 * n) {
+            r = t % n;
+            const int h = t / n;
+            const int c1 = h ? n : half;
+            int c = h ? half : 0;
+            double a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            for (; c + 4 <= c1; c += 4) {
+                acc += (double)Ainv[(size_t)c * n + r] * x[c];
+                a1 += (double)Ainv[(size_t)(c + 1) * n + r] * x[c + 1];
+                a2 += (double)Ainv[(size_t)(c + 2) * n + r] * x[c + 2];
+                a3 += (double)Ainv[(size_t)(c + 3) * n + r] * x[c + 3];
+            }
+            for (; c < c1; c++) acc += (double)Ainv[(size_t)c * n + r] * x[c];
+            acc = (acc + a1) + (a2 + a3);
+            if (h) scratch[r] = acc;
+        }
+        __syncthreads();
+        if (t < n) x[r] = acc + scratch[r];
+    } else {
+        for (int r = t; r < n; r += NT) {
+            double a0 = 0.0, a1 = 0.0;
+            int c = 0;
+            for (; c + 2 <= n; c += 2) {
+                a0 += (double)Ainv[(size_t)c * n + r] * x[c];
+                a1 += (double)Ainv[(size_t)(c + 1) * n + r] * x[c + 1];
+            }
+            if (c < n) a0 += (double)Ainv[(size_t)c * n + r] * x[c];
+            scratch[r] = a0 + a1;
+        }
+        __syncthreads();
+        for (int r = t; r < n; r += NT) x[r] = scratch[r];
+    }
+    __syncthreads();
+}
+
 #ifndef UCG_SB
 #define UCG_SB 32
 #endif
@@ -1208,9 +1375,10 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
 }
 
 // Crout elimination of the sparse pivots in plan phases (one thread per target entry,
-// dependencies only on earlier phases), then the blocked dense LU of the trailing block.
-// FP32 factors are staged in LDS (region) for the whole factorisation; FP64 factors stay
-// in the workspace and only the dense tail is staged for its LU.  The factors end in V.
+// dependencies only on earlier phases), then the trailing block (its Schur complement):
+// inverted in place by blocked Gauss-Jordan when the plan is staged, LU-factored when it
+// is not.  FP32 factors are staged in LDS (region) for the whole factorisation; FP64
+// factors stay in the workspace and only the dense tail is staged.  The results end in V.
 // stage: 512 doubles of LDS.
 template <typename LUT, typename WP>
 __device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red)
@@ -1261,14 +1429,20 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     int r = 0;
     const uint64_t c0 = wall_clock64();
     uint64_t c1;
-    if constexpr (sizeof(LUT) == 4) {
+    if (!N.nw_stage) {                    // factors too large to stage: everything in the workspace
+        __syncthreads();
+        crout_phases<LUT>(N, V, red);
+        c1 = wall_clock64();
+        if (red[9] != 0.0) return 1;
+        if (T > 0) r = block_lu_tiled<LUT>(V + nsp, T, T, red, stage);
+    } else if constexpr (sizeof(LUT) == 4) {
         LUT *Vl = (LUT *)region;
         for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
         __syncthreads();
         crout_phases<LUT>(N, Vl, red);
         c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
-        if (T > 0) r = block_lu_tiled<LUT>(Vl + nsp, T, T, red, stage);
+        if (T > 0) r = block_gj_inverse<LUT>(Vl + nsp, T, T, red, stage);
         for (int q = tid(); q < nval; q += NT) V[q] = Vl[q];
     } else {
         __syncthreads();
@@ -1278,7 +1452,7 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
         LUT *tl = (LUT *)region;
         for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
         __syncthreads();
-        if (T > 0) r = block_lu_tiled<LUT>(tl, T, T, red, stage);
+        if (T > 0) r = block_gj_inverse<LUT>(tl, T, T, red, stage);
         for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
     }
     __syncthreads();
@@ -1289,53 +1463,16 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     return r;
 }
 
-// Solve M x = b with the factors of sparse_lu: b (length neq, in LDS) is overwritten by x.
-// The solve program (row records in level order, column indices, permutation) and the
-// sparse factors are staged in LDS (region) so that every dependent access of the
-// level-scheduled substitutions is an LDS access; FP32 factors bring their dense tail
-// too, FP64 tails are read from the workspace by the dense triangular solves.
-template <typename LUT>
-UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, LDS double *b, LDS double *region,
-                                       uint64_t *split = nullptr)
+// Level-scheduled substitutions of the static plan on the permuted work vector w (LDS):
+// forward over the sparse rows by level, the tail rows' L21 part, the dense tail, then
+// backward by level.  V, the column indices and the row records are LDS or workspace
+// pointers (staged or unstaged plan); b (length neq, LDS) is overwritten by x.
+template <typename LUT, typename VP, typename IP, typename TP>
+__device__ __forceinline__ void solve_program(const DevNet &N, VP V, IP cols, IP recs, IP inv, IP lev, TP tail,
+                                              LDS double *w, LDS double *b, double sb, double ss, uint64_t *split,
+                                              uint64_t c0, uint64_t cs)
 {
-    const uint64_t c0 = wall_clock64();
-    uint64_t c1 = c0, c2 = c0;
-    const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns, nsp = N.nw_nsp, T = N.nw_T;
-    constexpr bool STAGE_TAIL = sizeof(LUT) == 4;
-    const int nst = STAGE_TAIL ? N.nw_nval : nsp;                       // staged factor entries
-    // layout in bytes, 16-byte aligned parts (solve_stage_bytes sizes the same)
-    LDS char *base = (LDS char *)region;
-    int off = 0;
-    LDS LUT *Vs = (LDS LUT *)base;
-    off = (nst * (int)sizeof(LUT) + 15) & ~15;
-    LDS int *cols = (LDS int *)(base + off);
-    off += (nsp * 4 + 15) & ~15;
-    LDS int *recs = (LDS int *)(base + off);   // 4 ints per row: {row, first, diagonal, end}
-    off += NN * 16;
-    LDS int *inv = (LDS int *)(base + off);
-    off += ((NN + N.nw_nlev + 1) * 4 + 15) & ~15;   // permutation, then the level pointers
-    LDS double *w = (LDS double *)(base + off);
-    for (int q = tid(); q < nst; q += NT) Vs[q] = Vg[q];
-    for (int q = tid(); q < nsp; q += NT) cols[q] = N.nw_sp_col[q];
-    {
-        const int4 *gr = reinterpret_cast<const int4 *>(N.sv_rec);
-        for (int q = tid(); q < NN; q += NT) {
-            const int4 g = gr[q];
-            recs[4 * q] = g.x;
-            recs[4 * q + 1] = g.y;
-            recs[4 * q + 2] = g.z;
-            recs[4 * q + 3] = g.w;
-        }
-    }
-    for (int q = tid(); q < NN; q += NT) inv[q] = N.nw_inv[q];
-    LDS int *lev = inv + NN;   // level pointers (nlev + 1), after the permutation
-    for (int q = tid(); q <= N.nw_nlev; q += NT) lev[q] = N.nw_lev_ptr[q];
-    const LDS LUT *V = Vs;
-    double sb = 0.0, ss = 0.0;
-    for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
-    for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
-    block_sum2(sb, ss, L.red);   // its barriers also publish the staged program
-    const uint64_t cs = wall_clock64();
+    const int neq = N.neq, NN = N.nw_N, ns = N.nw_ns, T = N.nw_T;
     for (int i = tid(); i < NN; i += NT) {
         double v = 0.0;
         if (i < neq) {
@@ -1378,12 +1515,12 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, 
         w[r] = (a0 + a1) + (a2 + a3);
     }
     __syncthreads();
-    c1 = wall_clock64();
+    const uint64_t c1 = wall_clock64();
     if (T > 0) {
-        if constexpr (STAGE_TAIL) block_lu_solve<LUT>(Vs + nsp, T, T, w + ns);
-        else block_lu_solve<LUT>(Vg + nsp, T, T, w + ns);
+        if (N.nw_stage) tail_matvec(tail, T, w + ns, w + NN);   // staged plans hold the block's inverse
+        else block_lu_solve<LUT>(tail, T, T, w + ns);          // unstaged plans hold its LU factors
     }
-    c2 = wall_clock64();
+    const uint64_t c2 = wall_clock64();
     for (int l = N.nw_nlev - 1; l >= 0; l--) {   // backward, sparse rows by level
         const int q1 = lev[l + 1];
         for (int q = lev[l] + tid(); q < q1; q += NT) {
@@ -1409,6 +1546,67 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, 
         split[2] += cs - c0;                             // staging + right-hand-side sums
         split[3] += c1 - cf0;                            // forward levels + tail rows
     }
+}
+
+// Solve M x = b with the factors of sparse_lu: b (length neq, in LDS) is overwritten by x.
+// A staged plan copies the solve program (row records in level order, column indices,
+// permutation) and the sparse factors into LDS (region) so that every dependent access of
+// the level-scheduled substitutions is an LDS access; FP32 factors bring their dense tail
+// too, FP64 tails are read from the workspace by the dense triangular solves.  An unstaged
+// plan reads all of it from the workspace and keeps only the work vector in region.
+template <typename LUT>
+UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, LDS double *b, LDS double *region,
+                                       uint64_t *split = nullptr)
+{
+    const uint64_t c0 = wall_clock64();
+    const int NN = N.nw_N, nsp = N.nw_nsp;
+    double sb = 0.0, ss = 0.0;
+    for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
+    for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
+    if (!N.nw_stage) {
+        block_sum2(sb, ss, L.red);
+        solve_program<LUT>(N, Vg, N.nw_sp_col, N.sv_rec, N.nw_inv, N.nw_lev_ptr, Vg + nsp, region, b, sb, ss, split, c0,
+                           wall_clock64());
+        return;
+    }
+    constexpr bool STAGE_TAIL = sizeof(LUT) == 4;
+    const int nst = STAGE_TAIL ? N.nw_nval : nsp;                       // staged factor entries
+    // layout in bytes, 16-byte aligned parts (solve_stage_bytes sizes the same)
+    LDS char *base = (LDS char *)region;
+    int off = 0;
+    LDS LUT *Vs = (LDS LUT *)base;
+    off = (nst * (int)sizeof(LUT) + 15) & ~15;
+    LDS int *cols = (LDS int *)(base + off);
+    off += (nsp * 4 + 15) & ~15;
+    LDS int *recs = (LDS int *)(base + off);   // 4 ints per row: {row, first, diagonal, end}
+    off += NN * 16;
+    LDS int *inv = (LDS int *)(base + off);
+    off += ((NN + N.nw_nlev + 1) * 4 + 15) & ~15;   // permutation, then the level pointers
+    LDS double *w = (LDS double *)(base + off);
+    for (int q = tid(); q < nst; q += NT) Vs[q] = Vg[q];
+    for (int q = tid(); q < nsp; q += NT) cols[q] = N.nw_sp_col[q];
+    {
+        const int4 *gr = reinterpret_cast<const int4 *>(N.sv_rec);
+        for (int q = tid(); q < NN; q += NT) {
+            const int4 g = gr[q];
+            recs[4 * q] = g.x;
+            recs[4 * q + 1] = g.y;
+            recs[4 * q + 2] = g.z;
+            recs[4 * q + 3] = g.w;
+        }
+    }
+    for (int q = tid(); q < NN; q += NT) inv[q] = N.nw_inv[q];
+    LDS int *lev = inv + NN;   // level pointers (nlev + 1), after the permutation
+    for (int q = tid(); q <= N.nw_nlev; q += NT) lev[q] = N.nw_lev_ptr[q];
+    block_sum2(sb, ss, L.red);   // its barriers also publish the staged program
+    const uint64_t cs = wall_clock64();
+    const LDS LUT *V = Vs;
+    if constexpr (STAGE_TAIL)
+        solve_program<LUT>(N, V, (const LDS int *)cols, (const LDS int *)recs, (const LDS int *)inv,
+                           (const LDS int *)lev, V + nsp, w, b, sb, ss, split, c0, cs);
+    else
+        solve_program<LUT>(N, V, (const LDS int *)cols, (const LDS int *)recs, (const LDS int *)inv,
+                           (const LDS int *)lev, Vg + nsp, w, b, sb, ss, split, c0, cs);
 }
 
 // ---------------------------------------------------------------- BDF ops policy
@@ -1707,6 +1905,12 @@ struct SolveArgs {
                               // STATUS_CANCELLED)
     int equilibrate;          // factor the error-weight-equilibrated Newton matrix (dense form)
     int sparse;               // 1: static-pattern sparse plan, 0: dense LU
+    // request-queue mode (all null/0 for a batch): n_points is the queue capacity and the
+    // host posts requests while the kernel runs
+    const int *q_avail;       // requests posted so far (host-written, read at system scope)
+    const int *q_closed;      // nonzero once the host admits no more requests
+    int *q_done;              // [n]: set to 1 after request i's outputs are written
+    long long q_idle_ticks;   // a workgroup that waits this long (100 MHz ticks) for a request leaves
 };
 
 __device__ void elemental_start(const DevNet &N, const LDS double *prm, LDS double *y, double density)
@@ -1797,7 +2001,23 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
     for (;;) {
         if (tid() == 0) {
             const bool stop = A.cancel && __hip_atomic_load(A.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
-            s_point = stop ? A.n_points : atomicAdd(A.counter, 1);
+            int p = stop ? A.n_points : atomicAdd(A.counter, 1);
+            if (A.q_avail && p < A.n_points) {   // request queue: wait until request p is posted
+                const uint64_t w0 = wall_clock64();
+                for (;;) {
+                    if (__hip_atomic_load(A.q_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > p) break;
+                    const bool closed = __hip_atomic_load(A.q_closed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+                    if (closed && __hip_atomic_load(A.q_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > p) break;
+                    const bool cancelled =
+                        A.cancel && __hip_atomic_load(A.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+                    if (closed || cancelled || (long long)(wall_clock64() - w0) > A.q_idle_ticks) {
+                        p = A.n_points;   // every path of this loop ends: post, close, cancel or idle limit
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(127);
+                }
+            }
+            s_point = p;
         }
         __syncthreads();
         const int pidx = s_point;
@@ -1950,6 +2170,11 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
             }
         }
         __syncthreads();
+        if (A.q_done) {   // request queue: publish the outputs, then the completion flag
+            __threadfence_system();
+            __syncthreads();
+            if (tid() == 0) __hip_atomic_store(A.q_done + pidx, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 

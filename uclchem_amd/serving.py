@@ -1,0 +1,179 @@
+"""Serving loop: continuous batching of single-point solves over several resident networks.
+
+The reference runs one model per call (``uclchem.model.cloud`` -> ``uclchemwrap.cloud``,
+``src/uclchem/model.py:277-289``, ``src/fortran_src/wrap.f90:15-80``).  Here every network
+of the mix keeps a request queue open on the GPU (``ucg_queue_*`` in
+include/uclchem_gpu.h): a persistent solve kernel with ``n_slots`` resident workgroups
+that takes each posted request as soon as one of its workgroups is free.  Requests that
+arrive while others are running therefore start without waiting for a batch boundary,
+and the work of all networks overlaps on the device.
+
+Admission by HBM budget: each network's slot count is its share of the CUs, cut so that
+the workspaces of all open queues (slots x per-slot bytes, ``ucg_queue_slot_bytes``) stay
+within the budget; requests beyond the free slots wait inside the device queue (posted,
+not yet taken).  Network tables and workspaces stay resident for the life of the server.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .gpu import GpuError, Network, _check, _p, lib
+from .params import NPARAM, make_params
+
+
+class RequestQueue:
+    """ctypes wrapper of one ucg_queue on a loaded network."""
+
+    def __init__(self, net: Network, capacity: int, n_slots: int, lu_bits: int = 64, idle_timeout_s: float = 600.0):
+        self.net = net
+        q = ctypes.c_int(-1)
+        _check(lib().ucg_queue_open(ctypes.c_int(net.handle), ctypes.c_int(capacity), ctypes.c_int(n_slots),
+                                    ctypes.c_int(lu_bits), ctypes.c_double(idle_timeout_s), ctypes.byref(q)),
+               "ucg_queue_open")
+        self.id = q.value
+        self.open = True
+
+    @staticmethod
+    def slot_bytes(net: Network, lu_bits: int = 64) -> int:
+        b = ctypes.c_size_t(0)
+        _check(lib().ucg_queue_slot_bytes(ctypes.c_int(net.handle), ctypes.c_int(lu_bits), ctypes.byref(b)),
+               "ucg_queue_slot_bytes")
+        return int(b.value)
+
+    def submit(self, params: np.ndarray) -> int:
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(NPARAM)
+        r = ctypes.c_int(-1)
+        _check(lib().ucg_queue_submit(ctypes.c_int(self.id), _p(p), ctypes.byref(r)), "ucg_queue_submit")
+        return r.value
+
+    def done(self, request: int) -> bool:
+        d = ctypes.c_int(0)
+        _check(lib().ucg_queue_test(ctypes.c_int(self.id), ctypes.c_int(request), ctypes.byref(d)), "ucg_queue_test")
+        return bool(d.value)
+
+    def result(self, request: int):
+        y = np.zeros(self.net.neq)
+        st = ctypes.c_int(0)
+        stats = np.zeros(8, dtype=np.int32)
+        _check(lib().ucg_queue_result(ctypes.c_int(self.id), ctypes.c_int(request), _p(y), ctypes.byref(st),
+                                      _p(stats)), "ucg_queue_result")
+        return y, st.value, stats
+
+    def close(self) -> None:
+        if self.open:
+            self.open = False
+            _check(lib().ucg_queue_close(ctypes.c_int(self.id)), "ucg_queue_close")
+
+
+@dataclass
+class Ticket:
+    network: str
+    request: int
+    t_arrival: float
+    t_done: float = 0.0
+    y: Optional[np.ndarray] = None
+    status: int = 0
+    stats: Optional[np.ndarray] = None
+
+
+@dataclass
+class Server:
+    """Resident networks with one open request queue each.
+
+    slots: {network: resident workgroups}; chosen by :func:`plan_slots` from the CU count
+    and the HBM budget when not given."""
+    networks: Dict[str, Network]
+    slots: Dict[str, int]
+    capacity: int = 4096
+    lu_bits: int = 64
+    queues: Dict[str, RequestQueue] = field(default_factory=dict)
+    pending: List[Ticket] = field(default_factory=list)
+
+    def __post_init__(self):
+        for name, net in self.networks.items():
+            self.queues[name] = RequestQueue(net, self.capacity, self.slots[name], self.lu_bits)
+
+    def submit(self, network: str, params: np.ndarray, t_arrival: Optional[float] = None) -> Ticket:
+        t = Ticket(network, self.queues[network].submit(params), time.perf_counter() if t_arrival is None else t_arrival)
+        self.pending.append(t)
+        return t
+
+    def poll(self) -> List[Ticket]:
+        """Tickets that finished since the last poll (results fetched)."""
+        out, keep = [], []
+        now = time.perf_counter()
+        for t in self.pending:
+            q = self.queues[t.network]
+            if q.done(t.request):
+                t.t_done = now
+                t.y, t.status, t.stats = q.result(t.request)
+                out.append(t)
+            else:
+                keep.append(t)
+        self.pending = keep
+        return out
+
+    def close(self) -> None:
+        for q in self.queues.values():
+            q.close()
+
+
+def plan_slots(networks: Dict[str, Network], n_cu: int, hbm_budget_bytes: float, lu_bits: int = 64,
+               weights: Optional[Dict[str, float]] = None) -> Dict[str, int]:
+    """Resident workgroups per network: a share of the CUs in proportion to `weights`
+    (default equal), reduced until the workspaces fit the HBM budget.  The total stays
+    within the CU count so that every queue's workgroups are resident together."""
+    names = list(networks)
+    w = np.array([(weights or {}).get(n, 1.0) for n in names], dtype=float)
+    share = np.maximum(1, np.floor((n_cu - 4) * w / w.sum())).astype(int)
+    per = np.array([RequestQueue.slot_bytes(networks[n], lu_bits) for n in names], dtype=float)
+    while (share * per).sum() > hbm_budget_bytes and share.max() > 1:
+        share[np.argmax(share * per)] -= 1
+    if (share * per).sum() > hbm_budget_bytes:
+        raise GpuError("HBM budget too small for one slot per network")
+    return {n: int(s) for n, s in zip(names, share)}
+
+
+def poisson_requests(n: int, rate_per_s: float, networks: List[str], seed: int = 20261202,
+                     t_end_range=(1e3, 1e7)):
+    """Seeded request stream: exponential inter-arrival gaps (rate lambda), a network chosen
+    uniformly from `networks`, t_end log-uniform in t_end_range years, physics of the
+    single static cloud point (n_H=1e4, T=10 K, zeta=1, G0=1, default Av)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gaps = rng.exponential(1.0 / rate_per_s, n)
+    arrive = np.cumsum(gaps) - gaps[0]
+    which = rng.integers(0, len(networks), n)
+    t_end = 10 ** rng.uniform(np.log10(t_end_range[0]), np.log10(t_end_range[1]), n)
+    reqs = []
+    for i in range(n):
+        p = make_params(initialdens=1e4, initialtemp=10.0, zeta=1.0, radfield=1.0, finaltime=float(t_end[i]))
+        reqs.append((float(arrive[i]), networks[int(which[i])], p))
+    return reqs
+
+
+def run_stream(server: Server, requests, poll_interval_s: float = 2e-4, timeout_s: float = 600.0,
+               progress=None) -> List[Ticket]:
+    """Replay `requests` (arrival offset s, network, params) in real time against the
+    server; returns every finished ticket (latency = t_done - t_arrival)."""
+    done: List[Ticket] = []
+    t0 = time.perf_counter()
+    i = 0
+    last = t0
+    while len(done) < len(requests):
+        now = time.perf_counter()
+        while i < len(requests) and now - t0 >= requests[i][0]:
+            server.submit(requests[i][1], requests[i][2], t_arrival=t0 + requests[i][0])
+            i += 1
+        done += server.poll()
+        if progress and now - last > 10.0:
+            progress(len(done), i, now - t0)
+            last = now
+        if now - t0 > timeout_s:
+            raise GpuError(f"stream not finished after {timeout_s} s ({len(done)}/{len(requests)} done)")
+        time.sleep(poll_interval_s)
+    return done

@@ -6,7 +6,8 @@
   log G0 ~ U[0,3], Av ~ U[1,20] mapped through baseAv = Av - rout*pc*n_H/1.6e21 so the
   point's Av is the sampled one (physics-core.f90:50,68-71); all to 1 Myr.
 * ``hot_core_grid``: n points, log n_H ~ U[5,8], hot_core(temp_indx=4, max_temp=300 K).
-* ``collapse_grid``: free-fall collapse from n_H,0 log-uniform in [1e2,1e4] to finalDens=1e6.
+* ``collapse_grid``: free-fall collapse from n_H,0 log-uniform in [1e2,1e6] cm^-3 towards
+  finalDens=1e6 (densdot is zero once capped, physics-core.f90:97), T=10 K, to 1 Myr.
 Random draws use numpy's PCG64 with the seed given (default 20261202).
 """
 from __future__ import annotations
@@ -49,7 +50,7 @@ def hot_core_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.ndarr
 
 def collapse_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed + 3))
-    lognh = rng.uniform(2.0, 4.0, n)
+    lognh = rng.uniform(2.0, 6.0, n)
     out = np.zeros((n, 24))
     for i in range(n):
         out[i] = make_params(initialdens=10.0 ** lognh[i], initialtemp=10.0, freefall=1, finaldens=1e6,

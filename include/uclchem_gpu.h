@@ -52,6 +52,11 @@ extern "C" {
 #define UCG_P_ENDATFINALDENS 17
 #define UCG_P_FREEFALLFAC 18  /* freefallFactor */
 #define UCG_P_MXSTEP 19       /* MXSTEP per integrator call */
+#define UCG_P_MAXSTEPS 20     /* total BDF steps a point may take (0: unbounded, the reference's
+                                 semantics); a point that reaches it stops with UCG_E_TOO_MANY_FAILS,
+                                 the code UCLCHEM reserves for integrations that cannot advance.
+                                 No counterpart in the reference: a work bound for batch and serving
+                                 callers whose grids hold a few pathologically stiff points. */
 
 /* status codes (UCLCHEM's own, src/fortran_src/constants.f90:22-27) */
 #define UCG_OK 0

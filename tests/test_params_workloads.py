@@ -28,7 +28,9 @@ def test_grid_av_mapping():
 
 def test_oracle_and_product_param_layouts_agree():
     import oracle as orc
-    assert orc.P_NAMES[:len(params.NAMES)] == [
+    # the product's extra trailing slot (maxsteps, a work bound) has no oracle counterpart
+    assert params.NAMES[-1] == "maxsteps" and params.DEFAULTS["maxsteps"] == 0.0
+    assert orc.P_NAMES == [
         "dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall", "finaldens",
         "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp", "freezefactor",
         "endatfinaldensity", "freefallfactor", "mxstep"]

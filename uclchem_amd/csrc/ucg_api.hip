@@ -203,7 +203,7 @@ int check_params(int n, const double *params)
     for (int p = 0; p < n; p++) {
         const double *q = params + (size_t)p * UCG_NPARAM;
         if (!(q[UCG_P_DENS] > 0) || !(q[UCG_P_TEMP] > 0) || !(q[UCG_P_RELTOL] > 0) || !(q[UCG_P_ABSMIN] >= 0) ||
-            !(q[UCG_P_MXSTEP] >= 1) || !(q[UCG_P_ROUT] > q[UCG_P_RIN]))
+            !(q[UCG_P_MXSTEP] >= 1) || !(q[UCG_P_ROUT] > q[UCG_P_RIN]) || !(q[UCG_P_MAXSTEPS] >= 0))
             return fail(UCG_E_PARAM, "invalid parameter vector for point " + std::to_string(p));
         int model = (int)q[UCG_P_MODEL];
         if (model != 0 && model != 1) return fail(UCG_E_PARAM, "model must be 0 (cloud) or 1 (hot core)");
@@ -624,8 +624,36 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         N.nw_stage = 1;   // cleared below when the staged layout exceeds the device's LDS
         const size_t NN = z[0], ns = z[1], nsp = z[3], nlev = z[4], ntg = z[5], npair = z[6], nph = z[7];
         UP(nw_perm, NN); UP(nw_inv, NN); UP(nw_sp_ptr, NN + 1); UP(nw_sp_col, nsp); UP(nw_sp_dpos, ns);
-        UP(nw_lev_ptr, nlev + 1); UP(nw_lev_rows, ns); UP(nw_ph_ptr, nph + 1); UP(nw_tg_slot, ntg);
-        UP(nw_tg_ptr, ntg + 1); UP(nw_tg_div, ntg); UP(nw_pr_l, npair); UP(nw_pr_u, npair);
+        UP(nw_lev_ptr, nlev + 1); UP(nw_lev_rows, ns); UP(nw_ph_ptr, nph + 1);
+        // targets of each elimination phase, longest pair list first: a pass of the workgroup
+        // over a phase then takes the long chains together instead of one per pass
+        std::vector<int> tslot(ntg), tptr(ntg + 1, 0), tdiv(ntg), pl(npair), pu(npair);
+        for (size_t p = 0; p < nph; p++) {
+            std::vector<int> ord;
+            for (int t = d->nw_ph_ptr[p]; t < d->nw_ph_ptr[p + 1]; t++) ord.push_back(t);
+#ifndef UCG_SORT_TARGETS
+#define UCG_SORT_TARGETS 1
+#endif
+            if (UCG_SORT_TARGETS)
+                std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+                    return d->nw_tg_ptr[x + 1] - d->nw_tg_ptr[x] > d->nw_tg_ptr[y + 1] - d->nw_tg_ptr[y];
+                });
+            int t2 = d->nw_ph_ptr[p];
+            for (int t : ord) {
+                tslot[t2] = d->nw_tg_slot[t];
+                tdiv[t2] = d->nw_tg_div[t];
+                int q2 = tptr[t2];
+                for (int q = d->nw_tg_ptr[t]; q < d->nw_tg_ptr[t + 1]; q++, q2++) {
+                    pl[q2] = d->nw_pr_l[q];
+                    pu[q2] = d->nw_pr_u[q];
+                }
+                tptr[t2 + 1] = q2;
+                t2++;
+            }
+        }
+        if ((rc = upload(h, tslot.data(), tslot.size(), N.nw_tg_slot))) return rc;
+        if ((rc = upload(h, tptr.data(), tptr.size(), N.nw_tg_ptr))) return rc;
+        if ((rc = upload(h, tdiv.data(), tdiv.size(), N.nw_tg_div))) return rc;
         UP(nw_jnz_slot, (size_t)d->nnz); UP(nw_diag_slot, Q); UP(nw_ph_slot, (size_t)z[8]);
         UP(nw_tr_slot, (size_t)8 * d->nsurfl); UP(nw_s1c, Q); UP(nw_s1r, Q); UP(nw_s2c, Q); UP(nw_s2r, Q);
         UP(nw_gs_ptr, Q + 1); UP(nw_gs_slot, (size_t)z[9]); UP(nw_misc, (size_t)5);
@@ -642,10 +670,34 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         if ((rc = upload(h, sv.data(), sv.size(), N.sv_rec))) return rc;
         std::vector<int> pr(2 * npair);
         for (size_t q = 0; q < npair; q++) {
-            pr[2 * q] = d->nw_pr_l[q];
-            pr[2 * q + 1] = d->nw_pr_u[q];
+            pr[2 * q] = pl[q];
+            pr[2 * q + 1] = pu[q];
         }
         if ((rc = upload(h, pr.data(), pr.size(), N.nw_pr))) return rc;
+        {   // sources of the last phase when it is the pure Schur update of the tail
+            const int *pp = d->nw_ph_ptr;
+            const long t0 = pp[nph - 1], t1 = pp[nph];
+            const long q0 = tptr[t0], q1 = tptr[t1];
+            bool ok = nph >= 2 && q1 > q0;
+            for (long t = t0; t < t1 && ok; t++) ok = tslot[t] >= (int)nsp && tdiv[t] < 0;
+            for (long q = q0; q < q1 && ok; q++) ok = pl[q] < (int)nsp && pu[q] < (int)nsp;
+            N.nlp_src = 0;
+            if (ok) {
+                std::vector<int> local(nsp, -1), src, lpp(2 * (q1 - q0));
+                for (long q = q0; q < q1; q++)
+                    for (int h2 = 0; h2 < 2; h2++) {
+                        const int sl = h2 ? pu[q] : pl[q];
+                        if (local[sl] < 0) {
+                            local[sl] = (int)src.size();
+                            src.push_back(sl);
+                        }
+                        lpp[2 * (q - q0) + h2] = local[sl];
+                    }
+                N.nlp_src = (int)src.size();
+                if ((rc = upload(h, src.data(), src.size(), N.lp_src))) return rc;
+                if ((rc = upload(h, lpp.data(), lpp.size(), N.lp_pr))) return rc;
+            }
+        }
         std::vector<signed char> uns((size_t)N.nw_nval, 0);
         for (size_t i = 0; i < Q; i++) {
             if (d->nw_s1c[i] >= 0) uns[d->nw_s1c[i]] = 1;

@@ -31,7 +31,7 @@ enum { OP_NONE = -1, OP_BLR = -2, OP_SWAP = -3, OP_DIVSM = -4, FAC_D = -5 };
 // point-parameter vector layout (shared with include/uclchem_gpu.h)
 enum { P_DENS, P_TEMP, P_ZETA, P_RADFIELD, P_BASEAV, P_ROUT, P_RIN, P_FINALTIME, P_FREEFALL,
        P_FINALDENS, P_RELTOL, P_ABSFAC, P_ABSMIN, P_MODEL, P_TEMPINDX, P_MAXTEMP, P_FREEZEFAC,
-       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_NPARAM = 24 };
+       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_MAXSTEPS, P_NPARAM = 24 };
 
 // physical constants (constants.f90, surfacereactions.f90:33-51)
 constexpr double K_BOLTZ = 1.38065040e-16;
@@ -83,6 +83,11 @@ struct DevNet {
     const int32_t *nw_misc, *nw_nbrow, *nw_nsrow;
     const int32_t *sv_rec;   // solve rows {row, first, diagonal, end}: sparse rows in level order, then tail rows
     const int32_t *nw_pr;    // Crout pairs interleaved {L slot, U slot}
+    // the last elimination phase is the Schur-complement update of the dense tail; its
+    // sources (sparse factor entries) are gathered into LDS beside the staged tail
+    int32_t nlp_src;          // number of distinct source entries (0: the phase runs in place)
+    const int32_t *lp_src;    // [nlp_src] factor slots of the sources
+    const int32_t *lp_pr;     // the phase's pairs as {source index, source index}
     const int8_t *nw_unscaled;   // [nval] 1 where the Newton matrix takes the J-form value without gamma
 };
 
@@ -110,7 +115,7 @@ __host__ __device__ inline int flux_region_len(const DevNet &N)
     const int b0 = N.nreac > a ? N.nreac : a;
     const int b = b0 > g ? b0 : g;
     if (!N.nw_stage) return b > Npad ? b : Npad;               // unstaged plan: the work vector only
-    const int t64 = N.nw_T * N.nw_T + Npad;                    // FP64 factorisation: staged dense tail
+    const int t64 = N.nw_T * N.nw_T + ((N.nlp_src + 7) & ~7) + Npad;   // FP64: staged tail + Schur sources
     const int s32 = (solve_stage_bytes(N, 4) + 7) / 8, s64 = (solve_stage_bytes(N, 8) + 7) / 8;
     const int f = s32 > s64 ? s32 : s64;
     const int c = t64 > f ? t64 : f;

@@ -31,6 +31,9 @@ enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI
 #ifndef UCG_WAVES_PER_EU
 #define UCG_WAVES_PER_EU 1
 #endif
+#ifndef UCG_LDS_SCHUR
+#define UCG_LDS_SCHUR 1     // FP64: the tail's Schur update from sources gathered into LDS
+#endif
 
 struct Lds {
     double *rate;                                   // workspace (global memory)
@@ -971,7 +974,7 @@ UCG_BIG __device__ int block_lu_tiled(MP M, int n, int ld, LDS double *red, LDS 
 // blocked Gauss-Jordan elimination without pivoting, four pivots per sweep: the 4x4 pivot
 // block is inverted in thread 0's registers, its row panel is multiplied by that inverse
 // (one column per thread), every entry outside the pivot rows and columns takes the
-// rank-4 update in 4x4 tiles, then the pivot column panel becomes -A_IK inv(A_KK).  The
+// rank-4 update in 4x4 register tiles, then the pivot column panel becomes -A_IK inv(A_KK).  The
 // pivots are those of the unpivoted LU and are checked the same way; it is used for the
 // trailing block of the sparse plan so that each Newton solve applies the block as one
 // matrix-vector product instead of two dependent triangular sweeps.  stage: >= 16
@@ -981,7 +984,6 @@ UCG_BIG __device__ int block_gj_inverse(MP M, int n, int ld, LDS double *red, LD
 {
     constexpr int B = 4;
     LDS double *P = stage;   // P[l*4+m] = inv(A_KK)(l, m)
-    const int nt = (n + 3) >> 2;
     for (int kb = 0; kb < n; kb += B) {
         const int nb = min(B, n - kb);
         __syncthreads();
@@ -1037,7 +1039,7 @@ UCG_BIG __device__ int block_gj_inverse(MP M, int n, int ld, LDS double *red, LD
             }
         }
         __syncthreads();
-        const int kt = kb >> 2;
+        const int nt = (n + 3) >> 2, kt = kb >> 2;
         for (int tt = tid(); tt < nt * nt; tt += NT) {   // A_IJ -= A_IK A_KJ outside the pivot block
             const int ti = tt % nt, tj = tt / nt;
             if (ti == kt || tj == kt) continue;
@@ -1381,9 +1383,9 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
 // factors stay in the workspace and only the dense tail is staged.  The results end in V.
 // stage: 512 doubles of LDS.
 template <typename LUT, typename WP>
-__device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red)
+__device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red, int nph)
 {
-    for (int p = 0; p < N.nw_nph; p++) {
+    for (int p = 0; p < nph; p++) {
         const int t1 = N.nw_ph_ptr[p + 1];
         for (int t = N.nw_ph_ptr[p] + tid(); t < t1; t += NT) {
             const int tg = N.nw_tg_slot[t];
@@ -1431,7 +1433,7 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     uint64_t c1;
     if (!N.nw_stage) {                    // factors too large to stage: everything in the workspace
         __syncthreads();
-        crout_phases<LUT>(N, V, red);
+        crout_phases<LUT>(N, V, red, N.nw_nph);
         c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
         if (T > 0) r = block_lu_tiled<LUT>(V + nsp, T, T, red, stage);
@@ -1439,18 +1441,44 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
         LUT *Vl = (LUT *)region;
         for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
         __syncthreads();
-        crout_phases<LUT>(N, Vl, red);
+        crout_phases<LUT>(N, Vl, red, N.nw_nph);
         c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
         if (T > 0) r = block_gj_inverse<LUT>(Vl + nsp, T, T, red, stage);
         for (int q = tid(); q < nval; q += NT) V[q] = Vl[q];
     } else {
         __syncthreads();
-        crout_phases<LUT>(N, V, red);
-        c1 = wall_clock64();
+        const bool lp = UCG_LDS_SCHUR && N.nlp_src > 0;   // the last phase (Schur update of the tail) in LDS
+        crout_phases<LUT>(N, V, red, lp ? N.nw_nph - 1 : N.nw_nph);
         if (red[9] != 0.0) return 1;
         LUT *tl = (LUT *)region;
         for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
+        if (lp) {
+            LDS double *src = region + T * T;
+            for (int k = tid(); k < N.nlp_src; k += NT) src[k] = (double)V[N.lp_src[k]];
+            __syncthreads();
+            const int p = N.nw_nph - 1, t0 = N.nw_ph_ptr[p], t1 = N.nw_ph_ptr[p + 1], qb = N.nw_tg_ptr[t0];
+            const int2 *pr = reinterpret_cast<const int2 *>(N.lp_pr);
+            for (int t = t0 + tid(); t < t1; t += NT) {
+                const int e = N.nw_tg_slot[t] - nsp;
+                double a0 = (double)tl[e], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                const int q1 = N.nw_tg_ptr[t + 1] - qb;
+                int q = N.nw_tg_ptr[t] - qb;
+                for (; q + 4 <= q1; q += 4) {
+                    const int2 p0 = pr[q], p1 = pr[q + 1], p2 = pr[q + 2], p3 = pr[q + 3];
+                    a0 -= src[p0.x] * src[p0.y];
+                    a1 -= src[p1.x] * src[p1.y];
+                    a2 -= src[p2.x] * src[p2.y];
+                    a3 -= src[p3.x] * src[p3.y];
+                }
+                for (; q < q1; q++) {
+                    const int2 p0 = pr[q];
+                    a0 -= src[p0.x] * src[p0.y];
+                }
+                tl[e] = (LUT)((a0 + a1) + (a2 + a3));
+            }
+        }
+        c1 = wall_clock64();
         __syncthreads();
         if (T > 0) r = block_gj_inverse<LUT>(tl, T, T, red, stage);
         for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
@@ -2062,6 +2090,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
         const int model = (int)prm[P_MODEL];
         const double reltol = prm[P_RELTOL], absmin = prm[P_ABSMIN];
         const int mxstep = (int)prm[P_MXSTEP];
+        const double maxsteps = prm[P_MAXSTEPS];   // 0: unbounded
         const bool end_dens = prm[P_ENDATFINALDENS] != 0.0;
         // ---- time loop (wrap.f90:657-690).  The reference loops forever when the
         // integrator cannot advance t at all; here such a point stops with
@@ -2081,6 +2110,10 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevN
             const double orig = target;
             int loops = 0;
             while (t < target && loops < 10) {   // updateChemistry retry loop (chemistry.f90:164-223)
+                if (maxsteps > 0.0 && (double)st.nst >= maxsteps) {
+                    flag = -5;   // the caller's work bound (INT_TOO_MANY_FAILS_ERROR)
+                    break;
+                }
                 const uint64_t tr0 = wall_clock64();
                 block_rates(N, L);
                 t_rates += wall_clock64() - tr0;

@@ -21,7 +21,7 @@ MODULE uclchem_gpu
         UCGP_BASEAV = 5, UCGP_ROUT = 6, UCGP_RIN = 7, UCGP_FINALTIME = 8, UCGP_FREEFALL = 9, &
         UCGP_FINALDENS = 10, UCGP_RELTOL = 11, UCGP_ABSFAC = 12, UCGP_ABSMIN = 13, UCGP_MODEL = 14, &
         UCGP_TEMPINDX = 15, UCGP_MAXTEMP = 16, UCGP_FREEZEFAC = 17, UCGP_ENDATFINALDENS = 18, &
-        UCGP_FREEFALLFAC = 19, UCGP_MXSTEP = 20
+        UCGP_FREEFALLFAC = 19, UCGP_MXSTEP = 20, UCGP_MAXSTEPS = 21
 
     INTERFACE
         INTEGER(C_INT) FUNCTION ucg_init(device) BIND(C, name="ucg_init")
@@ -144,6 +144,7 @@ CONTAINS
         p(UCGP_ENDATFINALDENS) = 0.0D0
         p(UCGP_FREEFALLFAC) = 1.0D0
         p(UCGP_MXSTEP) = 10000.0D0
+        p(UCGP_MAXSTEPS) = 0.0D0
     END SUBROUTINE ucg_default_params
 
 END MODULE uclchem_gpu

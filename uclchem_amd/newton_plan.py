@@ -288,3 +288,53 @@ def plan_summary(plan) -> Dict[str, int]:
     s["values"] = s["nsp"] + s["T"] * s["T"]
     s["tail_lu_mads"] = s["T"] ** 3 // 3
     return s
+
+
+LDS_BYTES_PER_CU = 160 * 1024
+
+
+def lds_bytes(tables, staged: bool = True) -> int:
+    """LDS one solve workgroup needs (mirror of lds_bytes/flux_region_len in the HIP
+    library): the shared region (fluxes and gather partials, Jacobian term partials, and
+    for a staged plan the staged factors, solve program and dense tail) plus the NEQ-long
+    vectors, reduction slots, scalars and parameters."""
+    a = tables.arrays
+    z = a["nw_sizes"]
+    N, T, nsp, nlev = int(z[0]), int(z[2]), int(z[3]), int(z[4])
+    npad = (tables.neq + 7) & ~7
+    Npad = (N + 7) & ~7
+    npart = int(a["nfac"].sum())
+    ngt = sum(int(np.sum((np.diff(a[k]) + 23) // 24)) for k in ("prod_ptr", "loss_ptr"))
+    b = max(tables.nreac, ((npart + 7) & ~7) + 2 * npad, ((tables.nreac + 7) & ~7) + ngt)
+    if staged:
+        al = lambda x: (x + 15) & ~15
+        stage = lambda nst, lut: al(nst * lut) + al(nsp * 4) + N * 16 + al((N + nlev + 1) * 4) + N * 8 + T * 8
+        c = max(T * T + ((schur_sources(tables) + 7) & ~7) + Npad, (stage(nsp + T * T, 4) + 7) // 8,
+                (stage(nsp, 8) + 7) // 8)
+    else:
+        c = Npad
+    return 8 * (max(b, c) + 4 * npad + 16 + 24 + 24)
+
+
+def schur_sources(tables) -> int:
+    """Distinct sparse factor entries the last elimination phase reads when that phase is
+    the pure Schur-complement update of the dense tail (0 otherwise); the library gathers
+    them into LDS beside the staged tail."""
+    a = tables.arrays
+    nsp, nph = int(a["nw_sizes"][3]), int(a["nw_sizes"][7])
+    ph, tp = a["nw_ph_ptr"], a["nw_tg_ptr"]
+    if nph < 2:
+        return 0
+    t0, t1 = int(ph[nph - 1]), int(ph[nph])
+    q0, q1 = int(tp[t0]), int(tp[t1])
+    if q1 <= q0 or (a["nw_tg_slot"][t0:t1] < nsp).any() or (a["nw_tg_div"][t0:t1] >= 0).any():
+        return 0
+    refs = np.concatenate([a["nw_pr_l"][q0:q1], a["nw_pr_u"][q0:q1]])
+    if (refs >= nsp).any():
+        return 0
+    return int(len(np.unique(refs)))
+
+
+def plan_is_staged(tables) -> bool:
+    """Whether the library stages this network's plan in LDS (else it runs it from the workspace)."""
+    return lds_bytes(tables, True) <= LDS_BYTES_PER_CU

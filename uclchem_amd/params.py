@@ -14,14 +14,15 @@ import numpy as np
 NPARAM = 24
 NAMES = ["initialdens", "initialtemp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime",
          "freefall", "finaldens", "reltol", "abstol_factor", "abstol_min", "model", "tempindx",
-         "maxtemp", "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep"]
+         "maxtemp", "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps"]
 INDEX = {n: i for i, n in enumerate(NAMES)}
 
 DEFAULTS: Dict[str, float] = dict(
     initialdens=1.0e2, initialtemp=10.0, zeta=1.0, radfield=1.0, baseav=2.0,
     rout=float(np.float32(0.05)), rin=0.0, finaltime=5.0e6, freefall=0.0, finaldens=1.0e5,
     reltol=1e-8, abstol_factor=1e-14, abstol_min=1e-25, model=0.0, tempindx=1.0, maxtemp=300.0,
-    freezefactor=1.0, endatfinaldensity=0.0, freefallfactor=1.0, mxstep=10000.0)
+    freezefactor=1.0, endatfinaldensity=0.0, freefallfactor=1.0, mxstep=10000.0,
+    maxsteps=0.0)   # total-step bound per point, 0 = unbounded (no counterpart in the reference)
 
 ALIASES = {"dens": "initialdens", "temp": "initialtemp", "t_end": "finaltime"}
 SECONDS_PER_YEAR = 3.16e7   # constants.f90:14

@@ -39,6 +39,7 @@ struct NetHandle {
     size_t ws_bytes = 0;
     int *counter = nullptr;
     int *cancel = nullptr;              // device flag read by running solves
+    ucg::DevNet *d_dev = nullptr;       // device copy of dev, the kernels' table descriptor
     hipStream_t cancel_stream = nullptr;
     int linear_solver = UCG_LINSOL_SPARSE;
 };
@@ -231,17 +232,18 @@ SlotLayout slot_layout(const NetHandle &h, int lu_bits)
     return s;
 }
 
-int launch_kernel(const ucg::DevNet &N, const ucg::SolveArgs &A, int n_slots, int lu_bits, hipStream_t stream)
+int launch_kernel(const ucg::DevNet &N, const ucg::DevNet *dN, const ucg::SolveArgs &A, int n_slots, int lu_bits,
+                  hipStream_t stream)
 {
     const size_t lds = lds_bytes(N);
     if (lu_bits == 64) {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::solve_points_kernel<double>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+        hipLaunchKernelGGL(ucg::solve_points_kernel<double>, dim3(n_slots), dim3(ucg::NT), lds, stream, dN, A);
     } else {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<float>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::solve_points_kernel<float>, dim3(n_slots), dim3(ucg::NT), lds, stream, N, A);
+        hipLaunchKernelGGL(ucg::solve_points_kernel<float>, dim3(n_slots), dim3(ucg::NT), lds, stream, dN, A);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -286,7 +288,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.ws_jac_bytes = jac;
     A.counter = h.counter;
     A.cancel = h.cancel;
-    return launch_kernel(N, A, n_slots, lu_bits, stream);
+    return launch_kernel(N, h.d_dev, A, n_slots, lu_bits, stream);
 }
 
 int piece_common(NetHandle &h, int n, const double *params, const double *phys, ucg::PieceArgs &A,
@@ -742,6 +744,9 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     if (lds > (size_t)prop.sharedMemPerBlock)
         return fail(UCG_E_ARG, "network needs " + std::to_string(lds) + " B of LDS per workgroup, device allows " +
                                    std::to_string(prop.sharedMemPerBlock));
+    HIPCHK(hipMalloc(&h.d_dev, sizeof(ucg::DevNet)));   // the descriptor the kernels read (final values)
+    h.allocs.push_back(h.d_dev);
+    HIPCHK(hipMemcpy(h.d_dev, &N, sizeof(ucg::DevNet), hipMemcpyHostToDevice));
     HIPCHK(hipStreamCreateWithFlags(&h.cancel_stream, hipStreamNonBlocking));   // last: nothing fails after it
     return 0;
 }
@@ -887,7 +892,7 @@ int ucg_rates(int handle, int n, const double *params, const double *phys, const
     A.scal_out = (double *)ds.p;
     size_t lds = lds_bytes(N);
     HIPCHK(hipFuncSetAttribute((const void *)ucg::rates_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(ucg::rates_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    hipLaunchKernelGGL(ucg::rates_kernel, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(rate_out, dr.p, (size_t)n * N.nreac * sizeof(double), hipMemcpyDeviceToHost));
@@ -918,7 +923,7 @@ int ucg_rhs(int handle, int n, const double *params, const double *phys, const d
     A.rate_out = (double *)dro.p;
     size_t lds = lds_bytes(N);
     HIPCHK(hipFuncSetAttribute((const void *)ucg::rhs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(ucg::rhs_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    hipLaunchKernelGGL(ucg::rhs_kernel, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(ydot, dyo.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
@@ -946,7 +951,7 @@ int ucg_jacobian(int handle, int n, const double *params, const double *phys, co
     A.jac_out = (double *)dj.p;
     size_t lds = lds_bytes(N);
     HIPCHK(hipFuncSetAttribute((const void *)ucg::jac_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(ucg::jac_kernel, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+    hipLaunchKernelGGL(ucg::jac_kernel, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(jac, dj.p, (size_t)n * N.neq * N.neq * sizeof(double), hipMemcpyDeviceToHost));
@@ -978,11 +983,11 @@ int ucg_newton_solve(int handle, int n, const double *params, const double *phys
     if (lu_bits == 64) {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::newton_solve_kernel<double>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::newton_solve_kernel<double>, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+        hipLaunchKernelGGL(ucg::newton_solve_kernel<double>, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
     } else {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::newton_solve_kernel<float>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(ucg::newton_solve_kernel<float>, dim3(n), dim3(ucg::NT), lds, 0, N, A);
+        hipLaunchKernelGGL(ucg::newton_solve_kernel<float>, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
@@ -1083,7 +1088,7 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     A.q_closed = (const int *)d_ctrl + 1;
     A.q_done = (int *)d_done;
     A.q_idle_ticks = (long long)(idle_timeout_s * 1.0e8);
-    int rc = launch_kernel(N, A, n_slots, lu_bits, q.stream);
+    int rc = launch_kernel(N, h->d_dev, A, n_slots, lu_bits, q.stream);
     if (rc) {
         queue_release(q);
         return rc;

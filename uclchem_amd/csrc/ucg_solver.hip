@@ -107,7 +107,7 @@ __device__ inline double freeze_out(const DevNet &N, const LDS double *prm, int 
 // Rate coefficients for the current interval attempt (rates.f90:21-343 preceded by the
 // ice bookkeeping of chemistry.f90:167-200).  Reads y (LDS), writes rate (LDS), updates
 // y[BULK], y[SURFACE], the column densities, the ice scalars and phi.
-UCG_BIG __device__ void block_rates(const DevNet &N, Lds &L)
+UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
 {
     const int neq = N.neq;
     const LDS double *prm = L.prm;
@@ -324,7 +324,7 @@ __device__ __forceinline__ double term_value(const DevNet &N, int k, const doubl
 
 // RHS f(y) (chemistry.f90:294-352 + the generated GETYDOT, odes.f90).  Input L.y, output
 // L.f; also leaves the pre-transfer surface total in sc[S_SPRE] for the Jacobian.
-UCG_BIG __device__ void block_rhs(const DevNet &N, Lds &L, uint64_t *split = nullptr)
+UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split = nullptr)
 {
     const uint64_t c0 = wall_clock64();
     const int neq = N.neq;
@@ -536,7 +536,7 @@ __device__ __forceinline__ double jac_entry(const DevNet &N, const LDS double *P
 // BULK/SURFACE totals).  With scale != 0 the matrix is equilibrated by the error weights:
 // M' = W M W^-1, W = diag(ewt) (held in L.g).
 template <typename LUT>
-UCG_BIG __device__ void block_newton_matrix(const DevNet &N, Lds &L, LUT *M, double gamma, bool convert = true,
+UCG_BIG __device__ void block_newton_matrix(const DevNet &N, const Lds L, LUT *M, double gamma, bool convert = true,
                                     bool scaled = false)
 {
     const int neq = N.neq, ld = N.ld;
@@ -1208,7 +1208,7 @@ UCG_BIG __device__ void block_lu_solve(MP M, int n, int ld, LDS double *b)
 // transfer couplings (columns u1 and w2, rows -g1 and -gsum0, and c1); the BULK and
 // SURFACE rows are replaced by constant rows when the Newton matrix is formed
 // (sparse_form_newton).  Values are written in plan slot order; fill slots start at zero.
-UCG_BIG __device__ void sparse_jacobian(const DevNet &N, Lds &L, double *V)
+UCG_BIG __device__ void sparse_jacobian(const DevNet &N, const Lds L, double *V)
 {
     const int neq = N.neq;
     LDS double *y = L.y, *sc = L.sc;
@@ -1583,7 +1583,7 @@ __device__ __forceinline__ void solve_program(const DevNet &N, VP V, IP cols, IP
 // too, FP64 tails are read from the workspace by the dense triangular solves.  An unstaged
 // plan reads all of it from the workspace and keeps only the work vector in region.
 template <typename LUT>
-UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, LDS double *b, LDS double *region,
+UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, const Lds L, const LUT *Vg, LDS double *b, LDS double *region,
                                        uint64_t *split = nullptr)
 {
     const uint64_t c0 = wall_clock64();
@@ -1642,7 +1642,7 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, Lds &L, const LUT *Vg, 
 template <typename LUT>
 struct GpuOps {
     const DevNet &N;
-    Lds &L;
+    const Lds L;   // by value: the LDS and workspace pointers live in registers
     LUT *M;
     int neq;
     double z[6][EPT], acor[EPT], ewt[EPT], atol[EPT], f0[EPT];
@@ -1973,7 +1973,7 @@ __device__ void elemental_start(const DevNet &N, const LDS double *prm, LDS doub
     __syncthreads();
 }
 
-__device__ void record_row(const DevNet &N, Lds &L, const SolveArgs &A, int pidx, int row, double tyears)
+__device__ void record_row(const DevNet &N, const Lds L, const SolveArgs &A, int pidx, int row, double tyears)
 {
     if (A.max_rec <= 0 || row >= A.max_rec) return;
     const int ns = N.nspec;
@@ -2016,8 +2016,9 @@ __global__ void solve_init_kernel(int *counter, int *cancel, int *status, int n)
 }
 
 template <typename LUT>
-__global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(DevNet N, SolveArgs A)
+__global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(const DevNet *__restrict__ Np, SolveArgs A)
 {
+    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
     const int neq = N.neq;
     Lds L;
@@ -2253,8 +2254,9 @@ __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
 }
 
 
-__global__ __launch_bounds__(NT) void rates_kernel(DevNet N, PieceArgs A)
+__global__ __launch_bounds__(NT) void rates_kernel(const DevNet *__restrict__ Np, PieceArgs A)
 {
+    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
     Lds L;
     lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
@@ -2271,8 +2273,9 @@ __global__ __launch_bounds__(NT) void rates_kernel(DevNet N, PieceArgs A)
     }
 }
 
-__global__ __launch_bounds__(NT) void rhs_kernel(DevNet N, PieceArgs A)
+__global__ __launch_bounds__(NT) void rhs_kernel(const DevNet *__restrict__ Np, PieceArgs A)
 {
+    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
     Lds L;
     lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
@@ -2285,8 +2288,9 @@ __global__ __launch_bounds__(NT) void rhs_kernel(DevNet N, PieceArgs A)
         for (int k = tid(); k < N.nreac; k += NT) A.rate_out[(size_t)p * N.nreac + k] = L.rate[k];
 }
 
-__global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
+__global__ __launch_bounds__(NT) void jac_kernel(const DevNet *__restrict__ Np, PieceArgs A)
 {
+    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
     Lds L;
     lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
@@ -2305,8 +2309,9 @@ __global__ __launch_bounds__(NT) void jac_kernel(DevNet N, PieceArgs A)
 
 // LU + solve of M = I - gamma J: returns x = M^-1 b (b passed in y, x returned in y_out)
 template <typename LUT>
-__global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
+__global__ __launch_bounds__(NT) void newton_solve_kernel(const DevNet *__restrict__ Np, PieceArgs A)
 {
+    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
     Lds L;
     lds_layout(N, L, lds_raw, (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes + A.ws_jac_bytes));
@@ -2338,9 +2343,9 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(DevNet N, PieceArgs A)
     for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = bad ? NAN : L.b[i];
 }
 
-template __global__ void solve_points_kernel<double>(DevNet, SolveArgs);
-template __global__ void solve_points_kernel<float>(DevNet, SolveArgs);
-template __global__ void newton_solve_kernel<double>(DevNet, PieceArgs);
-template __global__ void newton_solve_kernel<float>(DevNet, PieceArgs);
+template __global__ void solve_points_kernel<double>(const DevNet *, SolveArgs);
+template __global__ void solve_points_kernel<float>(const DevNet *, SolveArgs);
+template __global__ void newton_solve_kernel<double>(const DevNet *, PieceArgs);
+template __global__ void newton_solve_kernel<float>(const DevNet *, PieceArgs);
 
 }  // namespace ucg

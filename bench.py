@@ -222,6 +222,12 @@ def main():
     lu_flop, solve_flop = newton_flop_counts(tables)
     flops = algorithmic_flops(main_r["stats"], lu_flop, solve_flop, rhs_flop_count(tables))
     achieved = flops / (main_r["ms"] / 1e3)
+    traffic = None      # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
+    tpath = os.path.join(ROOT, "profiles", "solve_points_kernel_grid4096_traffic.txt")
+    if a.workload == "grid" and a.lu == 64 and n == 4096 and a.max_steps == 100000.0 and os.path.exists(tpath):
+        for ln in open(tpath):
+            if ln.startswith("traffic_bytes_per_launch"):
+                traffic = float(ln.split()[1])
     wl = {"grid": (f"static cloud grid, default network ({tables.nspec} species, {tables.nreac} reactions), "
                    f"{n} points to 1e6 yr, FP{a.lu} factorisation" + (" (plain batched FP64)" if a.lu == 64 else "")),
           "collapse500": (f"free-fall collapse grid (n_H,0 log-uniform 1e2..1e6 towards 1e6 cm^-3, 1 Myr), "
@@ -247,11 +253,12 @@ def main():
                             "newton_setups": int(main_r["stats"][:, 2].sum()),
                             "newton_iters": int(main_r["stats"][:, 3].sum())},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "note": (f"solve_points_kernel (one launch = the whole grid); FP64 FLOPs of the sparse-plan "
                               f"factorisation ({lu_flop:.3g} per Newton-matrix setup), solves ({solve_flop:.3g} per "
                               f"Newton iteration) and RHS ({rhs_flop_count(tables):.3g} per evaluation) from the "
-                              f"solver counters, over the launch time")},
+                              f"solver counters, over the launch time; traffic: HBM bytes per launch from "
+                              f"rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy (profiles/)")},
     }
     if 32 in results and a.lu == 64:
         r32 = results[32]

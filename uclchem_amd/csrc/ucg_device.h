@@ -14,6 +14,28 @@ namespace ucg {
 // pointer into workgroup-local memory (LDS); keeps the compiler from falling back to
 // flat addressing, whose loads wait on both the LDS and the vector-memory counters
 #define LDS __attribute__((address_space(3)))
+// global-memory views of the table and workspace pointers: loads through them are global_load
+// (counted by vmcnt only), so waiting for an LDS value does not also wait for them, as it
+// must for flat (generic) loads
+#define GLB __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ const GLB T *glb(const T *p) { return (const GLB T *)p; }
+template <class T> __device__ __forceinline__ GLB T *glbw(T *p) { return (GLB T *)p; }
+__device__ __forceinline__ int2 ld_int2(const GLB int *p, int i)
+{
+    int2 r;
+    r.x = p[2 * i];
+    r.y = p[2 * i + 1];
+    return r;
+}
+__device__ __forceinline__ int4 ld_int4(const GLB int *p, int i)
+{
+    int4 r;
+    r.x = p[4 * i];
+    r.y = p[4 * i + 1];
+    r.z = p[4 * i + 2];
+    r.w = p[4 * i + 3];
+    return r;
+}
 
 constexpr int NT = 256;          // threads per workgroup (one workgroup solves one grid point)
 constexpr int EPT = 2;           // state elements owned per thread => NEQ <= 512

@@ -364,7 +364,8 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split 
             flux[N.r_h2_hv] = term_value(N, N.r_h2_hv, rate, y, D, blr, TS, sm);
         }
     } else {
-        const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
+        const GLB int *fp = glb(N.facp);
+        const GLB double *rg = glb(rate);
         const int nr = N.nreac;
         const int w = tid(), nw = NT - 64;
         const int kh = N.r_h2_hv, kc = N.r_co_hv;
@@ -375,8 +376,8 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split 
             for (int u = 0; u < 4; u++) {
                 const int k = k0 + u * nw;
                 if (k < nr) {
-                    wr[u] = fp[k];
-                    r[u] = rate[k];
+                    wr[u] = ld_int4(fp, k);
+                    r[u] = rg[k];
                 }
             }
 #pragma unroll
@@ -396,10 +397,11 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split 
     // then per species in chunk order
     LDS double *part = flux + N.gt_off;
     {
-        const int4 *gr = reinterpret_cast<const int4 *>(N.gt_rec);
+        const GLB int *gr = glb(N.gt_rec);
+        const GLB int *lidx = glb(N.loss_idx), *pidx = glb(N.prod_idx);
         for (int t = tid(); t < N.ngt; t += NT) {
-            const int4 rc = gr[t];
-            const int *src = rc.w ? N.loss_idx : N.prod_idx;
+            const int4 rc = ld_int4(gr, t);
+            const GLB int *src = rc.w ? lidx : pidx;
             double acc = 0.0;
 #pragma unroll 8
             for (int q = rc.x; q < rc.y; q++) acc += flux[src[q]];
@@ -416,9 +418,10 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split 
         pre[e] = 0.0;
         if (i < N.nspec) {
             double prod = 0.0, loss = 0.0;
-            const int q0 = N.gt_sp_ptr[2 * i], q1 = N.gt_sp_ptr[2 * i + 1], q2 = N.gt_sp_ptr[2 * i + 2];
-            for (int q = q0; q < q1; q++) prod += part[N.gt_sp_task[q]];
-            for (int q = q1; q < q2; q++) loss += part[N.gt_sp_task[q]];
+            const GLB int *spp = glb(N.gt_sp_ptr), *spt = glb(N.gt_sp_task);
+            const int q0 = spp[2 * i], q1 = spp[2 * i + 1], q2 = spp[2 * i + 2];
+            for (int q = q0; q < q1; q++) prod += part[spt[q]];
+            for (int q = q1; q < q2; q++) loss += part[spt[q]];
             pre[e] = prod - loss;
             if (N.is_surface[i]) spre += pre[e];
         }
@@ -1385,27 +1388,28 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
 template <typename LUT, typename WP>
 __device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red, int nph)
 {
+    const GLB int *ph = glb(N.nw_ph_ptr), *tgs = glb(N.nw_tg_slot), *tgp = glb(N.nw_tg_ptr), *tgd = glb(N.nw_tg_div);
+    const GLB int *pr = glb(N.nw_pr);
     for (int p = 0; p < nph; p++) {
-        const int t1 = N.nw_ph_ptr[p + 1];
-        for (int t = N.nw_ph_ptr[p] + tid(); t < t1; t += NT) {
-            const int tg = N.nw_tg_slot[t];
+        const int t1 = ph[p + 1];
+        for (int t = ph[p] + tid(); t < t1; t += NT) {
+            const int tg = tgs[t];
             double a0 = (double)W[tg], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            const int q1 = N.nw_tg_ptr[t + 1];
-            const int2 *pr = reinterpret_cast<const int2 *>(N.nw_pr);
-            int q = N.nw_tg_ptr[t];
+            const int q1 = tgp[t + 1];
+            int q = tgp[t];
             for (; q + 4 <= q1; q += 4) {
-                const int2 p0 = pr[q], p1 = pr[q + 1], p2 = pr[q + 2], p3 = pr[q + 3];
+                const int2 p0 = ld_int2(pr, q), p1 = ld_int2(pr, q + 1), p2 = ld_int2(pr, q + 2), p3 = ld_int2(pr, q + 3);
                 a0 -= (double)W[p0.x] * (double)W[p0.y];
                 a1 -= (double)W[p1.x] * (double)W[p1.y];
                 a2 -= (double)W[p2.x] * (double)W[p2.y];
                 a3 -= (double)W[p3.x] * (double)W[p3.y];
             }
             for (; q < q1; q++) {
-                const int2 p0 = pr[q];
+                const int2 p0 = ld_int2(pr, q);
                 a0 -= (double)W[p0.x] * (double)W[p0.y];
             }
             double acc = (a0 + a1) + (a2 + a3);
-            const int dv = N.nw_tg_div[t];
+            const int dv = tgd[t];
             if (dv >= 0) {
                 const double piv = (double)W[dv];
                 if (!(piv == piv) || piv == 0.0 || isinf(piv)) red[9] = 1.0;
@@ -1433,12 +1437,12 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     uint64_t c1;
     if (!N.nw_stage) {                    // factors too large to stage: everything in the workspace
         __syncthreads();
-        crout_phases<LUT>(N, V, red, N.nw_nph);
+        crout_phases<LUT>(N, glbw(V), red, N.nw_nph);
         c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
         if (T > 0) r = block_lu_tiled<LUT>(V + nsp, T, T, red, stage);
     } else if constexpr (sizeof(LUT) == 4) {
-        LUT *Vl = (LUT *)region;
+        LDS LUT *Vl = (LDS LUT *)region;   // LDS-typed: ds_read/ds_write, not flat accesses
         for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
         __syncthreads();
         crout_phases<LUT>(N, Vl, red, N.nw_nph);
@@ -1449,30 +1453,32 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     } else {
         __syncthreads();
         const bool lp = UCG_LDS_SCHUR && N.nlp_src > 0;   // the last phase (Schur update of the tail) in LDS
-        crout_phases<LUT>(N, V, red, lp ? N.nw_nph - 1 : N.nw_nph);
+        crout_phases<LUT>(N, glbw(V), red, lp ? N.nw_nph - 1 : N.nw_nph);
         if (red[9] != 0.0) return 1;
-        LUT *tl = (LUT *)region;
+        LDS LUT *tl = (LDS LUT *)region;
         for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
         if (lp) {
             LDS double *src = region + T * T;
-            for (int k = tid(); k < N.nlp_src; k += NT) src[k] = (double)V[N.lp_src[k]];
+            const GLB LUT *Vg = glb(V);
+            const GLB int *lsrc = glb(N.lp_src), *tgs = glb(N.nw_tg_slot), *tgp = glb(N.nw_tg_ptr);
+            for (int k = tid(); k < N.nlp_src; k += NT) src[k] = (double)Vg[lsrc[k]];
             __syncthreads();
             const int p = N.nw_nph - 1, t0 = N.nw_ph_ptr[p], t1 = N.nw_ph_ptr[p + 1], qb = N.nw_tg_ptr[t0];
-            const int2 *pr = reinterpret_cast<const int2 *>(N.lp_pr);
+            const GLB int *pr = glb(N.lp_pr);
             for (int t = t0 + tid(); t < t1; t += NT) {
-                const int e = N.nw_tg_slot[t] - nsp;
+                const int e = tgs[t] - nsp;
                 double a0 = (double)tl[e], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-                const int q1 = N.nw_tg_ptr[t + 1] - qb;
-                int q = N.nw_tg_ptr[t] - qb;
+                const int q1 = tgp[t + 1] - qb;
+                int q = tgp[t] - qb;
                 for (; q + 4 <= q1; q += 4) {
-                    const int2 p0 = pr[q], p1 = pr[q + 1], p2 = pr[q + 2], p3 = pr[q + 3];
+                    const int2 p0 = ld_int2(pr, q), p1 = ld_int2(pr, q + 1), p2 = ld_int2(pr, q + 2), p3 = ld_int2(pr, q + 3);
                     a0 -= src[p0.x] * src[p0.y];
                     a1 -= src[p1.x] * src[p1.y];
                     a2 -= src[p2.x] * src[p2.y];
                     a3 -= src[p3.x] * src[p3.y];
                 }
                 for (; q < q1; q++) {
-                    const int2 p0 = pr[q];
+                    const int2 p0 = ld_int2(pr, q);
                     a0 -= src[p0.x] * src[p0.y];
                 }
                 tl[e] = (LUT)((a0 + a1) + (a2 + a3));
@@ -1611,21 +1617,25 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, const Lds L, const LUT 
     LDS int *inv = (LDS int *)(base + off);
     off += ((NN + N.nw_nlev + 1) * 4 + 15) & ~15;   // permutation, then the level pointers
     LDS double *w = (LDS double *)(base + off);
-    for (int q = tid(); q < nst; q += NT) Vs[q] = Vg[q];
-    for (int q = tid(); q < nsp; q += NT) cols[q] = N.nw_sp_col[q];
     {
-        const int4 *gr = reinterpret_cast<const int4 *>(N.sv_rec);
+        const GLB LUT *vg = glb(Vg);
+        const GLB int *cg = glb(N.nw_sp_col);
+        for (int q = tid(); q < nst; q += NT) Vs[q] = vg[q];
+        for (int q = tid(); q < nsp; q += NT) cols[q] = cg[q];
+    }
+    {
+        const GLB int *gr = glb(N.sv_rec);
         for (int q = tid(); q < NN; q += NT) {
-            const int4 g = gr[q];
+            const int4 g = ld_int4(gr, q);
             recs[4 * q] = g.x;
             recs[4 * q + 1] = g.y;
             recs[4 * q + 2] = g.z;
             recs[4 * q + 3] = g.w;
         }
     }
-    for (int q = tid(); q < NN; q += NT) inv[q] = N.nw_inv[q];
+    for (int q = tid(); q < NN; q += NT) inv[q] = glb(N.nw_inv)[q];
     LDS int *lev = inv + NN;   // level pointers (nlev + 1), after the permutation
-    for (int q = tid(); q <= N.nw_nlev; q += NT) lev[q] = N.nw_lev_ptr[q];
+    for (int q = tid(); q <= N.nw_nlev; q += NT) lev[q] = glb(N.nw_lev_ptr)[q];
     block_sum2(sb, ss, L.red);   // its barriers also publish the staged program
     const uint64_t cs = wall_clock64();
     const LDS LUT *V = Vs;

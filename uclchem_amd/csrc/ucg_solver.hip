@@ -983,7 +983,7 @@ UCG_BIG __device__ int block_lu_tiled(MP M, int n, int ld, LDS double *red, LDS 
 // matrix-vector product instead of two dependent triangular sweeps.  stage: >= 16
 // doubles of LDS.  Returns nonzero (uniformly) on a zero or non-finite pivot.
 template <typename LUT, typename MP>
-UCG_BIG __device__ int block_gj_inverse(MP M, int n, int ld, LDS double *red, LDS double *stage)
+__device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int ld, LDS double *red, LDS double *stage)
 {
     constexpr int B = 4;
     LDS double *P = stage;   // P[l*4+m] = inv(A_KK)(l, m)
@@ -1382,9 +1382,9 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
 // Crout elimination of the sparse pivots in plan phases (one thread per target entry,
 // dependencies only on earlier phases), then the trailing block (its Schur complement):
 // inverted in place by blocked Gauss-Jordan when the plan is staged, LU-factored when it
-// is not.  FP32 factors are staged in LDS (region) for the whole factorisation; FP64
-// factors stay in the workspace and only the dense tail is staged.  The results end in V.
-// stage: 512 doubles of LDS.
+// is not.  The sparse factors stay in the workspace in their precision (LUT); a staged
+// plan's tail, with the Schur-update sources, is worked on in FP64 in LDS (region) and
+// stored back in LUT.  stage: 512 doubles of LDS.
 template <typename LUT, typename WP>
 __device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red, int nph)
 {
@@ -1441,22 +1441,13 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
         c1 = wall_clock64();
         if (red[9] != 0.0) return 1;
         if (T > 0) r = block_lu_tiled<LUT>(V + nsp, T, T, red, stage);
-    } else if constexpr (sizeof(LUT) == 4) {
-        LDS LUT *Vl = (LDS LUT *)region;   // LDS-typed: ds_read/ds_write, not flat accesses
-        for (int q = tid(); q < nval; q += NT) Vl[q] = V[q];
-        __syncthreads();
-        crout_phases<LUT>(N, Vl, red, N.nw_nph);
-        c1 = wall_clock64();
-        if (red[9] != 0.0) return 1;
-        if (T > 0) r = block_gj_inverse<LUT>(Vl + nsp, T, T, red, stage);
-        for (int q = tid(); q < nval; q += NT) V[q] = Vl[q];
     } else {
         __syncthreads();
         const bool lp = UCG_LDS_SCHUR && N.nlp_src > 0;   // the last phase (Schur update of the tail) in LDS
         crout_phases<LUT>(N, glbw(V), red, lp ? N.nw_nph - 1 : N.nw_nph);
         if (red[9] != 0.0) return 1;
-        LDS LUT *tl = (LDS LUT *)region;
-        for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
+        LDS double *tl = region;   // the tail in FP64 whatever the factor precision
+        for (int q = tid(); q < T * T; q += NT) tl[q] = (double)V[nsp + q];
         if (lp) {
             LDS double *src = region + T * T;
             const GLB LUT *Vg = glb(V);
@@ -1481,13 +1472,13 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
                     const int2 p0 = ld_int2(pr, q);
                     a0 -= src[p0.x] * src[p0.y];
                 }
-                tl[e] = (LUT)((a0 + a1) + (a2 + a3));
+                tl[e] = (a0 + a1) + (a2 + a3);
             }
         }
         c1 = wall_clock64();
         __syncthreads();
-        if (T > 0) r = block_gj_inverse<LUT>(tl, T, T, red, stage);
-        for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
+        if (T > 0) r = block_gj_inverse<double>(tl, T, T, red, stage);
+        for (int q = tid(); q < T * T; q += NT) V[nsp + q] = (LUT)tl[q];
     }
     __syncthreads();
     if (split) {

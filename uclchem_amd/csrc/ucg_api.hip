@@ -619,6 +619,16 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     UP(aff_ptr, R + 1); UP(aff_sp, (size_t)d->naff); UP(aff_cf, (size_t)d->naff);
     UP(sw_ptr, Q + 1); UP(sw_k, (size_t)d->nsw); UP(sw_c, (size_t)d->nsw);
     if ((rc = check_plan(d))) return rc;
+    {   // the Jacobian sums d totalSwap/dy per bulk species without ordering: one BULKSWAP each
+        std::vector<char> seen((size_t)d->neq, 0);
+        for (int q = 0; q < d->nswap; q++) {
+            const int k = d->swap_list[q];
+            if (k < 0 || k >= d->nreac) return fail(UCG_E_ARG, "swap list index out of range");
+            const int b = d->ops[ucg::MAXOPS * k];
+            if (b < 0 || b >= d->neq || seen[b]) return fail(UCG_E_ARG, "BULKSWAP reactants must be distinct species");
+            seen[b] = 1;
+        }
+    }
     {
         const int *z = d->nw_sizes;
         N.nw_N = z[0]; N.nw_ns = z[1]; N.nw_T = z[2]; N.nw_nsp = z[3]; N.nw_nlev = z[4]; N.nw_nph = z[7];

@@ -498,10 +498,11 @@ __device__ __forceinline__ double fac_deriv(int c, double dblr, double dism, dou
 __device__ void block_term_partials(const DevNet &N, const double *rate, const LDS double *y, LDS double *P, double D,
                                     double blr, double ism, double tsm, double dblr, double dism, double dtsm)
 {
-    const int4 *fp = reinterpret_cast<const int4 *>(N.facp);
+    const GLB int *fp = glb(N.facp);
+    const GLB double *rg = glb(rate);
     for (int k = tid(); k < N.nreac; k += NT) {
         int c[8];
-        decode_codes(fp[k], c);
+        decode_codes(ld_int4(fp, k), c);
         double v[8], pre[8];
         double acc = 1.0;
 #pragma unroll
@@ -510,8 +511,8 @@ __device__ void block_term_partials(const DevNet &N, const double *rate, const L
             pre[m] = acc;
             acc *= v[m];
         }
-        const double r = rate[k];
-        const int off = N.fac_off[k];
+        const double r = rg[k];
+        const int off = glb(N.fac_off)[k];
         double suf = 1.0;
 #pragma unroll
         for (int m = 7; m >= 0; m--) {
@@ -525,10 +526,11 @@ __device__ void block_term_partials(const DevNet &N, const double *rate, const L
 __device__ __forceinline__ double jac_entry(const DevNet &N, const LDS double *P, int n)
 {
     double acc = 0.0;
-    const int q1 = N.jnz_ptr[n + 1];
+    const GLB int *jp = glb(N.jnz_ptr), *pk_ = glb(N.jc_pk);
+    const int q1 = jp[n + 1];
 #pragma unroll 4
-    for (int q = N.jnz_ptr[n]; q < q1; q++) {
-        const int pk = N.jc_pk[q];
+    for (int q = jp[n]; q < q1; q++) {
+        const int pk = pk_[q];
         acc += (double)((pk & 15) - 8) * P[pk >> 4];
     }
     return acc;
@@ -1229,12 +1231,15 @@ UCG_BIG __device__ void sparse_jacobian(const DevNet &N, const Lds L, double *V)
     const double TS = block_sum(ts, L.red);
     const double tsm = TS / sm;
     const double dtsm = (ysur > 1e-30) ? -TS * ism * ism : 0.0;
-    for (int q = tid(); q < N.nw_nval; q += NT) V[q] = 0;
+    {
+        GLB double *Vw = glbw(V);
+        for (int q = tid(); q < N.nw_nval; q += NT) Vw[q] = 0;
+    }
     LDS double *P = L.flux;
     block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 1. generated entries of J0, stored as -dJ; u1 (totalSwap column factor)
-    for (int n = tid(); n < N.nnz; n += NT) V[N.nw_jnz_slot[n]] = (-jac_entry(N, P, n));
+    for (int n = tid(); n < N.nnz; n += NT) glbw(V)[glb(N.nw_jnz_slot)[n]] = (-jac_entry(N, P, n));
     for (int i = tid(); i < neq; i += NT) {
         double u = 0.0;
         for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {
@@ -1251,13 +1256,14 @@ UCG_BIG __device__ void sparse_jacobian(const DevNet &N, const Lds L, double *V)
         scr[neq + i] = 0.0;
     }
     __syncthreads();
-    if (tid() == 0) {   // g1 = d totalSwap / dy
-        for (int q = 0; q < N.nswap; q++) {
-            const int k = N.swap_list[q];
-            scr[neq + N.ops[MAXOPS * k]] += rate[k] * blr;
-        }
-        scr[neq + N.nbulk] = (blr < 1.0 && ybulk > 1e-30) ? -TS / sbk : 0.0;
+    // g1 = d totalSwap / dy: one BULKSWAP per bulk species, so the targets are distinct
+    // (checked when the network is loaded) and the sums need no ordering
+    for (int q = tid(); q < N.nswap; q += NT) {
+        const int k = glb(N.swap_list)[q];
+        scr[neq + glb(N.ops)[MAXOPS * k]] += glb(rate)[k] * blr;
     }
+    __syncthreads();
+    if (tid() == 0) scr[neq + N.nbulk] = (blr < 1.0 && ybulk > 1e-30) ? -TS / sbk : 0.0;
     __syncthreads();
     // 2. border rows and columns, constant sum rows
     for (int j = tid(); j < neq; j += NT) {

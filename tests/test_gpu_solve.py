@@ -167,3 +167,22 @@ def test_cancel_stops_a_running_batch(gpu_net):
     assert (st == gpu_net.CANCELLED).any()
     again = gpu_net.solve(params[:2])
     assert np.all(again["status"] == 0)
+
+
+def test_step_bound_stops_a_point(gpu_net):
+    """UCG_P_MAXSTEPS: a point that reaches the bound stops with INT_TOO_MANY_FAILS_ERROR after at
+    most one more integrator call of work; 0 (the default) leaves the reference semantics."""
+    from uclchem_amd import workloads
+    from uclchem_amd.params import INDEX
+    p = workloads.static_cloud_point()
+    free = gpu_net.solve(p)
+    bounded = p.copy()
+    bounded[:, INDEX["maxsteps"]] = 500
+    out = gpu_net.solve(bounded)
+    assert free["status"][0] == 0 and free["stats"][0, 0] > 500
+    assert out["status"][0] == -5
+    assert 500 <= out["stats"][0, 0] <= 500 + int(p[0, INDEX["mxstep"]])
+    loose = p.copy()
+    loose[:, INDEX["maxsteps"]] = 10 * free["stats"][0, 0]
+    again = gpu_net.solve(loose)
+    assert again["status"][0] == 0 and np.array_equal(again["y"], free["y"])

@@ -1494,6 +1494,37 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     return r;
 }
 
+// Lanes per row for a level of R independent rows: the largest power of two with R groups
+// fitting the workgroup, at most a wave (groups never straddle a wave).
+__device__ __forceinline__ int row_group(int R)
+{
+    int G = 64;
+    while (G > 1 && G * R > NT) G >>= 1;
+    return G;
+}
+
+// Partial dot product of one row's entries e0, e0+G, ... < e1 with the work vector.
+template <typename VP, typename IP>
+__device__ __forceinline__ double row_dot(VP V, IP cols, const LDS double *w, int e0, int e1, int G)
+{
+    double a0 = 0.0, a1 = 0.0;
+    int e = e0;
+    for (; e + G < e1; e += 2 * G) {
+        a0 += (double)V[e] * w[cols[e]];
+        a1 += (double)V[e + G] * w[cols[e + G]];
+    }
+    if (e < e1) a0 += (double)V[e] * w[cols[e]];
+    return a0 + a1;
+}
+
+// Sum over an aligned group of G lanes (G a power of two <= 64); every lane of the wave
+// takes part, each lane ends with its group's sum.
+__device__ __forceinline__ double group_sum(double v, int G)
+{
+    for (int off = G >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 // Level-scheduled substitutions of the static plan on the permuted work vector w (LDS):
 // forward over the sparse rows by level, the tail rows' L21 part, the dense tail, then
 // backward by level.  V, the column indices and the row records are LDS or workspace
@@ -1516,34 +1547,37 @@ __device__ __forceinline__ void solve_program(const DevNet &N, VP V, IP cols, IP
     __syncthreads();
     const uint64_t cf0 = wall_clock64();
     for (int l = 0; l < N.nw_nlev; l++) {   // forward, sparse rows by level
-        const int q1 = lev[l + 1];
-        for (int q = lev[l] + tid(); q < q1; q += NT) {
-            const int r = recs[4 * q], e0 = recs[4 * q + 1], ed = recs[4 * q + 2];
-            double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;   // four chains hide the FMA latency
-            int e = e0;
-            for (; e + 4 <= ed; e += 4) {
-                a0 -= (double)V[e] * w[cols[e]];
-                a1 -= (double)V[e + 1] * w[cols[e + 1]];
-                a2 -= (double)V[e + 2] * w[cols[e + 2]];
-                a3 -= (double)V[e + 3] * w[cols[e + 3]];
+        const int q0 = lev[l];
+        const int R = lev[l + 1] - q0;
+        const int G = row_group(R);
+        for (int base = 0; base < R; base += NT / G) {
+            const int g = base + tid() / G, ln = tid() % G;
+            double acc = 0.0;
+            int r = -1;
+            if (g < R) {
+                const int q = q0 + g;
+                r = recs[4 * q];
+                acc = row_dot(V, cols, w, recs[4 * q + 1] + ln, recs[4 * q + 2], G);
             }
-            for (; e < ed; e++) a0 -= (double)V[e] * w[cols[e]];
-            w[r] = (a0 + a1) + (a2 + a3);
+            acc = group_sum(acc, G);
+            if (r >= 0 && ln == 0) w[r] -= acc;
         }
         __syncthreads();
     }
-    for (int q = ns + tid(); q < NN; q += NT) {   // tail rows: L21
-        const int r = recs[4 * q], e0 = recs[4 * q + 1], e1 = recs[4 * q + 3];
-        double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int e = e0;
-        for (; e + 4 <= e1; e += 4) {
-            a0 -= (double)V[e] * w[cols[e]];
-            a1 -= (double)V[e + 1] * w[cols[e + 1]];
-            a2 -= (double)V[e + 2] * w[cols[e + 2]];
-            a3 -= (double)V[e + 3] * w[cols[e + 3]];
+    {   // tail rows: L21
+        const int R = NN - ns, G = row_group(R);
+        for (int base = 0; base < R; base += NT / G) {
+            const int g = base + tid() / G, ln = tid() % G;
+            double acc = 0.0;
+            int r = -1;
+            if (g < R) {
+                const int q = ns + g;
+                r = recs[4 * q];
+                acc = row_dot(V, cols, w, recs[4 * q + 1] + ln, recs[4 * q + 3], G);
+            }
+            acc = group_sum(acc, G);
+            if (r >= 0 && ln == 0) w[r] -= acc;
         }
-        for (; e < e1; e++) a0 -= (double)V[e] * w[cols[e]];
-        w[r] = (a0 + a1) + (a2 + a3);
     }
     __syncthreads();
     const uint64_t c1 = wall_clock64();
@@ -1553,19 +1587,21 @@ __device__ __forceinline__ void solve_program(const DevNet &N, VP V, IP cols, IP
     }
     const uint64_t c2 = wall_clock64();
     for (int l = N.nw_nlev - 1; l >= 0; l--) {   // backward, sparse rows by level
-        const int q1 = lev[l + 1];
-        for (int q = lev[l] + tid(); q < q1; q += NT) {
-            const int r = recs[4 * q], ed = recs[4 * q + 2], e1 = recs[4 * q + 3];
-            double a0 = w[r], a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int e = ed + 1;
-            for (; e + 4 <= e1; e += 4) {
-                a0 -= (double)V[e] * w[cols[e]];
-                a1 -= (double)V[e + 1] * w[cols[e + 1]];
-                a2 -= (double)V[e + 2] * w[cols[e + 2]];
-                a3 -= (double)V[e + 3] * w[cols[e + 3]];
+        const int q0 = lev[l];
+        const int R = lev[l + 1] - q0;
+        const int G = row_group(R);
+        for (int base = 0; base < R; base += NT / G) {
+            const int g = base + tid() / G, ln = tid() % G;
+            double acc = 0.0;
+            int r = -1, ed = 0;
+            if (g < R) {
+                const int q = q0 + g;
+                r = recs[4 * q];
+                ed = recs[4 * q + 2];
+                acc = row_dot(V, cols, w, ed + 1 + ln, recs[4 * q + 3], G);
             }
-            for (; e < e1; e++) a0 -= (double)V[e] * w[cols[e]];
-            w[r] = ((a0 + a1) + (a2 + a3)) / (double)V[ed];
+            acc = group_sum(acc, G);
+            if (r >= 0 && ln == 0) w[r] = (w[r] - acc) / (double)V[ed];
         }
         __syncthreads();
     }

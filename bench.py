@@ -131,7 +131,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; grid points are independent and would shard "
-                          "by contiguous point ranges with no collective (see DESIGN.md)", "n_gpus": a.gpus}))
+                          "by contiguous point ranges with no collective", "n_gpus": a.gpus}))
         sys.exit(2)
     if a.copy_calibration > 0:
         import torch

@@ -186,3 +186,31 @@ def test_step_bound_stops_a_point(gpu_net):
     loose[:, INDEX["maxsteps"]] = 10 * free["stats"][0, 0]
     again = gpu_net.solve(loose)
     assert again["status"][0] == 0 and np.array_equal(again["y"], free["y"])
+
+
+def test_scheduling_order_leaves_results_unchanged(gpu_net):
+    """solve_device(order=...) takes the points densest first (or in a given permutation) and
+    returns every output in index order, bit-identical to the index-order solve."""
+    import torch
+    from uclchem_amd import workloads
+    from uclchem_amd.params import INDEX
+    params = workloads.static_cloud_grid(24)
+    params[:, INDEX["finaltime"]] = 1e4
+    dev = torch.device("cuda:0")
+    dp = torch.tensor(params, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    runs = []
+    for order in (None, "densest", torch.arange(23, -1, -1, device=dev)):
+        dy = torch.zeros((len(params), gpu_net.neq), dtype=torch.float64, device=dev)
+        dst = torch.zeros(len(params), dtype=torch.int32, device=dev)
+        dstats = torch.zeros((len(params), 8), dtype=torch.int32, device=dev)
+        t0 = torch.zeros(len(params), dtype=torch.float64, device=dev)
+        t1 = torch.zeros(len(params), dtype=torch.float64, device=dev)
+        gpu_net.solve_device(dp, dy, dst, dstats, t_start=t0, t_done=t1, stream=stream.cuda_stream, order=order)
+        torch.cuda.synchronize()
+        assert bool((t1 >= t0).all()) and bool((t0 > 0).all())
+        runs.append((dy.cpu().numpy(), dst.cpu().numpy(), dstats.cpu().numpy()))
+    for y, st, stats in runs[1:]:
+        assert np.array_equal(y, runs[0][0])
+        assert np.array_equal(st, runs[0][1]) and np.all(st == 0)
+        assert np.array_equal(stats, runs[0][2])

@@ -1,0 +1,75 @@
+"""Host side of the point scheduling order of Network.solve_device (CPU tensors)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+
+def test_densest_first_is_a_stable_descending_permutation():
+    from uclchem_amd.gpu import schedule_order
+    from uclchem_amd.params import INDEX
+    from uclchem_amd import workloads
+    p = workloads.static_cloud_grid(64)
+    p[10, INDEX["initialdens"]] = p[3, INDEX["initialdens"]]      # a tie: index order kept
+    perm = schedule_order(torch.tensor(p), "densest").numpy()
+    assert sorted(perm.tolist()) == list(range(64))
+    d = p[perm, INDEX["initialdens"]]
+    assert np.all(d[:-1] >= d[1:])
+    assert list(perm).index(3) < list(perm).index(10)
+
+
+def test_explicit_order_checked():
+    from uclchem_amd.gpu import schedule_order
+    p = torch.zeros((5, 24), dtype=torch.float64)
+    assert schedule_order(p, torch.tensor([4, 3, 2, 1, 0])).tolist() == [4, 3, 2, 1, 0]
+    for bad in ([0, 1, 2, 3], [0, 1, 2, 3, 3], [0, 1, 2, 3, 5]):
+        with pytest.raises(ValueError):
+            schedule_order(p, torch.tensor(bad))
+    with pytest.raises(ValueError):
+        schedule_order(p, "random")
+
+
+def test_ordered_solve_scatters_outputs_back(monkeypatch):
+    """The gather/launch/scatter plumbing of solve_device(order=...), on host tensors with a
+    stand-in for the library entry point: the stand-in writes, for each position k of the
+    (permuted) batch, values that depend only on that point's parameters, plus the position
+    itself as its start stamp; after the scatter every row must be the one of its own point
+    and the start stamps must follow the requested order."""
+    import ctypes
+    from uclchem_amd import gpu
+    from uclchem_amd.params import INDEX
+    n, neq, nprm = 6, 4, 24
+    seen = {}
+
+    def arr(ptr, count, ctype):
+        return np.ctypeslib.as_array((ctype * count).from_address(ptr.value))
+
+    class Stub:
+        def ucg_solve_batch_device(self, h, nn, p, yi, yo, mr, rec, nrec, st, stats, t0, t1, bits, slots, stream):
+            k = nn.value
+            prm = arr(p, k * nprm, ctypes.c_double).reshape(k, nprm)
+            y = arr(yo, k * neq, ctypes.c_double).reshape(k, neq)
+            y[:] = prm[:, INDEX["initialdens"], None] * np.arange(1, neq + 1)
+            arr(st, k, ctypes.c_int)[:] = (prm[:, INDEX["initialdens"]] % 7).astype(np.int32)
+            arr(t0, k, ctypes.c_double)[:] = np.arange(k)
+            seen["dens"] = prm[:, INDEX["initialdens"]].copy()
+            return 0
+
+    monkeypatch.setattr(gpu, "_lib", Stub())
+    net = gpu.Network.__new__(gpu.Network)
+    net.handle, net.neq = 0, neq
+    p = torch.zeros((n, nprm), dtype=torch.float64)
+    p[:, INDEX["initialdens"]] = torch.tensor([3.0, 50.0, 7.0, 900.0, 1.0, 20.0])
+    y = torch.zeros((n, neq), dtype=torch.float64)
+    st = torch.zeros(n, dtype=torch.int32)
+    stats = torch.zeros((n, 8), dtype=torch.int32)
+    t0 = torch.zeros(n, dtype=torch.float64)
+    net.solve_device(p, y, st, stats, t_start=t0, order="densest")
+    dens = p[:, INDEX["initialdens"]].numpy()
+    assert seen["dens"].tolist() == sorted(dens.tolist(), reverse=True)
+    assert np.array_equal(y.numpy(), dens[:, None] * np.arange(1, neq + 1))
+    assert np.array_equal(st.numpy(), (dens % 7).astype(np.int32))
+    assert t0.numpy().tolist() == [4, 1, 3, 0, 5, 2]          # rank of each point in densest-first order
+    t0.zero_()
+    net.solve_device(p, y, st, stats, t_start=t0, order=torch.tensor([5, 4, 3, 2, 1, 0]))
+    assert t0.numpy().tolist() == [5, 4, 3, 2, 1, 0]

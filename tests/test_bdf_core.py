@@ -1,0 +1,152 @@
+"""Known-answer tests of the solver's BDF core on the CPU.
+
+uclchem_amd/csrc/bdf_core.h (the integrator the GPU kernel runs: DVODE's method family,
+src/fortran_src/dvode.f90 DVODE/DVSTEP/DVSET/DVJUST/DVNLSD/DVNORM) is compiled for the host by
+g++ with the dense Ops of tests/bdf_host.cpp and checked against answers the mathematics
+fixes, independent of the reference:
+* linear decay y' = -k y: the closed form exp(-k t);
+* Robertson's kinetics problem: published reference values at t = 40 (the IVP test set of
+  Mazzia & Magherini; scipy's Radau at rtol 1e-13 agrees to 1e-13), scipy Radau at tight
+  tolerance at t = 4e5 and 4e10, and the exact linear invariant y1 + y2 + y3 = 1;
+* HIRES: published reference values at t = 321.8122 (same test set);
+* tolerance proportionality (a 1e4 tighter rtol gives a much smaller global error), which a
+  wrong error-test constant or Nordsieck update breaks;
+* DVODE's return states: -1 at MXSTEP, -2 for a tolerance below machine precision, -3 for an
+  illegal call;
+* the reference's own DVODE_F90 (oracle/_ref, compiled from src/fortran_src/dvode.f90 and run
+  with UCLCHEM's options by oracle/ref_dvode_problems.f90) on the same problems: the same
+  answers within 1e-6, and on linear decay, where the Jacobian is constant and the two
+  integrators differ only in bookkeeping, the same number of steps within 5% — a wrong BDF
+  coefficient, error constant or order-change update shows up there as extra steps.
+"""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECAY, ROBER, HIRES = 0, 1, 2
+HIRES_T = 321.8122
+HIRES_Y0 = [1.0, 0, 0, 0, 0, 0, 0, 0.0057]
+HIRES_REF = np.array([7.371312573325668e-4, 1.442485726316185e-4, 5.888729740967575e-5, 1.175651343283149e-3,
+                      2.386356198831331e-3, 6.238968252742796e-3, 2.849998395185769e-3, 2.850001604814231e-3])
+ROBER_40 = np.array([0.7158270687193685, 9.185534764557630e-06, 0.2841637457458670])
+
+
+@pytest.fixture(scope="module")
+def bdf(tmp_path_factory):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    out = str(tmp_path_factory.mktemp("bdf_host") / "libbdfhost.so")
+    subprocess.check_call([cxx, "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out,
+                           os.path.join(HERE, "bdf_host.cpp")])
+    lib = ctypes.CDLL(out)
+
+    def run(problem, y0, tout, rtol, atol, mxstep=10000, t0=0.0):
+        y = np.array(y0, dtype=np.float64)
+        at = np.full(len(y), atol, dtype=np.float64)
+        st = np.zeros(7, dtype=np.int32)
+        assert lib.bdf_host_n(problem) == len(y)
+        r = lib.bdf_host_solve(problem, ctypes.c_double(t0), ctypes.c_double(tout), ctypes.c_double(rtol),
+                               at.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p),
+                               mxstep, st.ctypes.data_as(ctypes.c_void_p))
+        return r, y, dict(zip(["nst", "nfe", "nje", "nsetups", "nni", "ncfn", "netf"], st.tolist()))
+    return run
+
+
+def _rober(t, y):
+    return [-0.04 * y[0] + 1e4 * y[1] * y[2], 0.04 * y[0] - 1e4 * y[1] * y[2] - 3e7 * y[1] ** 2, 3e7 * y[1] ** 2]
+
+
+def test_linear_decay_matches_closed_form(bdf):
+    k = np.array([1.0, 1e3, 1e6])
+    for t, check in ((1e-3, [0, 1]), (1.0, [0])):
+        r, y, st = bdf(DECAY, [1.0, 1.0, 1.0], t, 1e-8, 1e-20)
+        assert r == 2
+        exact = np.exp(-k * t)
+        assert np.all(np.abs(y[check] - exact[check]) <= 1e-6 * exact[check])
+        assert np.all(np.abs(y) <= np.maximum(2 * exact, 1e-19))    # decayed components stay at atol scale
+        assert st["nje"] == st["nsetups"] > 0 and st["nst"] > 0
+
+
+def test_robertson_reference_values_and_invariant(bdf):
+    from scipy.integrate import solve_ivp
+    r, y, st = bdf(ROBER, [1.0, 0.0, 0.0], 40.0, 1e-8, 1e-20)
+    assert r == 2
+    assert np.all(np.abs(y - ROBER_40) <= 1e-6 * ROBER_40)
+    assert abs(y.sum() - 1.0) < 1e-12
+    for t in (4e5, 4e10):
+        r, y, st = bdf(ROBER, [1.0, 0.0, 0.0], t, 1e-8, 1e-20)
+        ref = solve_ivp(_rober, (0.0, t), [1.0, 0.0, 0.0], method="Radau", rtol=1e-12, atol=1e-22).y[:, -1]
+        assert r == 2
+        assert np.all(np.abs(y - ref) <= 1e-6 * ref)
+        assert abs(y.sum() - 1.0) < 1e-12
+        assert st["nst"] < 5000                 # stiff problem over 10 decades: BDF keeps steps long
+
+
+def test_hires_reference_values(bdf):
+    r, y, st = bdf(HIRES, HIRES_Y0, HIRES_T, 1e-8, 1e-20)
+    assert r == 2
+    assert np.all(np.abs(y - HIRES_REF) <= 1e-6 * HIRES_REF)
+
+
+def test_tolerance_proportionality(bdf):
+    err = {}
+    for rtol in (1e-6, 1e-10):
+        r, y, st = bdf(HIRES, HIRES_Y0, HIRES_T, rtol, rtol * 1e-4)
+        assert r == 2
+        err[rtol] = np.max(np.abs(y - HIRES_REF) / HIRES_REF)
+    assert err[1e-10] < err[1e-6] / 100
+
+
+def test_dvode_return_states(bdf):
+    assert bdf(HIRES, HIRES_Y0, HIRES_T, 1e-8, 1e-20, mxstep=50)[0] == -1
+    assert bdf(HIRES, HIRES_Y0, HIRES_T, 1e-30, 1e-300)[0] == -2
+    assert bdf(HIRES, HIRES_Y0, 0.0, 1e-8, 1e-20)[0] == -3      # tout not after t
+    assert bdf(HIRES, HIRES_Y0, 1.0, 0.0, 0.0)[0] == -3         # zero error weights
+
+
+@pytest.fixture(scope="module")
+def dvode():
+    path = os.path.join(HERE, "..", "oracle", "_ref", "libucref.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    lib = ctypes.CDLL(path)
+    if not hasattr(lib, "ucref_dvode_problem"):
+        pytest.skip("oracle/_ref predates ucref_dvode_problem")
+
+    def run(problem, y0, tout, rtol, atol, mxstep=10000):
+        y = np.array(y0, dtype=np.float64)
+        at = np.full(len(y), atol, dtype=np.float64)
+        ist = ctypes.c_int(0)
+        st = np.zeros(3, dtype=np.int32)
+        lib.ucref_dvode_problem(problem, ctypes.c_double(0.0), ctypes.c_double(tout), ctypes.c_double(rtol),
+                                at.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p), len(y),
+                                mxstep, ctypes.byref(ist), st.ctypes.data_as(ctypes.c_void_p))
+        return ist.value, y, dict(zip(["nst", "nfe", "nje"], st.tolist()))
+    return run
+
+
+def test_same_answers_as_reference_dvode(bdf, dvode):
+    cases = [(ROBER, [1.0, 0.0, 0.0], 40.0), (ROBER, [1.0, 0.0, 0.0], 4e10), (HIRES, HIRES_Y0, HIRES_T)]
+    for problem, y0, t in cases:
+        r_ref, y_ref, st_ref = dvode(problem, y0, t, 1e-8, 1e-20, mxstep=100000)
+        r, y, st = bdf(problem, y0, t, 1e-8, 1e-20, mxstep=100000)
+        assert r_ref == 2 and r == 2
+        assert np.all(np.abs(y - y_ref) <= 1e-6 * np.abs(y_ref)), (problem, t)
+        assert st["nst"] <= 1.1 * st_ref["nst"], (problem, t, st["nst"], st_ref["nst"])
+    r_ref, y_ref, _ = dvode(ROBER, [1.0, 0.0, 0.0], 40.0, 1e-8, 1e-20)
+    assert np.all(np.abs(y_ref - ROBER_40) <= 1e-6 * ROBER_40)        # the oracle itself on the known answer
+
+
+def test_step_count_matches_reference_dvode_on_linear_decay(bdf, dvode):
+    for t in (1.0, 100.0):
+        r_ref, y_ref, st_ref = dvode(DECAY, [1.0, 1.0, 1.0], t, 1e-8, 1e-20)
+        r, y, st = bdf(DECAY, [1.0, 1.0, 1.0], t, 1e-8, 1e-20)
+        assert r_ref == 2 and r == 2
+        assert 0.95 * st_ref["nst"] <= st["nst"] <= 1.05 * st_ref["nst"], (t, st["nst"], st_ref["nst"])
+    assert abs(y[0] - np.exp(-100.0)) <= 1e-6 * np.exp(-100.0) or abs(y[0]) <= 1e-19

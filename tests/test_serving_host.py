@@ -1,0 +1,50 @@
+"""Host-side logic of the serving loop (uclchem_amd/serving.py) on the CPU: the HBM-budget
+slot planner and the seeded Poisson request stream.  The per-slot byte count, which the
+library reports for a loaded network, is replaced by fixed sizes here."""
+import numpy as np
+import pytest
+
+from uclchem_amd import serving
+from uclchem_amd.gpu import GpuError
+from uclchem_amd.params import INDEX
+
+
+@pytest.fixture
+def fake_sizes(monkeypatch):
+    sizes = {"small": 100.0, "default": 300.0, "big": 1000.0}
+    monkeypatch.setattr(serving.RequestQueue, "slot_bytes", staticmethod(lambda net, lu_bits=64: sizes[net]))
+    return {k: k for k in sizes}          # "networks": the planner only passes them to slot_bytes
+
+
+def test_plan_slots_shares_cus_within_budget(fake_sizes):
+    slots = serving.plan_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1e9)
+    assert set(slots) == set(fake_sizes)
+    assert sum(slots.values()) <= 256                     # every queue's workgroups resident together
+    assert max(slots.values()) - min(slots.values()) <= 1  # equal weights: equal shares
+    w = serving.plan_slots(fake_sizes, 256, 1e9, weights={"small": 1.0, "default": 1.0, "big": 2.0})
+    assert w["big"] >= 2 * w["small"] - 1
+
+
+def test_plan_slots_trims_the_largest_consumer_to_fit(fake_sizes):
+    budget = 40_000.0
+    slots = serving.plan_slots(fake_sizes, n_cu=256, hbm_budget_bytes=budget)
+    used = sum(slots[n] * {"small": 100.0, "default": 300.0, "big": 1000.0}[n] for n in slots)
+    assert used <= budget and min(slots.values()) >= 1
+    tight = serving.plan_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1500.0)   # one slot each fits
+    assert min(tight.values()) >= 1
+    assert sum(tight[n] * {"small": 100.0, "default": 300.0, "big": 1000.0}[n] for n in tight) <= 1500.0
+    with pytest.raises(GpuError):
+        serving.plan_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1000.0)
+
+
+def test_poisson_requests_seeded_and_well_formed():
+    names = ["a", "b", "c"]
+    r1 = serving.poisson_requests(400, 20.0, names)
+    r2 = serving.poisson_requests(400, 20.0, names)
+    assert [(t, n) for t, n, _ in r1] == [(t, n) for t, n, _ in r2]
+    t = np.array([x[0] for x in r1])
+    assert t[0] == 0.0 and np.all(np.diff(t) >= 0)
+    assert abs(np.mean(np.diff(t)) - 1 / 20.0) < 0.25 / 20.0          # mean gap 1/rate
+    assert {x[1] for x in r1} == set(names)
+    t_end = np.array([x[2][INDEX["finaltime"]] for x in r1])
+    assert t_end.min() >= 1e3 and t_end.max() <= 1e7

@@ -133,7 +133,7 @@ def plan_slots(networks: Dict[str, Network], n_cu: int, hbm_budget_bytes: float,
     share = np.maximum(1, np.floor((n_cu - 4) * w / w.sum())).astype(int)
     per = np.array([RequestQueue.slot_bytes(networks[n], lu_bits) for n in names], dtype=float)
     while (share * per).sum() > hbm_budget_bytes and share.max() > 1:
-        share[np.argmax(share * per)] -= 1
+        share[int(np.argmax(np.where(share > 1, share * per, -1.0)))] -= 1   # never below one slot
     if (share * per).sum() > hbm_budget_bytes:
         raise GpuError("HBM budget too small for one slot per network")
     return {n: int(s) for n, s in zip(names, share)}

@@ -118,7 +118,8 @@ int ucg_network_free(int handle);
  *           convergence failures, error-test failures, intervals, integrator retries
  *   lu_bits 64: FP64 Newton-matrix factorisation; 32: FP32 factorisation with the FP64
  *           residual (every Newton iterate is re-evaluated in FP64).
- * Returns when all points are done. */
+ * Returns when all points are done.  It waits for the whole device, so while a request
+ * queue's kernel runs (ucg_queue_open) use ucg_solve_batch_device on a stream instead. */
 int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out,
                     int max_rec, double *rec, int *n_rec, int *status, int *stats, int lu_bits);
 

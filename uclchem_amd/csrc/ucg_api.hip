@@ -1169,6 +1169,7 @@ int ucg_queue_close(int queue)
     hipError_t e = hipStreamSynchronize(st);   // posted requests finish, idle workgroups leave
     std::lock_guard<std::mutex> lk(g_mu);
     Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "queue closed concurrently");   // another thread's close won
     q->drained = true;
     // Freeing device or pinned memory may wait for every kernel on the device, including
     // other queues' kernels that only end when those queues close: release drained queues

@@ -73,3 +73,45 @@ def test_ordered_solve_scatters_outputs_back(monkeypatch):
     t0.zero_()
     net.solve_device(p, y, st, stats, t_start=t0, order=torch.tensor([5, 4, 3, 2, 1, 0]))
     assert t0.numpy().tolist() == [5, 4, 3, 2, 1, 0]
+
+
+def test_host_solve_order_round_trip(monkeypatch):
+    """Network.solve(order="densest") hands the library the points densest first and returns
+    every output in index order (stand-in library as above)."""
+    import ctypes
+    from uclchem_amd import gpu
+    from uclchem_amd.params import INDEX
+    n, neq, nprm, max_rec, nspec = 5, 4, 24, 2, 3
+    seen = {}
+
+    def arr(ptr, count, ctype):
+        return np.ctypeslib.as_array((ctype * count).from_address(ptr.value))
+
+    class Stub:
+        def ucg_solve_batch(self, h, nn, p, yi, yo, mr, rec, nrec, st, stats, bits):
+            k = nn.value
+            prm = arr(p, k * nprm, ctypes.c_double).reshape(k, nprm)
+            d = prm[:, INDEX["initialdens"]]
+            arr(yo, k * neq, ctypes.c_double).reshape(k, neq)[:] = d[:, None] + np.arange(neq)
+            arr(rec, k * max_rec * (4 + nspec), ctypes.c_double).reshape(k, max_rec, 4 + nspec)[:] = d[:, None, None]
+            arr(nrec, k, ctypes.c_int)[:] = (d % 3).astype(np.int32)
+            arr(st, k, ctypes.c_int)[:] = np.arange(k)          # position in the launch
+            arr(stats, k * 8, ctypes.c_int).reshape(k, 8)[:] = (d[:, None] % 11).astype(np.int32)
+            seen["dens"] = d.copy()
+            return 0
+
+    monkeypatch.setattr(gpu, "_lib", Stub())
+    net = gpu.Network.__new__(gpu.Network)
+    net.handle, net.neq, net.nspec = 0, neq, nspec
+    p = np.zeros((n, nprm))
+    dens = np.array([5.0, 400.0, 30.0, 2.0, 30.0])
+    p[:, INDEX["initialdens"]] = dens
+    out = net.solve(p, max_rec=max_rec, order="densest")
+    assert seen["dens"].tolist() == [400.0, 30.0, 30.0, 5.0, 2.0]
+    assert np.array_equal(out["y"], dens[:, None] + np.arange(neq))
+    assert np.all(out["rec"] == dens[:, None, None])
+    assert np.array_equal(out["n_rec"], (dens % 3).astype(np.int32))
+    assert out["status"].tolist() == [3, 0, 1, 4, 2]       # rank in densest-first order, ties by index
+    assert np.array_equal(out["stats"], np.repeat((dens[:, None] % 11).astype(np.int32), 8, axis=1))
+    with pytest.raises(ValueError):
+        net.solve(p, order="random")

@@ -124,7 +124,10 @@ class Network:
 
     # ------------------------------------------------------------ full solves
     def solve(self, params: np.ndarray, y_init: Optional[np.ndarray] = None, max_rec: int = 0,
-              lu_bits: int = 64):
+              lu_bits: int = 64, order=None):
+        """Host-array solve (ucg_solve_batch).  order: None (index order) or "densest"
+        (points taken highest initial density first, as solve_device); results are returned
+        in index order and are the same either way."""
         params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
         n = params.shape[0]
         if params.shape[1] != NPARAM:
@@ -134,6 +137,14 @@ class Network:
             y0 = np.ascontiguousarray(np.atleast_2d(y_init), dtype=np.float64)
             if y0.shape != (n, self.neq):
                 raise ValueError("y_init must be [n, neq]")
+        if order is not None:
+            if order != "densest":
+                raise ValueError(f"unknown order {order!r} (expected 'densest')")
+            perm = np.argsort(-params[:, INDEX["initialdens"]], kind="stable")
+            out = self.solve(params[perm], None if y0 is None else y0[perm], max_rec=max_rec, lu_bits=lu_bits)
+            back = np.empty_like(perm)
+            back[perm] = np.arange(n)
+            return {k: (None if v is None else v[back]) for k, v in out.items()}
         y_out = np.zeros((n, self.neq))
         rec = np.zeros((n, max_rec, 4 + self.nspec)) if max_rec > 0 else None
         n_rec = np.zeros(n, dtype=np.int32)

@@ -3,8 +3,9 @@
 The reference runs one grid point per call: ``uclchem.model.cloud`` / ``hot_core``
 (``src/uclchem/model.py:277-316``) call ``uclchemwrap.cloud`` / ``hot_core``
 (``src/fortran_src/wrap.f90:15-217``).  The functions here take a list of parameter
-dictionaries (the same case-insensitive keys) and solve all of them in one launch,
-returning results in the reference's own layouts:
+dictionaries (the same case-insensitive keys) and solve all of them in one launch (the
+densest points taken first, which shortens heavy-tailed grids), returning results in the
+reference's own layouts:
 
 * ``return_array=False``: per point, ``[success_flag] + abundances of out_species``
   (``_format_output``, ``model.py:71-76``);
@@ -76,7 +77,7 @@ def _run(param_dicts: Iterable[Dict], out_species, return_array, starting_chemis
         if sc.shape[0] == 1 and params.shape[0] > 1:
             sc = np.repeat(sc, params.shape[0], axis=0)
         y0 = np.concatenate([sc, params[:, INDEX["initialdens"]][:, None]], axis=1)
-    out = net.solve(params, y_init=y0, max_rec=timepoints + 1, lu_bits=lu_bits)
+    out = net.solve(params, y_init=y0, max_rec=timepoints + 1, lu_bits=lu_bits, order="densest")
     flags = out["status"].astype(int)
     if return_array:
         phys, chem, start = _arrays(net, params, out, timepoints)

@@ -175,6 +175,35 @@ CONTAINS
         ucref_rhs = 0
     END FUNCTION ucref_rhs
 
+    ! One DVODE_F90 call of the chemistry (integrateODESystem, chemistry.f90:241-292) on the
+    ! given state from t0 to t1 seconds, with the module's current rates (ucref_rates) and
+    ! tolerances: y in/out, istate_out = DVODE's ISTATE, stats = NST, NFE, NJE, NQU (order of
+    ! the last step), NNI (Newton iterations), NCFN (convergence failures) of the call.
+    INTEGER(c_int) FUNCTION ucref_integrate(y, t0, t1, istate_out, stats) BIND(C, name="ucref_integrate")
+        REAL(c_double), INTENT(INOUT) :: y(nspec+1)
+        REAL(c_double), VALUE :: t0, t1
+        INTEGER(c_int), INTENT(OUT) :: istate_out
+        INTEGER(c_int), INTENT(OUT) :: stats(6)
+        INTEGER :: flag
+        REAL(dp) :: rstats(22)
+        INTEGER :: istats(31)
+        dstep = 1
+        abund(:,1) = y
+        currentTime = t0
+        targetTime = t1
+        CALL integrateODESystem(flag)
+        y = abund(:,1)
+        istate_out = ISTATE
+        CALL GET_STATS(rstats, istats)
+        stats(1) = istats(11)
+        stats(2) = istats(12)
+        stats(3) = istats(13)
+        stats(4) = istats(14)
+        stats(5) = istats(20)
+        stats(6) = istats(21)
+        ucref_integrate = flag
+    END FUNCTION ucref_integrate
+
     ! Overwrite the module rate array (so RHS tests can use arbitrary rates).
     INTEGER(c_int) FUNCTION ucref_set_rates(rate_in) BIND(C, name="ucref_set_rates")
         REAL(c_double), INTENT(IN) :: rate_in(nreac)

@@ -5,6 +5,7 @@
 // HostOps mirrors the GPU policy's vector semantics one for one (Nordsieck history z0..z5,
 // acor, ewt, WRMS norms over all equations, residual gamma f - (rl1 z1 + acor), correction
 // y = z0 + acor) with a dense Newton matrix I - gamma J factored by LU with partial pivoting.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -12,8 +13,6 @@
 #include "../uclchem_amd/csrc/bdf_core.h"
 
 namespace {
-
-constexpr int NMAX = 16;
 
 struct Problem {
     int n;
@@ -77,11 +76,22 @@ const Problem PROBLEMS[] = {{3, decay_f, decay_j}, {3, rober_f, rober_j}, {8, hi
 
 struct HostOps {
     int n;
-    const Problem *P;
-    double z[6][NMAX], acor[NMAX], ewt[NMAX], atol[NMAX], f0[NMAX], y[NMAX], f[NMAX], b[NMAX];
-    double M[NMAX * NMAX], J[NMAX * NMAX];
-    int piv[NMAX];
+    const Problem *P = nullptr;
+    int (*cb)(double *y, double *ydot) = nullptr;   // external RHS; its Jacobian by differences
+    std::vector<double> zs, acor, ewt, atol, f0, y, f, b, M, J, yp, fp;
+    std::vector<int> piv;
+    double *z[6];
 
+    explicit HostOps(int n_) : n(n_), zs(6 * n_), acor(n_), ewt(n_), atol(n_), f0(n_), y(n_), f(n_), b(n_),
+                               M((size_t)n_ * n_), J((size_t)n_ * n_), yp(n_), fp(n_), piv(n_)
+    {
+        for (int j = 0; j < 6; j++) z[j] = zs.data() + (size_t)j * n_;
+    }
+    void eval(const double *yy, double *ff)
+    {
+        if (cb) cb(const_cast<double *>(yy), ff);
+        else P->f(yy, ff);
+    }
     double zget(int v, int i) const { return v == bdf::ACOR ? acor[i] : z[v][i]; }
     double wrms(int v) const
     {
@@ -118,8 +128,8 @@ struct HostOps {
             y[i] = z[0][i];
         }
     }
-    void rhs() { P->f(y, f); }
-    void save_f0() { std::memcpy(f0, f, sizeof(double) * n); }
+    void rhs() { eval(y.data(), f.data()); }
+    void save_f0() { f0 = f; }
     void scale_into_z_from(int j, double a, int src)
     {
         for (int i = 0; i < n; i++) z[j][i] = a * (src == bdf::F0 ? f0[i] : f[i]);
@@ -157,14 +167,34 @@ struct HostOps {
         for (int j = 0; j <= q; j++)
             for (int i = 0; i < n; i++) z[j][i] += l[j] * acor[i];
     }
-    void copy_acor_to_z(int j) { std::memcpy(z[j], acor, sizeof(double) * n); }
+    void copy_acor_to_z(int j) { std::copy(acor.begin(), acor.end(), z[j]); }
     void form_residual(double gamma, double rl1)
     {
         for (int i = 0; i < n; i++) b[i] = gamma * f[i] - (rl1 * z[1][i] + acor[i]);
     }
+    // Jacobian by forward differences with DVODE's increments (dvode.f90:8336-8356), the
+    // gamma of the setup standing in for the step size in the r0 floor
+    void fd_jacobian(double gamma)
+    {
+        const double srur = std::sqrt(bdf::UROUND);
+        double fac = 0.0;
+        for (int i = 0; i < n; i++) fac += (f[i] * ewt[i]) * (f[i] * ewt[i]);
+        fac = std::sqrt(fac / n);
+        double r0 = 1000.0 * std::fabs(gamma) * bdf::UROUND * n * fac;
+        if (r0 == 0.0) r0 = 1.0;
+        yp = y;
+        for (int j = 0; j < n; j++) {
+            const double r = std::fmax(srur * std::fabs(y[j]), r0 / ewt[j]);
+            yp[j] = y[j] + r;
+            eval(yp.data(), fp.data());
+            for (int i = 0; i < n; i++) J[(size_t)i * n + j] = (fp[i] - f[i]) / r;
+            yp[j] = y[j];
+        }
+    }
     int setup(double gamma, bool)
     {
-        P->jac(y, J);
+        if (cb) fd_jacobian(gamma);
+        else P->jac(y.data(), J.data());
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++) M[i * n + j] = (i == j ? 1.0 : 0.0) - gamma * J[i * n + j];
         for (int k = 0; k < n; k++) {   // LU with partial pivoting, row-major
@@ -216,7 +246,7 @@ struct HostOps {
     double ydd_norm(double, double hg)
     {
         for (int i = 0; i < n; i++) y[i] = z[0][i] + hg * f0[i];
-        P->f(y, f);
+        eval(y.data(), f.data());
         double a = 0.0;
         for (int i = 0; i < n; i++) {
             const double t = (f[i] - f0[i]) / hg * ewt[i];
@@ -236,21 +266,8 @@ struct HostOps {
     bool cancelled() const { return false; }
 };
 
-}  // namespace
-
-extern "C" {
-
-// Integrates problem `problem` (0 decay, 1 Robertson, 2 HIRES) from t0 to tout with one
-// cold-start integrate() call; y [n] in/out, atol [n]; stats [7] = nst nfe nje nsetups nni
-// ncfn netf.  Returns integrate()'s DVODE-style state (2 = success).
-int bdf_host_solve(int problem, double t0, double tout, double rtol, const double *atol, double *y, int mxstep,
-                   int *stats)
+int run(HostOps &o, double t0, double tout, double rtol, const double *atol, double *y, int mxstep, int *stats)
 {
-    if (problem < 0 || problem > 2) return -100;
-    HostOps o;
-    std::memset(&o, 0, sizeof(o));
-    o.P = &PROBLEMS[problem];
-    o.n = o.P->n;
     for (int i = 0; i < o.n; i++) {
         o.z[0][i] = y[i];
         o.atol[i] = atol[i];
@@ -262,6 +279,32 @@ int bdf_host_solve(int problem, double t0, double tout, double rtol, const doubl
     const int s[7] = {st.nst, st.nfe, st.nje, st.nsetups, st.nni, st.ncfn, st.netf};
     std::memcpy(stats, s, sizeof(s));
     return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Integrates problem `problem` (0 decay, 1 Robertson, 2 HIRES) from t0 to tout with one
+// cold-start integrate() call; y [n] in/out, atol [n]; stats [7] = nst nfe nje nsetups nni
+// ncfn netf.  Returns integrate()'s DVODE-style state (2 = success).
+int bdf_host_solve(int problem, double t0, double tout, double rtol, const double *atol, double *y, int mxstep,
+                   int *stats)
+{
+    if (problem < 0 || problem > 2) return -100;
+    HostOps o(PROBLEMS[problem].n);
+    o.P = &PROBLEMS[problem];
+    return run(o, t0, tout, rtol, atol, y, mxstep, stats);
+}
+
+// Same for an external RHS f(y, ydot) of n equations (autonomous), Jacobian by differences.
+int bdf_host_solve_fn(int n, int (*rhs)(double *, double *), double t0, double tout, double rtol,
+                      const double *atol, double *y, int mxstep, int *stats)
+{
+    if (n <= 0 || !rhs) return -100;
+    HostOps o(n);
+    o.cb = rhs;
+    return run(o, t0, tout, rtol, atol, y, mxstep, stats);
 }
 
 int bdf_host_n(int problem) { return (problem >= 0 && problem <= 2) ? PROBLEMS[problem].n : -1; }

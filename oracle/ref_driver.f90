@@ -26,7 +26,7 @@ MODULE ucref_driver
         & GAS_DUST_DENSITY_RATIO, bulkGainFromMantleBuildUp
     USE postprocess_mod, only: lusecoldens, usepostprocess
     USE cloud_mod, only: cloudInitializePhysics => initializePhysics, cloudUpdateTargetTime => updateTargetTime
-    USE DVODE_F90_M, only: GET_STATS
+    USE DVODE_F90_M, only: GET_STATS, VODE_OPTS, SET_OPTS, DVODE_F90
     IMPLICIT NONE
 
     ! hotcore.f90:20-22 tables (restated: hotcore.f90 itself does not compile with amdflang)
@@ -203,6 +203,42 @@ CONTAINS
         stats(6) = istats(21)
         ucref_integrate = flag
     END FUNCTION ucref_integrate
+
+    ! DVODE_F90 on the chemistry one step at a time (ITASK=2) from t0 until t1 is passed, with
+    ! the tolerances integrateODESystem sets (chemistry.f90:248-253): per step the time
+    ! reached, the step size used and its order (GET_STATS HU, NQU).  Returns the step count.
+    INTEGER(c_int) FUNCTION ucref_step_trace(y, t0, t1, maxsteps, tt, hh, qq) BIND(C, name="ucref_step_trace")
+        REAL(c_double), INTENT(INOUT) :: y(nspec+1)
+        REAL(c_double), VALUE :: t0, t1
+        INTEGER(c_int), VALUE :: maxsteps
+        REAL(c_double), INTENT(OUT) :: tt(maxsteps), hh(maxsteps)
+        INTEGER(c_int), INTENT(OUT) :: qq(maxsteps)
+        TYPE(VODE_OPTS) :: OPTS
+        REAL(dp) :: t, tout, rstats(22)
+        INTEGER :: istats(31), itask, ist, k
+        dstep = 1
+        abund(:,1) = y
+        abstol = abstol_factor*abund(:,1)
+        WHERE(abstol < abstol_min) abstol = abstol_min
+        OPTS = SET_OPTS(METHOD_FLAG=22, ABSERR_VECTOR=abstol, RELERR=reltol, USER_SUPPLIED_JACOBIAN=.FALSE., &
+            & MXSTEP=MXSTEP)
+        t = t0
+        tout = t1
+        itask = 2
+        ist = 1
+        k = 0
+        DO WHILE (t < t1 .AND. k < maxsteps)
+            CALL DVODE_F90(F, NEQ, abund(:,1), t, tout, itask, ist, OPTS)
+            IF (ist < 0) EXIT
+            CALL GET_STATS(rstats, istats)
+            k = k + 1
+            tt(k) = t
+            hh(k) = rstats(11)
+            qq(k) = istats(14)
+        END DO
+        y = abund(:,1)
+        ucref_step_trace = k
+    END FUNCTION ucref_step_trace
 
     ! Overwrite the module rate array (so RHS tests can use arbitrary rates).
     INTEGER(c_int) FUNCTION ucref_set_rates(rate_in) BIND(C, name="ucref_set_rates")

@@ -216,10 +216,11 @@ struct HostOps {
     bool can_reuse_jacobian() const { return false; }
     void solve()
     {
-        for (int k = 0; k < n; k++) {
-            std::swap(b[k], b[piv[k]]);
+        // the factorisation swapped whole rows (multipliers included), so every interchange
+        // applies to b before the forward substitution
+        for (int k = 0; k < n; k++) std::swap(b[k], b[piv[k]]);
+        for (int k = 0; k < n; k++)
             for (int i = k + 1; i < n; i++) b[i] -= M[i * n + k] * b[k];
-        }
         for (int k = n - 1; k >= 0; k--) {
             for (int j = k + 1; j < n; j++) b[k] -= M[k * n + j] * b[j];
             b[k] /= M[k * n + k];

@@ -162,13 +162,12 @@ def test_step_count_matches_reference_dvode_on_linear_decay(bdf, dvode):
 
 
 def test_chemistry_intervals_match_reference_dvode(bdf_lib):
-    """The core on UCLCHEM's own chemistry: the first seven output intervals of the
-    static-cloud example model (n_H = 1e4, T = 10 K; 0 -> 0.1 yr), each integrated from the
-    reference's state by one cold-start call, with the reference's F (ucref_rhs) and a
-    finite-difference Jacobian, against the reference's DVODE_F90 on the same interval
-    (ucref_integrate, chemistry.f90:241-292): abundances above 1e-14 within 1e-6 and step
-    counts within 25%.  Later intervals are not compared: there this host harness (finite-
-    difference Jacobian at every Newton-matrix setup) needs far more steps than DVODE."""
+    """The core on UCLCHEM's own chemistry: the output intervals of the static-cloud example
+    model (n_H = 1e4, T = 10 K) from 0 to 1000 yr, each integrated from the reference's state by
+    one cold-start call, with the reference's F (ucref_rhs) and a finite-difference Jacobian
+    evaluated at every Newton-matrix setup, against the reference's DVODE_F90 on the same
+    interval (ucref_integrate, chemistry.f90:241-292): abundances above 1e-14 within 1e-6, no
+    convergence failure, and step counts between 0.8 and 1.15 times DVODE's."""
     import sys
     sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
     from oracle import REF_SO, Oracle, RefLib, make_params
@@ -187,7 +186,7 @@ def test_chemistry_intervals_match_reference_dvode(bdf_lib):
     rc, y = ref.init(make_params(dens=1e4, temp=10.0, finaltime=1e6))
     assert rc == 0
     t_yr = 0.0
-    for k in range(7):
+    for k in range(27):
         t1 = orc.next_target(0, t_yr)
         t0 = t_yr * spy
         _, y, _ = ref.rates(y)                           # the interval's rates, as updateChemistry
@@ -199,7 +198,8 @@ def test_chemistry_intervals_match_reference_dvode(bdf_lib):
         assert ist.value == 2 and r == 2, k
         m = y_ref[:-1] > 1e-14
         assert np.all(np.abs(y_ours[:-1][m] - y_ref[:-1][m]) <= 1e-6 * y_ref[:-1][m]), k
-        assert 0.75 * st_ref[0] <= st[0] <= 1.25 * st_ref[0], (k, st[0], st_ref[0])
+        assert 0.8 * st_ref[0] <= st[0] <= 1.15 * st_ref[0], (k, st[0], st_ref[0])
+        assert st[5] == 0, (k, st[5])                    # no Newton convergence failure
         y = np.maximum(y_ref, 1e-30)                     # the clamp after each interval
         y[-1] = y_ref[-1]
         t_yr = t1 / spy

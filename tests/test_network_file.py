@@ -108,3 +108,42 @@ def test_fortran_caller(tables, netfile, gpu_net, tmp_path):
     params = np.stack([make_params(initialdens=10 ** (2 + 5 * q / (npts - 1)), finaltime=1e6) for q in range(npts)])
     y_p = gpu_net.solve(params)["y"]
     assert np.array_equal(y_f, y_p)
+
+
+def test_fortran_binding_covers_the_c_abi(tmp_path):
+    """Every batch, control and request-queue entry of include/uclchem_gpu.h has a Fortran
+    interface in uclchem_amd/fortran/uclchem_gpu.f90, and a caller using them compiles
+    (compile only: nothing runs without a GPU)."""
+    import shutil
+    fc = "/opt/rocm/bin/amdflang"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    bdir = os.path.join(root, "uclchem_amd", "fortran", "_build")
+    if not (os.path.exists(fc) and os.path.exists(os.path.join(bdir, "uclchem_gpu.mod"))):
+        pytest.skip("Fortran binding not built")
+    src = open(os.path.join(root, "uclchem_amd", "fortran", "uclchem_gpu.f90")).read()
+    for name in ["ucg_init", "ucg_network_load_file", "ucg_network_info", "ucg_solve_batch", "ucg_cancel",
+                 "ucg_queue_slot_bytes", "ucg_queue_open", "ucg_queue_submit", "ucg_queue_test",
+                 "ucg_queue_result", "ucg_queue_close", "ucg_last_error"]:
+        assert f'name="{name}"' in src, name
+    prog = tmp_path / "queue_caller.f90"
+    prog.write_text("""
+PROGRAM queue_caller
+    USE uclchem_gpu
+    USE, INTRINSIC :: ISO_C_BINDING
+    IMPLICIT NONE
+    INTEGER(C_INT) :: rc, q, r, done, st, stats(8)
+    INTEGER(C_SIZE_T) :: bytes
+    REAL(C_DOUBLE) :: params(UCG_NPARAM), y(336)
+    INTEGER :: handle
+    rc = ucg_network_load_file_f('default.ucgnet', handle)
+    CALL ucg_default_params(params)
+    rc = ucg_queue_slot_bytes(handle, 64, bytes)
+    rc = ucg_queue_open(handle, 16, 4, 64, 60.0d0, q)
+    rc = ucg_queue_submit(q, params, r)
+    rc = ucg_queue_test(q, r, done)
+    rc = ucg_queue_result(q, r, y, st, stats)
+    rc = ucg_queue_close(q)
+    rc = ucg_cancel(handle)
+END PROGRAM queue_caller
+""")
+    subprocess.check_call([fc, "-c", "-J", bdir, str(prog), "-o", str(tmp_path / "queue_caller.o")])

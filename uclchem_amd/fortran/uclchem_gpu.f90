@@ -15,7 +15,8 @@ MODULE uclchem_gpu
     IMPLICIT NONE
     PRIVATE
     PUBLIC :: ucg_init, ucg_network_load_file_f, ucg_network_info, ucg_solve_batch_f, ucg_error_message, &
-              ucg_default_params
+              ucg_default_params, ucg_cancel, ucg_queue_slot_bytes, ucg_queue_open, ucg_queue_submit, &
+              ucg_queue_test, ucg_queue_result, ucg_queue_close
     INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 24
     INTEGER, PARAMETER, PUBLIC :: UCGP_DENS = 1, UCGP_TEMP = 2, UCGP_ZETA = 3, UCGP_RADFIELD = 4, &
         UCGP_BASEAV = 5, UCGP_ROUT = 6, UCGP_RIN = 7, UCGP_FINALTIME = 8, UCGP_FREEFALL = 9, &
@@ -51,6 +52,54 @@ MODULE uclchem_gpu
             TYPE(C_PTR), VALUE :: rec
             INTEGER(C_INT), INTENT(OUT) :: n_rec(*), status(*), stats(*)
         END FUNCTION c_solve_batch
+
+        ! stop running device solves on a handle (points not finished report cancelled)
+        INTEGER(C_INT) FUNCTION ucg_cancel(handle) BIND(C, name="ucg_cancel")
+            IMPORT :: C_INT
+            INTEGER(C_INT), VALUE :: handle
+        END FUNCTION ucg_cancel
+
+        ! request queues: one persistent solve kernel per queue that takes single-point
+        ! requests posted while it runs (continuous batching); params(UCG_NPARAM) per request,
+        ! y(NEQ) and stats(8) per result
+        INTEGER(C_INT) FUNCTION ucg_queue_slot_bytes(handle, lu_bits, bytes) BIND(C, name="ucg_queue_slot_bytes")
+            IMPORT :: C_INT, C_SIZE_T
+            INTEGER(C_INT), VALUE :: handle, lu_bits
+            INTEGER(C_SIZE_T), INTENT(OUT) :: bytes
+        END FUNCTION ucg_queue_slot_bytes
+
+        INTEGER(C_INT) FUNCTION ucg_queue_open(handle, capacity, n_slots, lu_bits, idle_timeout_s, queue) &
+                BIND(C, name="ucg_queue_open")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), VALUE :: handle, capacity, n_slots, lu_bits
+            REAL(C_DOUBLE), VALUE :: idle_timeout_s
+            INTEGER(C_INT), INTENT(OUT) :: queue
+        END FUNCTION ucg_queue_open
+
+        INTEGER(C_INT) FUNCTION ucg_queue_submit(queue, params, request) BIND(C, name="ucg_queue_submit")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), VALUE :: queue
+            REAL(C_DOUBLE), INTENT(IN) :: params(*)
+            INTEGER(C_INT), INTENT(OUT) :: request
+        END FUNCTION ucg_queue_submit
+
+        INTEGER(C_INT) FUNCTION ucg_queue_test(queue, request, done) BIND(C, name="ucg_queue_test")
+            IMPORT :: C_INT
+            INTEGER(C_INT), VALUE :: queue, request
+            INTEGER(C_INT), INTENT(OUT) :: done
+        END FUNCTION ucg_queue_test
+
+        INTEGER(C_INT) FUNCTION ucg_queue_result(queue, request, y, status, stats) BIND(C, name="ucg_queue_result")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), VALUE :: queue, request
+            REAL(C_DOUBLE), INTENT(OUT) :: y(*)
+            INTEGER(C_INT), INTENT(OUT) :: status, stats(*)
+        END FUNCTION ucg_queue_result
+
+        INTEGER(C_INT) FUNCTION ucg_queue_close(queue) BIND(C, name="ucg_queue_close")
+            IMPORT :: C_INT
+            INTEGER(C_INT), VALUE :: queue
+        END FUNCTION ucg_queue_close
 
         TYPE(C_PTR) FUNCTION c_last_error() BIND(C, name="ucg_last_error")
             IMPORT :: C_PTR

@@ -114,7 +114,6 @@ def test_fortran_binding_covers_the_c_abi(tmp_path):
     """Every batch, control and request-queue entry of include/uclchem_gpu.h has a Fortran
     interface in uclchem_amd/fortran/uclchem_gpu.f90, and a caller using them compiles
     (compile only: nothing runs without a GPU)."""
-    import shutil
     fc = "/opt/rocm/bin/amdflang"
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     bdir = os.path.join(root, "uclchem_amd", "fortran", "_build")

@@ -32,7 +32,7 @@ import csv
 import dataclasses
 import os
 from collections import Counter
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, Sequence
 
 import numpy as np
 

@@ -1,0 +1,96 @@
+"""Host-side layout logic of the batched model API (uclchem_amd/model.py) on the CPU, with a
+stand-in for the resident network whose solve returns outputs that depend only on each
+point's parameters: the reference's physicsArray / chemicalAbunArray layouts (io.f90:74-82),
+the trimming to the last written row, abundanceStart, the out_species form (model.py:71-76)
+and the argument checks."""
+import numpy as np
+import pytest
+
+from uclchem_amd import model
+from uclchem_amd.params import INDEX
+
+NSPEC = 4
+
+
+class _Tables:
+    species_names = ["H", "H2", "CO", "#CO"]
+
+
+class _FakeNet:
+    nspec, neq = NSPEC, NSPEC + 1
+    t = _Tables()
+
+    def __init__(self):
+        self.calls = []
+
+    def solve(self, params, y_init=None, max_rec=0, lu_bits=64, order=None):
+        self.calls.append(dict(params=params.copy(), y_init=None if y_init is None else y_init.copy(),
+                               max_rec=max_rec, order=order))
+        n = params.shape[0]
+        dens = params[:, INDEX["initialdens"]]
+        n_rec = (2 + np.arange(n) % 3).astype(np.int32)          # 2, 3, 4, 2, ... rows written
+        rec = np.zeros((n, max_rec, 4 + NSPEC))
+        for q in range(n):
+            for r in range(n_rec[q]):
+                rec[q, r, 0] = 10.0 ** r                            # time
+                rec[q, r, 1] = dens[q]                              # density
+                rec[q, r, 2] = 10.0 + q                             # gasTemp
+                rec[q, r, 3] = 2.0 + q                              # Av
+                rec[q, r, 4:] = dens[q] * (r + 1) + np.arange(NSPEC)
+        y = np.zeros((n, NSPEC + 1))
+        for q in range(n):
+            y[q, :NSPEC] = rec[q, n_rec[q] - 1, 4:]
+            y[q, NSPEC] = dens[q]
+        status = np.where(dens > 1e6, -5, 0).astype(np.int32)
+        return dict(y=y, rec=rec, n_rec=n_rec, status=status, stats=np.zeros((n, 8), np.int32))
+
+
+@pytest.fixture
+def net(monkeypatch):
+    fake = _FakeNet()
+    monkeypatch.setattr(model, "network", lambda name="default": fake)
+    return fake
+
+
+def test_array_layout_matches_the_reference(net):
+    dicts = [{"initialDens": 1e3, "zeta": 2.0, "radfield": 5.0}, {"initialDens": 1e4}, {"initialDens": 1e5}]
+    phys, chem, rates, start, flags = model.cloud_grid(dicts, return_array=True, timepoints=10)
+    assert rates is None and list(flags) == [0, 0, 0]
+    assert net.calls[0]["max_rec"] == 11 and net.calls[0]["order"] == "densest"
+    rows = 4                                            # the most rows any point wrote
+    assert phys.shape == (rows, 3, 8) and chem.shape == (rows, 3, NSPEC)
+    n_rec = [2, 3, 4]
+    for q, d in enumerate(dicts):
+        k = n_rec[q]
+        assert np.array_equal(phys[:k, q, 0], 10.0 ** np.arange(k))         # time
+        assert np.all(phys[:k, q, 1] == d["initialDens"])                    # density
+        assert np.all(phys[:k, q, 2] == 10.0 + q) and np.all(phys[:k, q, 3] == 10.0 + q)   # gas = dust T
+        assert np.all(phys[:k, q, 4] == 2.0 + q)                             # Av
+        assert np.all(phys[:k, q, 5] == d.get("radfield", 1.0)) and np.all(phys[:k, q, 6] == d.get("zeta", 1.0))
+        assert np.all(phys[:k, q, 7] == 1)                                   # dstep
+        assert np.all(phys[k:, q] == 0) and np.all(chem[k:, q] == 0)         # rows never written stay zero
+        assert np.array_equal(start[q], chem[k - 1, q])                      # abundanceStart = last row
+
+
+def test_out_species_form_and_failed_points(net):
+    res = model.cloud_grid([{"initialDens": 1e4}, {"initialDens": 1e7}], out_species=["CO", "H"])
+    assert res[0] == [0, 1e4 * 2 + 2, 1e4 * 2 + 0]         # [flag] + final CO, H (point 0 wrote 2 rows)
+    assert res[1] == [-5]                                   # a failed point returns its flag only
+    assert model.cloud_grid([{"initialDens": 1e4}]) == [[0]]
+
+
+def test_starting_chemistry_and_hot_core_arguments(net):
+    sc = np.array([0.5, 0.25, 1e-4, 1e-6])
+    model.cloud_grid([{"initialDens": 1e3}, {"initialDens": 1e4}], starting_chemistry=sc)
+    y0 = net.calls[-1]["y_init"]
+    assert y0.shape == (2, NSPEC + 1) and np.all(y0[:, :NSPEC] == sc)
+    assert list(y0[:, NSPEC]) == [1e3, 1e4]                 # density appended as the last equation
+    with pytest.raises(ValueError):
+        model.cloud_grid([{}], starting_chemistry=np.zeros(3))
+    with pytest.raises(ValueError):
+        model.hot_core_grid(0, 300.0, [{}])
+    with pytest.raises(ValueError):
+        model.hot_core_grid(7, 300.0, [{}])
+    model.hot_core_grid(3, 250.0, [{"initialDens": 1e6}])
+    p = net.calls[-1]["params"][0]
+    assert (p[INDEX["model"]], p[INDEX["tempindx"]], p[INDEX["maxtemp"]]) == (1.0, 3.0, 250.0)

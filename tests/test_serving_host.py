@@ -48,3 +48,44 @@ def test_poisson_requests_seeded_and_well_formed():
     assert {x[1] for x in r1} == set(names)
     t_end = np.array([x[2][INDEX["finaltime"]] for x in r1])
     assert t_end.min() >= 1e3 and t_end.max() <= 1e7
+
+
+class _FakeQueue:
+    """Stands in for one device request queue: a request finishes `delay` seconds after it
+    is posted, with a result that depends only on its parameters."""
+
+    def __init__(self, net, capacity, n_slots, lu_bits=64, idle_timeout_s=600.0):
+        self.posted, self.open, self.delay = [], True, 0.02
+
+    def submit(self, params):
+        import time
+        self.posted.append((time.perf_counter(), np.array(params, dtype=float)))
+        return len(self.posted) - 1
+
+    def done(self, request):
+        import time
+        return time.perf_counter() - self.posted[request][0] >= self.delay
+
+    def result(self, request):
+        p = self.posted[request][1]
+        return np.full(3, p[INDEX["finaltime"]]), 0, np.zeros(8, np.int32)
+
+    def close(self):
+        self.open = False
+
+
+def test_stream_replay_returns_every_request_with_its_own_result(monkeypatch):
+    monkeypatch.setattr(serving, "RequestQueue", _FakeQueue)
+    names = ["small", "default", "big"]
+    srv = serving.Server({n: n for n in names}, {n: 4 for n in names}, capacity=64)
+    reqs = serving.poisson_requests(24, 400.0, names)
+    done = serving.run_stream(srv, reqs, timeout_s=30.0)
+    srv.close()
+    assert len(done) == 24 and all(not q.open for q in srv.queues.values())
+    by_net = {n: [r for r in reqs if r[1] == n] for n in names}
+    for t in done:
+        assert t.t_done - t.t_arrival >= 0.02 - 1e-3         # latency counts from arrival
+        sent = by_net[t.network][t.request][2]               # request ids count per queue
+        assert np.all(t.y == sent[INDEX["finaltime"]]) and t.status == 0
+    assert sorted((t.network, t.request) for t in done) == sorted(
+        (n, i) for n in names for i in range(len(by_net[n])))

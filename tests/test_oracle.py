@@ -182,6 +182,28 @@ def test_schedule_matches_example_output(oracle):
     assert np.allclose(times, g["times"], rtol=1e-12)
 
 
+def test_hot_core_schedule_and_temperature_match_example_output(oracle):
+    """The oracle's hot-core output schedule and temperature ramp (hotcore.f90:56-90;
+    temp_indx 3, 10 -> 300 K) against phase2-full.dat's time and gasTemp columns
+    (examples/example-output, 283 rows, printed to 4-5 significant digits)."""
+    from oracle import make_params
+    g = np.load(os.path.join(GOLD, "example_collapse_hotcore.npz"))
+    pt, _ = oracle.init_point(make_params(dens=1e5, temp=10.0, finaltime=1e6, model=1, tempindx=3, maxtemp=300.0,
+                                          freezefactor=0.0))
+    t, times, temps = 0.0, [0.0], [pt.gas_temp]
+    while t < 1e6:
+        t1 = oracle.next_target(1, t)
+        oracle.update_physics(pt, t1)
+        t = t1 / 3.16e7                       # SECONDS_PER_YEAR of constants.f90
+        times.append(t)
+        temps.append(pt.gas_temp)
+    ref_t, ref_T = g["phase2_times"], g["phase2_physics"][:, 2]
+    assert len(times) == len(ref_t)
+    assert np.all(np.abs(np.array(times) - ref_t) <= 2e-4 * np.maximum(ref_t, 1e-300))
+    assert np.all(np.abs(np.array(temps) - ref_T) <= 1e-3 * ref_T)
+    assert temps[-1] == 300.0 and min(temps) == 10.0
+
+
 def test_oracle_planted_mistake_is_caught(reflib, oracle, tables):
     """A dropped term in the oracle's RHS is detected by the bit-exact comparison."""
     from oracle import make_params

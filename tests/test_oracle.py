@@ -204,6 +204,39 @@ def test_hot_core_schedule_and_temperature_match_example_output(oracle):
     assert temps[-1] == 300.0 and min(temps) == 10.0
 
 
+def test_free_fall_density_matches_example_output(oracle):
+    """The oracle's free-fall density derivative (physics-core.f90:90-103, the last ODE row)
+    integrated alone (classical RK4, 50 substeps per output interval) reproduces
+    phase1-full.dat's density column (1e2 -> 1e5 cm^-3, printed to 5 significant digits) up
+    to the row where the reference stops at finalDens."""
+    from oracle import make_params
+    g = np.load(os.path.join(GOLD, "example_collapse_hotcore.npz"))
+    pt, y0 = oracle.init_point(make_params(dens=1e2, temp=10.0, freefall=1, endatfinaldensity=1, finaldens=1e5,
+                                           finaltime=5e6))
+    rate, y0, _ = oracle.rates(pt, y0)
+
+    def dndt(n):
+        y = y0.copy()
+        y[-1] = n
+        return oracle.rhs(pt, rate, y)[0][-1]
+    times = g["phase1_times"] * 3.16e7
+    n, t, out = y0[-1], 0.0, [y0[-1]]
+    for t_next in times[1:]:
+        h = (t_next - t) / 50
+        for _ in range(50):
+            k1 = dndt(n)
+            k2 = dndt(n + 0.5 * h * k1)
+            k3 = dndt(n + 0.5 * h * k2)
+            k4 = dndt(n + h * k3)
+            n += h * (k1 + 2 * k2 + 2 * k3 + k4) / 6
+        t = t_next
+        out.append(n)
+    ref = g["phase1_physics"][:, 1]
+    out = np.array(out)
+    assert np.all(np.abs(out[:-1] - ref[:-1]) <= 1e-4 * ref[:-1])
+    assert ref[-1] == 1e5 and out[-1] >= 1e5 and out[-2] < 1e5     # crosses finalDens in the last interval
+
+
 def test_oracle_planted_mistake_is_caught(reflib, oracle, tables):
     """A dropped term in the oracle's RHS is detected by the bit-exact comparison."""
     from oracle import make_params

@@ -33,3 +33,11 @@ def test_algorithmic_flops_from_counters():
     stats[:, 2] = [1, 2, 3]         # Newton-matrix setups
     stats[:, 3] = [4, 5, 6]         # Newton iterations
     assert bench.algorithmic_flops(stats, 100.0, 10.0, 1.0) == 6 * 100 + 15 * 10 + 60 * 1
+
+
+def test_committed_traffic_summary_parses():
+    """bench.py reads the HBM bytes per launch from the committed counter summary."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(root, "profiles", "solve_points_kernel_grid4096_traffic.txt")
+    vals = [float(ln.split()[1]) for ln in open(path) if ln.startswith("traffic_bytes_per_launch")]
+    assert len(vals) == 1 and 1e12 < vals[0] < 1e15

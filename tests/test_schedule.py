@@ -115,3 +115,40 @@ def test_host_solve_order_round_trip(monkeypatch):
     assert np.array_equal(out["stats"], np.repeat((dens[:, None] % 11).astype(np.int32), 8, axis=1))
     with pytest.raises(ValueError):
         net.solve(p, order="random")
+
+
+def test_device_solve_refuses_mismatched_buffers(monkeypatch):
+    """solve_device checks shapes, dtypes, layout and device before it hands pointers to the
+    kernel (which indexes them by point): nothing reaches the library on a mismatch."""
+    from uclchem_amd import gpu
+    calls = []
+
+    class Stub:
+        def ucg_solve_batch_device(self, *a):
+            calls.append(a)
+            return 0
+
+    monkeypatch.setattr(gpu, "_lib", Stub())
+    net = gpu.Network.__new__(gpu.Network)
+    net.handle, net.neq, net.nspec = 0, 5, 4
+    n = 3
+    good = dict(params=torch.zeros((n, 24), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
+                status=torch.zeros(n, dtype=torch.int32), stats=torch.zeros((n, 8), dtype=torch.int32))
+    net.solve_device(**good)
+    assert len(calls) == 1
+    bad = [("params", torch.zeros((n, 24), dtype=torch.float32)), ("params", torch.zeros((n, 23), dtype=torch.float64)),
+           ("y_out", torch.zeros((n, 4), dtype=torch.float64)), ("status", torch.zeros(n, dtype=torch.int64)),
+           ("stats", torch.zeros((n, 8), dtype=torch.int32).t().contiguous().t()),
+           ("stats", torch.zeros((n + 1, 8), dtype=torch.int32))]
+    for key, val in bad:
+        args = dict(good)
+        args[key] = val
+        with pytest.raises(ValueError):
+            net.solve_device(**args)
+    with pytest.raises(ValueError):
+        net.solve_device(**good, t_done=torch.zeros(n, dtype=torch.float32))
+    with pytest.raises(ValueError):
+        net.solve_device(**good, rec=torch.zeros((n, 2, 8), dtype=torch.float64), max_rec=2)   # n_rec missing
+    net.solve_device(**good, rec=torch.zeros((n, 2, 8), dtype=torch.float64), max_rec=2,
+                     n_rec=torch.zeros(n, dtype=torch.int32))
+    assert len(calls) == 2

@@ -180,6 +180,7 @@ class Network:
         def ptr(t):
             return ctypes.c_void_p(t.data_ptr()) if t is not None else None
         n = params.shape[0]
+        self._check_device_args(params, y_out, status, stats, y_init, rec, n_rec, max_rec, t_start, t_done)
         if order is not None:
             import contextlib
             import torch
@@ -203,6 +204,28 @@ class Network:
                                           ptr(stats), ptr(t_start), ptr(t_done), ctypes.c_int(lu_bits),
                                           ctypes.c_int(n_slots), ctypes.c_void_p(stream or 0))
         _check(rc, "ucg_solve_batch_device")
+
+    def _check_device_args(self, params, y_out, status, stats, y_init, rec, n_rec, max_rec, t_start, t_done):
+        """The kernel indexes these buffers by point with the sizes below; a wrong shape,
+        dtype or layout would make it read or write outside them, so refuse it here."""
+        import torch
+        if params.dim() != 2 or params.shape[1] != NPARAM:
+            raise ValueError(f"params must be [n, {NPARAM}]")
+        n = params.shape[0]
+        want = [("params", params, (n, NPARAM), torch.float64), ("y_out", y_out, (n, self.neq), torch.float64),
+                ("status", status, (n,), torch.int32), ("stats", stats, (n, 8), torch.int32),
+                ("y_init", y_init, (n, self.neq), torch.float64), ("t_start", t_start, (n,), torch.float64),
+                ("t_done", t_done, (n,), torch.float64)]
+        if rec is not None:
+            want += [("rec", rec, (n, max_rec, 4 + self.nspec), torch.float64), ("n_rec", n_rec, (n,), torch.int32)]
+        for name, t, shape, dtype in want:
+            if t is None:
+                if name in ("params", "y_out", "status", "stats", "n_rec"):
+                    raise ValueError(f"{name} is required")
+                continue
+            if tuple(t.shape) != shape or t.dtype != dtype or not t.is_contiguous() or t.device != params.device:
+                raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {shape} on {params.device}, "
+                                 f"got {t.dtype} {tuple(t.shape)} on {t.device}")
 
     # ------------------------------------------------------------ pieces
     def rates(self, params, phys, y):

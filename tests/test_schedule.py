@@ -152,3 +152,19 @@ def test_device_solve_refuses_mismatched_buffers(monkeypatch):
     net.solve_device(**good, rec=torch.zeros((n, 2, 8), dtype=torch.float64), max_rec=2,
                      n_rec=torch.zeros(n, dtype=torch.int32))
     assert len(calls) == 2
+
+
+def test_piece_wrappers_check_widths(monkeypatch):
+    """The rates / RHS / Jacobian / Newton-solve wrappers refuse arrays whose width differs
+    from what the library reads per point (NPARAM, 8 physics values, nreac, neq)."""
+    from uclchem_amd import gpu
+    monkeypatch.setattr(gpu, "_lib", object())        # any call into the library would fail loudly
+    net = gpu.Network.__new__(gpu.Network)
+    net.handle, net.neq, net.nspec, net.nreac = 0, 5, 4, 7
+    p, ph, y, r = np.zeros((2, 24)), np.zeros((2, 8)), np.zeros((2, 5)), np.zeros((2, 7))
+    for call in (lambda: net.rates(np.zeros((2, 23)), ph, y), lambda: net.rates(p, np.zeros((2, 7)), y),
+                 lambda: net.rates(p, ph, np.zeros((2, 4))), lambda: net.rhs(p, ph, np.zeros((2, 6)), y),
+                 lambda: net.jacobian(p, ph, r, np.zeros((3, 5))),
+                 lambda: net.newton_solve(p, ph, y, 1.0, np.zeros((2, 6)))):
+        with pytest.raises(ValueError):
+            call()

@@ -78,6 +78,14 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
+def _rows(a, n: int, width: int, name: str) -> np.ndarray:
+    """[n, width] float64 C-contiguous copy of `a`; the library reads exactly n*width values."""
+    a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64)
+    if a.shape != (n, width):
+        raise ValueError(f"{name} must be [{n}, {width}], got {list(a.shape)}")
+    return a
+
+
 class Network:
     """A network resident on the GPU (tables uploaded once, immutable)."""
 
@@ -128,15 +136,10 @@ class Network:
         """Host-array solve (ucg_solve_batch).  order: None (index order) or "densest"
         (points taken highest initial density first, as solve_device); results are returned
         in index order and are the same either way."""
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
-        if params.shape[1] != NPARAM:
-            raise ValueError(f"params must be [n, {NPARAM}]")
-        y0 = None
-        if y_init is not None:
-            y0 = np.ascontiguousarray(np.atleast_2d(y_init), dtype=np.float64)
-            if y0.shape != (n, self.neq):
-                raise ValueError("y_init must be [n, neq]")
+        params = _rows(params, n, NPARAM, "params")
+        y0 = None if y_init is None else _rows(y_init, n, self.neq, "y_init")
         if order is not None:
             if order != "densest":
                 raise ValueError(f"unknown order {order!r} (expected 'densest')")
@@ -157,8 +160,9 @@ class Network:
         return dict(y=y_out, rec=rec, n_rec=n_rec, status=status, stats=stats)
 
     def solve_profiled(self, params, lu_bits=64, n_slots=0):
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
+        params = _rows(params, n, NPARAM, "params")
         y_out = np.zeros((n, self.neq))
         status = np.zeros(n, dtype=np.int32)
         stats = np.zeros((n, 8), dtype=np.int32)
@@ -229,10 +233,11 @@ class Network:
 
     # ------------------------------------------------------------ pieces
     def rates(self, params, phys, y):
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
-        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
-        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        y = _rows(y, n, self.neq, "y")
         rate = np.zeros((n, self.nreac))
         y_out = np.zeros((n, self.neq))
         scal = np.zeros((n, 8))
@@ -241,11 +246,12 @@ class Network:
         return rate, y_out, scal
 
     def rhs(self, params, phys, rate, y):
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
-        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
-        rate = np.ascontiguousarray(np.atleast_2d(rate), dtype=np.float64)
-        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        rate = _rows(rate, n, self.nreac, "rate")
+        y = _rows(y, n, self.neq, "y")
         ydot = np.zeros((n, self.neq))
         rate_out = np.zeros((n, self.nreac))
         _check(lib().ucg_rhs(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(rate), _p(y),
@@ -254,11 +260,12 @@ class Network:
 
     def jacobian(self, params, phys, rate, y):
         """Returns J[n, neq, neq] with J[p, i, j] = d ydot_i / d y_j."""
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
-        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
-        rate = np.ascontiguousarray(np.atleast_2d(rate), dtype=np.float64)
-        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        rate = _rows(rate, n, self.nreac, "rate")
+        y = _rows(y, n, self.neq, "y")
         jac = np.zeros((n, self.neq, self.neq))   # column-major per point: [p, j, i]
         _check(lib().ucg_jacobian(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(rate),
                                   _p(y), _p(jac)), "ucg_jacobian")
@@ -266,11 +273,12 @@ class Network:
 
     def newton_solve(self, params, phys, y, gamma, b, lu_bits=64):
         """x = (I - gamma J(y))^-1 b with the library's LU (rates evaluated at y)."""
-        params = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        params = np.atleast_2d(params)
         n = params.shape[0]
-        phys = np.ascontiguousarray(np.atleast_2d(phys), dtype=np.float64)
-        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
-        b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float64)
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        y = _rows(y, n, self.neq, "y")
+        b = _rows(b, n, self.neq, "b")
         x = np.zeros((n, self.neq))
         _check(lib().ucg_newton_solve(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(y),
                                       ctypes.c_double(gamma), _p(b), _p(x), ctypes.c_int(lu_bits)),

@@ -37,3 +37,17 @@ def test_oracle_and_product_param_layouts_agree():
     a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0)
     b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0)
     assert np.array_equal(a, b)
+
+
+def test_collapse_and_hot_core_workloads():
+    """The seeded free-fall grid (log n_H U[2,6] towards 1e6 cm^-3) and hot-core grid
+    (log n_H U[5,8], temp_indx 4, 300 K) carry the model switches the kernel reads."""
+    I = params.INDEX
+    c = workloads.collapse_grid(64)
+    assert np.all(c[:, I["freefall"]] == 1.0) and np.all(c[:, I["finaldens"]] == 1e6)
+    assert np.all((c[:, I["initialdens"]] >= 1e2) & (c[:, I["initialdens"]] <= 1e6))
+    h = workloads.hot_core_grid(64)
+    assert np.all(h[:, I["model"]] == 1.0) and np.all(h[:, I["tempindx"]] == 4.0)
+    assert np.all(h[:, I["maxtemp"]] == 300.0) and np.all(h[:, I["initialtemp"]] == 10.0)
+    assert np.all((h[:, I["initialdens"]] >= 1e5) & (h[:, I["initialdens"]] <= 1e8))
+    assert np.array_equal(c, workloads.collapse_grid(64)) and np.array_equal(h, workloads.hot_core_grid(64))

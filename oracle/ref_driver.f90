@@ -240,6 +240,20 @@ CONTAINS
         ucref_step_trace = k
     END FUNCTION ucref_step_trace
 
+    ! The physics update the solveAbundances loop makes after an interval ends at t_s seconds
+    ! (wrap.f90:672-676: coreUpdatePhysics, then the hot-core temperature for model 1).
+    INTEGER(c_int) FUNCTION ucref_update_physics(t_s, temp_out) BIND(C, name="ucref_update_physics")
+        REAL(c_double), VALUE :: t_s
+        REAL(c_double), INTENT(OUT) :: temp_out
+        dstep = 1
+        currentTime = t_s
+        timeInYears = t_s/SECONDS_PER_YEAR
+        CALL coreUpdatePhysics
+        IF (modelKind .eq. 1) CALL hotcoreUpdatePhysics
+        temp_out = gasTemp(1)
+        ucref_update_physics = 0
+    END FUNCTION ucref_update_physics
+
     ! Overwrite the module rate array (so RHS tests can use arbitrary rates).
     INTEGER(c_int) FUNCTION ucref_set_rates(rate_in) BIND(C, name="ucref_set_rates")
         REAL(c_double), INTENT(IN) :: rate_in(nreac)

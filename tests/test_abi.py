@@ -50,3 +50,29 @@ def test_no_scalar_stores_in_device_code():
     for bad in ("s_store_", "s_atomic_", "s_dcache_wb", "s_buffer_store", "s_scratch_store"):
         assert bad not in text, bad
     assert "v_pk_fma_f32" not in text and "v_pk_mul_f32" not in text and "v_pk_add_f32" not in text
+
+
+def test_error_contract_without_compute():
+    """Error paths of the C ABI need no GPU: negative UCG_E_* codes with a message in
+    ucg_last_error() for a missing device, a missing network file, and bad network or queue
+    handles (no kernel is launched by any of these calls)."""
+    if not os.path.exists(LIB):
+        pytest.skip("library not built")
+    codes = dict(re.findall(r"#define (UCG_E_\w+) (-?\d+)", open(HEADER).read()))
+    L = ctypes.CDLL(LIB)
+    L.ucg_last_error.restype = ctypes.c_char_p
+    rc = L.ucg_init(0)
+    assert rc in (0, int(codes["UCG_E_NODEVICE"]))
+    if rc:
+        assert b"device" in L.ucg_last_error()
+    h = ctypes.c_int(-1)
+    assert L.ucg_network_load_file(b"/nonexistent/net.ucgnet", ctypes.byref(h)) == int(codes["UCG_E_ARG"])
+    assert b"nonexistent" in L.ucg_last_error()
+    bad = int(codes["UCG_E_HANDLE"])
+    assert L.ucg_solve_batch(77, 1, None, None, None, 0, None, None, None, None, 64) == bad
+    assert L.ucg_cancel(77) == bad and L.ucg_network_free(77) == bad
+    assert L.ucg_set_linear_solver(77, 0) == bad
+    for f, args in ((L.ucg_queue_submit, (77, None, None)), (L.ucg_queue_test, (77, 0, None)),
+                    (L.ucg_queue_close, (77,))):
+        assert f(*args) == bad
+        assert b"queue" in L.ucg_last_error()

@@ -5,41 +5,55 @@ Workload (one "step"): the 4096-point static-cloud grid on the default UCLCHEM n
 Av ~ U[1,20]), every point integrated 0 -> 1e6 yr with UCLCHEM's interval semantics,
 all points in one persistent-kernel launch, inputs resident in HBM, the workgroups taking
 the points densest first (--order index: in index order).  FP64 throughout (plain batched
-form); the FP32-factorisation form is reported beside it.
+form); the FP32-factorisation form is reported beside it.  Every point carries a total
+step bound (--max-steps, default 100,000 BDF steps; points reaching it stop with -5 and
+are counted in the line); the CPU baseline runs the reference under the same bound.
+
+A step is a whole grid (tens of seconds), so the timed launches are capped by a wall-time
+budget measured from process start (--budget): the line reports the steps actually
+timed, and the FP32 and CPU legs run only if they fit (the line says when they did not).
 
 Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--points 4096] [--lu 64]
 Prints one JSON line.
 """
 from __future__ import annotations
 
-import argparse
-import json
-import os
-import subprocess
-import sys
 import time
 
-import numpy as np
+T_PROC0 = time.time()
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+import signal  # noqa: E402
+import subprocess  # noqa: E402
+import sys  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64 = 78.6e12     # MI355X dense FP64 (vector and matrix), FLOP/s
+PEAK_FP64 = 78.6e12     # MI355X dense FP64, FLOP/s
 PEAK_HBM = 8.0e12       # bytes/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "solve_points_kernel_grid4096_traffic.txt")
+
+
+def elapsed() -> float:
+    return time.time() - T_PROC0
 
 
 def newton_flop_counts(tables) -> tuple:
-    """FLOPs of one factorisation and one solve of the static-pattern sparse plan:
-    2 per Crout multiply-add pair plus the division of every L entry, 2 T^3 for the
-    Gauss-Jordan inverse of the dense tail (2/3 T^3 for its LU when the plan is not staged);
-    2 per stored factor entry (L and U, both triangular solves) plus the tail's 2 T^2."""
+    """Algorithmic FLOPs of one Newton-matrix factorisation and one Newton solve of the
+    static-pattern sparse plan: 2 per Crout multiply-add pair plus the division of every L
+    entry, and (2/3) T^3 for an LU of the dense T x T tail (the staged plans invert the tail
+    by Gauss-Jordan, 2 T^3, but only the LU's work is counted); a solve is 2 per stored
+    factor entry (both triangular sweeps) plus 2 T^2 for the tail."""
     from uclchem_amd import newton_plan as npl
     s = npl.plan_summary(tables.arrays)
     a = tables.arrays
     ndiv = int((a["nw_tg_div"] >= 0).sum())
-    # the staged plans invert the trailing block by Gauss-Jordan (2 T^3), unstaged ones factor it
-    staged = npl.plan_is_staged(tables)
-    lu = 2.0 * s["npair"] + ndiv + (2.0 if staged else 2.0 / 3.0) * s["T"] ** 3
+    lu = 2.0 * s["npair"] + ndiv + (2.0 / 3.0) * s["T"] ** 3
     solve = 2.0 * s["nsp"] + 2.0 * s["T"] ** 2
     return lu, solve
 
@@ -60,20 +74,64 @@ def rhs_flop_count(tables) -> float:
     return float(mults + adds)
 
 
-def cpu_baseline(params: np.ndarray, cores: int, cap_s: float, n_sample: int) -> dict:
-    """The reference Fortran path (oracle/_ref) on the box's host cores, one process per
-    core, each solving one grid point with a wall-time cap."""
+def committed_traffic(path: str = TRAFFIC_FILE):
+    """HBM bytes per launch of the benchmarked grid launch from the committed counter
+    summary, or (None, reason) when the summary was measured on other kernel sources than
+    the ones built here (its source_sha line against uclchem_amd.build.source_hash())."""
+    from uclchem_amd.build import source_hash
+    if not os.path.exists(path):
+        return None, "no counter summary"
+    vals = {}
+    for ln in open(path):
+        parts = ln.split()
+        if len(parts) >= 2:
+            vals[parts[0]] = parts[1]
+    want = source_hash()
+    if vals.get("source_sha") != want:
+        return None, f"counter summary is of other kernel sources ({vals.get('source_sha')} != {want})"
+    return float(vals["traffic_bytes_per_launch"]), "rocprofv3 FETCH_SIZE/WRITE_SIZE of this build"
+
+
+def stratified_sample(params: np.ndarray, n: int, seed: int = 20261202) -> np.ndarray:
+    """n grid-point indices, one drawn at random from each of n equal-count strata of
+    initial density (a point's integration cost grows with density)."""
+    from uclchem_amd.params import INDEX
+    order = np.argsort(params[:, INDEX["initialdens"]], kind="stable")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    strata = np.array_split(order, n)
+    return np.array(sorted(int(rng.choice(s)) for s in strata if len(s)))
+
+
+def host_cpu() -> tuple:
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(params: np.ndarray, cores: int, cap_s: float, n_sample: int, max_steps: float) -> dict:
+    """The reference Fortran path (oracle/_ref, compiled -O3) on the box's host cores, one
+    single-threaded process per core pulling points from a queue, under the same total step
+    bound as the GPU run and a wall-time cap per point."""
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libucref.so")
     if not os.path.exists(ref_so):
         return None
-    sample = params[:n_sample]
-    procs = []
+    from uclchem_amd.params import INDEX
+    idx = stratified_sample(params, n_sample)
+    sample = params[idx].copy()
+    sample[:, INDEX["maxsteps"]] = max_steps
     t0 = time.time()
     code = ("import sys,time,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
             "p=np.array([float(x) for x in sys.argv[1].split(',')]); t=time.time(); "
-            "rc,rows,st=RefLib().model(p,maxrec=2000); print('%%d %%.6f'%%(rc,time.time()-t))") % os.path.join(ROOT, "oracle")
+            "rc,rows,st=RefLib().model(p,maxrec=2000); print('%%d %%.6f %%d'%%(rc,time.time()-t,st[4]))"
+            ) % os.path.join(ROOT, "oracle")
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    running, times, capped = [], [], 0
+    running, times, capped, bounded = [], [], 0, 0
     queue = list(range(len(sample)))
     while queue or running:
         while queue and len(running) < cores:
@@ -86,7 +144,8 @@ def cpu_baseline(params: np.ndarray, cores: int, cap_s: float, n_sample: int) ->
         for pr, ts in running:
             if pr.poll() is not None:
                 out = pr.stdout.read().split()
-                times.append(float(out[1]) if len(out) == 2 else time.time() - ts)
+                times.append(float(out[1]) if len(out) == 3 else time.time() - ts)
+                bounded += int(len(out) == 3 and int(out[0]) == -5)
             elif time.time() - ts > cap_s:
                 pr.kill()
                 pr.wait()
@@ -98,11 +157,26 @@ def cpu_baseline(params: np.ndarray, cores: int, cap_s: float, n_sample: int) ->
         time.sleep(0.05)
     wall = time.time() - t0
     per_point = float(np.mean(times))
+    model, nproc = host_cpu()
     return {"value": cores / per_point, "unit": "grid-point solves/s", "cores": cores, "kind": "reference",
-            "sample": (f"first {len(sample)} points of the same 4096-point grid through the reference Fortran "
-                       f"(oracle/_ref, amdflang -O2), one process per core; {capped} point(s) hit the "
-                       f"{cap_s:.0f} s cap and count as {cap_s:.0f} s (a lower bound); mean {per_point:.2f} s/point; "
-                       f"sample wall {wall:.1f} s")}
+            "sample": (f"{len(sample)} points of the same grid, one per density stratum (seeded), through the "
+                       f"reference Fortran hot path (oracle/_ref, amdflang -O3), one single-threaded process per "
+                       f"core on {cores} cores (the box's CPU share for one GPU; nproc {nproc}, {model}); same "
+                       f"step bound {max_steps:.0f} ({bounded} point(s) reached it); {capped} point(s) hit the "
+                       f"{cap_s:.0f} s wall cap and count as {cap_s:.0f} s (a lower bound on their time); mean "
+                       f"{per_point:.2f} s/point; throughput = cores / mean; sample wall {wall:.1f} s")}
+
+
+PARTIAL = {}
+
+
+def _emit_partial(signum, frame):
+    """Print what has been measured if the run is stopped from outside (SIGTERM)."""
+    if PARTIAL:
+        line = dict(PARTIAL)
+        line["partial"] = f"stopped by signal {signum} at {elapsed():.0f} s"
+        print(json.dumps(line), flush=True)
+    sys.exit(1)
 
 
 def main():
@@ -113,8 +187,11 @@ def main():
     ap.add_argument("--points", type=int, default=4096)
     ap.add_argument("--lu", type=int, default=64)
     ap.add_argument("--also-fp32", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=16)
-    ap.add_argument("--cpu-cap", type=float, default=25.0)
+    ap.add_argument("--budget", type=float, default=480.0,
+                    help="seconds from process start by which every leg must have ended; timed launches "
+                         "beyond the first, the FP32 leg and the CPU leg run only if they fit")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="CPU baseline points (0: one per core)")
+    ap.add_argument("--cpu-cap", type=float, default=30.0, help="wall-time cap per CPU baseline point, s")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump", default="", help="write per-point counters and times to <prefix>_fp<bits>.npz")
     ap.add_argument("--workload", default="grid", choices=["grid", "collapse500", "hotcore", "serving"],
@@ -148,6 +225,7 @@ def main():
         return
     if a.workload == "serving":
         return bench_serving(a)
+    signal.signal(signal.SIGTERM, _emit_partial)
     import torch
     from uclchem_amd import gpu, workloads
     from uclchem_amd import makerates_hip as mh
@@ -177,65 +255,68 @@ def main():
     d_t1 = torch.zeros(n, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    side = torch.cuda.Stream(dev)                 # default priority, like every stream here
-    host_done = torch.empty(n, dtype=torch.float64, pin_memory=True)
-
     def run(lu_bits, npts, label):
+        """One launch on `stream`; returns its event-timed duration in ms."""
         d_t1.zero_()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
         net.solve_device(d_params[:npts], d_y[:npts], d_status[:npts], d_stats[:npts], t_start=d_t0[:npts],
                          t_done=d_t1[:npts], lu_bits=lu_bits, stream=stream.cuda_stream,
                          order=None if a.order == "index" else a.order)
-        done_ev = torch.cuda.Event()
-        done_ev.record(stream)
-        t_last = t_launch = time.time()
-        while not done_ev.query():                # progress from the kernel's own retirement stamps
-            time.sleep(0.5)
-            if time.time() - t_last > 20.0:
-                if a.order == "index":
-                    with torch.cuda.stream(side):
-                        host_done[:npts].copy_(d_t1[:npts], non_blocking=True)
-                    side.synchronize()
-                    print(f"[bench] {label}: {int((host_done[:npts] > 0).sum())}/{npts} points retired",
-                          file=sys.stderr, flush=True)
-                else:                             # the stamps land in d_t1 when the launch ends
-                    print(f"[bench] {label}: running, {time.time() - t_launch:.0f} s", file=sys.stderr, flush=True)
-                t_last = time.time()
-
-    results = {}
-    for lu_bits in ([a.lu] + ([32] if (a.also_fp32 and a.lu == 64) else [])):
-        for _ in range(a.warmup):
-            run(lu_bits, min(n, 16), f"fp{lu_bits} warmup")
-        torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t_wall = time.time()
-        ev0.record(stream)
-        for _ in range(a.steps):
-            run(lu_bits, n, f"fp{lu_bits}")
         ev1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.time() - t_wall
-        ms = ev0.elapsed_time(ev1) / a.steps
+        t_last = t_launch = time.time()
+        while not ev1.query():
+            time.sleep(0.2)
+            if time.time() - t_last > 20.0:
+                print(f"[bench] {label}: running, {time.time() - t_launch:.0f} s", file=sys.stderr, flush=True)
+                t_last = time.time()
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1)
+
+    def collect(ms_list, lu_bits):
         st = d_stats.cpu().numpy()
         status = d_status.cpu().numpy()
         t0 = d_t0.cpu().numpy()
         t1 = d_t1.cpu().numpy()
         lat = (t1 - t0.min()) * 1e-8          # launch to retirement, device wall clock at 100 MHz
         solve = (t1 - t0) * 1e-8              # one point's own solve time on its workgroup
-        results[lu_bits] = dict(ms=ms, wall=wall, stats=st, status=status, lat=lat, solve=solve,
-                                y=d_y.cpu().numpy().copy())
+        r = dict(ms=float(np.mean(ms_list)), steps=len(ms_list), launches_ms=[round(m, 1) for m in ms_list],
+                 stats=st, status=status, lat=lat, solve=solve, y=d_y.cpu().numpy().copy())
         if a.dump:
             np.savez_compressed(a.dump + f"_fp{lu_bits}.npz", stats=st, status=status, solve=solve, lat=lat)
+        return r
+
+    cpu_reserve = 0.0 if a.no_cpu else a.cpu_cap + 20.0
+    skipped = {}
+    # warm-up: W untimed passes over the grid's first 16 points (a whole grid is a long launch)
+    for _ in range(a.warmup):
+        run(a.lu, min(n, 16), f"fp{a.lu} warmup")
+    torch.cuda.synchronize()
+    # timed launches of the main form: the first always, the others while they fit
+    results = {}
+    ms_list = []
+    want32 = bool(a.also_fp32 and a.lu == 64)
+    while len(ms_list) < a.steps:
+        if ms_list:
+            est = max(ms_list) / 1e3
+            after = (1.15 * est if want32 else 0.0) + cpu_reserve + 10.0
+            if elapsed() + 1.1 * est + after > a.budget:
+                break
+        torch.cuda.synchronize()
+        ms_list.append(run(a.lu, n, f"fp{a.lu} step {len(ms_list) + 1}"))
+        print(f"[bench] fp{a.lu} step {len(ms_list)}: {ms_list[-1] / 1e3:.1f} s (t={elapsed():.0f} s)",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    results[a.lu] = collect(ms_list, a.lu)
     main_r = results[a.lu]
     value = n / (main_r["ms"] / 1e3)
     lu_flop, solve_flop = newton_flop_counts(tables)
-    flops = algorithmic_flops(main_r["stats"], lu_flop, solve_flop, rhs_flop_count(tables))
+    rhs_flop = rhs_flop_count(tables)
+    flops = algorithmic_flops(main_r["stats"], lu_flop, solve_flop, rhs_flop)
     achieved = flops / (main_r["ms"] / 1e3)
-    traffic = None      # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
-    tpath = os.path.join(ROOT, "profiles", "solve_points_kernel_grid4096_traffic.txt")
-    if a.workload == "grid" and a.lu == 64 and n == 4096 and a.max_steps == 100000.0 and os.path.exists(tpath):
-        for ln in open(tpath):
-            if ln.startswith("traffic_bytes_per_launch"):
-                traffic = float(ln.split()[1])
+    traffic, traffic_src = (committed_traffic() if (a.workload == "grid" and a.lu == 64 and n == 4096 and
+                                                    a.max_steps == 100000.0 and a.order == "densest")
+                            else (None, "no counter summary for this workload"))
     wl = {"grid": (f"static cloud grid, default network ({tables.nspec} species, {tables.nreac} reactions), "
                    f"{n} points to 1e6 yr, FP{a.lu} factorisation" + (" (plain batched FP64)" if a.lu == 64 else "")),
           "collapse500": (f"free-fall collapse grid (n_H,0 log-uniform 1e2..1e6 towards 1e6 cm^-3, 1 Myr), "
@@ -243,13 +324,17 @@ def main():
                           f"FP{a.lu} factorisation"),
           "hotcore": (f"hot-core warm-up grid (temp_indx 4, 10->300 K, log n_H U[5,8]), default network, {n} "
                       f"points to 1e6 yr, FP{a.lu} factorisation")}[a.workload]
+    if a.max_steps > 0:
+        wl += f", total step bound {a.max_steps:.0f} BDF steps per point"
     line = {
         "metric": "grid-point solves/s to 1 Myr (p50/p99 solve latency, 1% abundance agreement)",
-        "value": round(value, 3), "unit": "grid-point solves/s", "n_gpus": 1, "steps": a.steps,
+        "value": round(value, 3), "unit": "grid-point solves/s", "n_gpus": 1, "steps": main_r["steps"],
         "warmup": a.warmup, "ms_per_step": round(main_r["ms"], 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": f"synthetic (seeded {n}-point {a.workload} workload)",
         "config": {"workload": wl, "points": n, "network": tables.name, "lu": f"fp{a.lu}",
-                   "schedule": "densest first" if a.order == "densest" else "index order"},
+                   "schedule": "densest first" if a.order == "densest" else "index order",
+                   "max_steps_per_point": a.max_steps},
+        "steps_requested": a.steps, "launches_ms": main_r["launches_ms"],
         "latency_s": {"p50": float(np.percentile(main_r["lat"], 50)), "p99": float(np.percentile(main_r["lat"], 99)),
                       "solve_p50": float(np.percentile(main_r["solve"], 50)),
                       "solve_p99": float(np.percentile(main_r["solve"], 99))},
@@ -261,31 +346,46 @@ def main():
         "solver_counters": {"steps": int(main_r["stats"][:, 0].sum()), "rhs": int(main_r["stats"][:, 1].sum()),
                             "newton_setups": int(main_r["stats"][:, 2].sum()),
                             "newton_iters": int(main_r["stats"][:, 3].sum())},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
-                     "note": (f"solve_points_kernel (one launch = the whole grid); FP64 FLOPs of the sparse-plan "
-                              f"factorisation ({lu_flop:.3g} per Newton-matrix setup), solves ({solve_flop:.3g} per "
-                              f"Newton iteration) and RHS ({rhs_flop_count(tables):.3g} per evaluation) from the "
-                              f"solver counters, over the launch time; traffic: HBM bytes per launch from "
-                              f"rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy (profiles/)")},
+                     "note": (f"solve_points_kernel (one launch = the whole grid), algorithmic FP64 FLOPs from the "
+                              f"solver counters over the event-timed launch: sparse-plan factorisation "
+                              f"({lu_flop:.4g} per Newton-matrix setup: Crout pairs x2, divisions, 2/3 T^3 of the "
+                              f"dense tail), solves ({solve_flop:.4g} per Newton iteration) and RHS "
+                              f"({rhs_flop:.4g} per evaluation); traffic: {traffic_src}")},
     }
-    if 32 in results and a.lu == 64:
-        r32 = results[32]
-        line["fp32_lu"] = {"value": round(n / (r32["ms"] / 1e3), 3), "ms_per_step": round(r32["ms"], 3),
-                           "latency_p50": float(np.percentile(r32["lat"], 50)),
-                           "latency_p99": float(np.percentile(r32["lat"], 99)),
-                           "status_nonzero": int(np.count_nonzero(r32["status"]))}
-        ok = (main_r["status"] == 0) & (r32["status"] == 0)
-        y64, y32 = main_r["y"][ok, :tables.nspec], r32["y"][ok, :tables.nspec]
-        rel = np.where(y64 > 1e-14, np.abs(y32 - y64) / np.maximum(y64, 1e-300), 0.0).max(axis=1)
-        line["fp32_lu"].update({"points_compared": int(ok.sum()),
-                                "points_within_1pct_of_fp64": int((rel <= 0.01).sum()),
-                                "median_max_rel_vs_fp64": float(np.median(rel)) if len(rel) else None})
-    if not a.no_cpu:
-        cores = min(16, len(os.sched_getaffinity(0)))
-        line["cpu_baseline"] = cpu_baseline(params, cores, a.cpu_cap, a.cpu_sample)
+    PARTIAL.update(line)
+    if want32:
+        est = main_r["ms"] / 1e3
+        if elapsed() + 1.15 * est + cpu_reserve + 10.0 <= a.budget:
+            torch.cuda.synchronize()
+            ms32 = run(32, n, "fp32")
+            torch.cuda.synchronize()
+            r32 = results[32] = collect([ms32], 32)
+            line["fp32_lu"] = {"value": round(n / (r32["ms"] / 1e3), 3), "ms_per_step": round(r32["ms"], 3),
+                               "steps": 1, "latency_p50": float(np.percentile(r32["lat"], 50)),
+                               "latency_p99": float(np.percentile(r32["lat"], 99)),
+                               "status_nonzero": int(np.count_nonzero(r32["status"]))}
+            ok = (main_r["status"] == 0) & (r32["status"] == 0)
+            y64, y32 = main_r["y"][ok, :tables.nspec], r32["y"][ok, :tables.nspec]
+            rel = np.where(y64 > 1e-14, np.abs(y32 - y64) / np.maximum(y64, 1e-300), 0.0).max(axis=1)
+            line["fp32_lu"].update({"points_compared": int(ok.sum()),
+                                    "points_within_1pct_of_fp64": int((rel <= 0.01).sum()),
+                                    "median_max_rel_vs_fp64": float(np.median(rel)) if len(rel) else None})
+        else:
+            skipped["fp32_lu"] = f"did not fit the {a.budget:.0f} s budget at t={elapsed():.0f} s"
+        PARTIAL.update(line)
+    cores = min(16, len(os.sched_getaffinity(0)))
+    if a.no_cpu:
+        line["cpu_baseline"] = None
+    elif elapsed() + cpu_reserve <= a.budget + 20.0:
+        line["cpu_baseline"] = cpu_baseline(params, cores, a.cpu_cap, a.cpu_sample or cores, a.max_steps)
     else:
         line["cpu_baseline"] = None
+        skipped["cpu_baseline"] = f"did not fit the {a.budget:.0f} s budget at t={elapsed():.0f} s"
+    if skipped:
+        line["skipped"] = skipped
+    line["wall_s"] = round(elapsed(), 1)
     print(json.dumps(line), flush=True)
 
 

@@ -21,14 +21,14 @@ REF_SO = os.path.join(HERE, "_ref", "libucref.so")
 # parameter vector layout (oracle/ucl_oracle.h, oracle/ref_driver.f90)
 P_NAMES = ["dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall",
            "finaldens", "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp",
-           "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep"]
+           "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps"]
 P_NPARAM = 24
 # Defaults of defaultparameters.f90:18-122.  rout is a default-real literal (0.05 -> float32).
 DEFAULTS = dict(dens=1.0e2, temp=10.0, zeta=1.0, radfield=1.0, baseav=2.0,
                 rout=float(np.float32(0.05)), rin=0.0, finaltime=5.0e6, freefall=0.0,
                 finaldens=1.0e5, reltol=1e-8, abstol_factor=1e-14, abstol_min=1e-25, model=0.0,
                 tempindx=1.0, maxtemp=300.0, freezefactor=1.0, endatfinaldensity=0.0,
-                freefallfactor=1.0, mxstep=10000.0)
+                freefallfactor=1.0, mxstep=10000.0, maxsteps=0.0)
 
 
 def make_params(**kw) -> np.ndarray:

@@ -40,7 +40,7 @@ MODULE ucref_driver
     INTEGER, PARAMETER :: P_DENS=1, P_TEMP=2, P_ZETA=3, P_RADFIELD=4, P_BASEAV=5, P_ROUT=6, P_RIN=7, &
         & P_FINALTIME=8, P_FREEFALL=9, P_FINALDENS=10, P_RELTOL=11, P_ABSFAC=12, P_ABSMIN=13, &
         & P_MODEL=14, P_TEMPINDX=15, P_MAXTEMP=16, P_FREEZEFAC=17, P_ENDATFINALDENS=18, P_FREEFALLFAC=19, &
-        & P_MXSTEP=20, P_NPARAM=24
+        & P_MXSTEP=20, P_MAXSTEPS=21, P_NPARAM=24
 CONTAINS
 
     SUBROUTINE reset_globals()
@@ -118,6 +118,54 @@ CONTAINS
         CALL initializeChemistry(.false.)
         dstep=1   ! coreInitializePhysics leaves its loop counter at points+1 (wrap.f90:643 resets it too)
     END SUBROUTINE setup_point
+
+    ! updateChemistry (chemistry.f90:145-239) for one point (points=1, no post-processed
+    ! column densities, no refractory species: the default network's refractoryList is
+    ! (/-999/), network.f90:262) restated with one addition: before every integrator call
+    ! the steps taken so far by the point (nsteps, all attempts of all intervals, from
+    ! GET_STATS NST) are compared with a total-step bound, and the point stops with
+    ! INT_TOO_MANY_FAILS_ERROR once it is reached.  The rates, the integrator call and its
+    ! retry handling are the reference's own (calculateReactionRates, integrateODESystem).
+    ! With usepostprocess (its default, postprocess.f90:14) the reference never raises its
+    ! failure counters, so they are not restated.
+    SUBROUTINE bounded_update_chemistry(maxsteps, nsteps, flag)
+        REAL(dp), INTENT(IN) :: maxsteps
+        REAL(dp), INTENT(INOUT) :: nsteps
+        INTEGER, INTENT(OUT) :: flag
+        REAL(dp) :: originalTargetTime, surfCov, rstats(22)
+        INTEGER :: loops, istats(31)
+        loops = 0
+        flag = 0
+        originalTargetTime = targetTime
+        DO WHILE ((currentTime .lt. targetTime) .and. (loops .lt. 10))
+            IF (nsteps >= maxsteps) THEN
+                flag = INT_TOO_MANY_FAILS_ERROR
+                RETURN
+            END IF
+            IF (.not. freefall) abund(nspec+1,dstep)=density(dstep)
+            h2ColToCell=0.0
+            coColToCell=0.0
+            cColToCell=0.0
+            h2Col=h2ColToCell+0.5*abund(nh2,dstep)*density(dstep)*(cloudSize/real(points))
+            coCol=coColToCell+0.5*abund(nco,dstep)*density(dstep)*(cloudSize/real(points))
+            cCol=cColToCell+0.5*abund(nc,dstep)*density(dstep)*(cloudSize/real(points))
+            abund(nBulk,dstep)=sum(abund(bulkList,dstep))
+            abund(nSurface,dstep)=sum(abund(surfaceList,dstep))
+            safeMantle=MAX(1d-30,abund(nSurface,dstep))
+            safeBulk=MAX(1d-30,abund(nBulk,dstep))
+            bulkLayersReciprocal=MIN(1.0,NUM_SITES_PER_GRAIN/(GAS_DUST_DENSITY_RATIO*safeBulk))
+            surfCov=bulkGainFromMantleBuildUp()
+            CALL calculateReactionRates(abund,safeMantle, h2col, cocol, ccol, rate)
+            CALL integrateODESystem(flag)
+            CALL GET_STATS(rstats, istats)
+            nsteps = nsteps + real(istats(11), dp)
+            IF (flag .lt. 0) RETURN
+            WHERE(abund<MIN_ABUND) abund=MIN_ABUND
+            density(dstep)=abund(NEQ,dstep)
+            loops = loops + 1
+            targetTime = originalTargetTime
+        END DO
+    END SUBROUTINE bounded_update_chemistry
 
     ! ---------------------------------------------------------------- C entry points
 
@@ -269,7 +317,9 @@ CONTAINS
 
     ! Full single-point model (solveAbundances loop, wrap.f90:598-690, points=1).
     ! rec(:, k) = (time_yr, density, gasTemp, av, y(1:nspec)) for each output row k,
-    ! row 1 being t=0.  stats = (nsteps_last, nfe_last, nje_last, n_intervals, n_dvode_calls_sum, ...)
+    ! row 1 being t=0.  stats = (NST, NFE, NJE summed over the last integrator call of each
+    ! interval, n_intervals, steps of every call when p(P_MAXSTEPS) > 0, 0, 0, 0).
+    ! p(P_MAXSTEPS) > 0 bounds the point's total steps (bounded_update_chemistry).
     INTEGER(c_int) FUNCTION ucref_model(p, maxrec, rec, nrec, stats) BIND(C, name="ucref_model")
         REAL(c_double), INTENT(IN) :: p(P_NPARAM)
         INTEGER(c_int), VALUE :: maxrec
@@ -277,10 +327,11 @@ CONTAINS
         INTEGER(c_int), INTENT(OUT) :: nrec
         INTEGER(c_int), INTENT(OUT) :: stats(8)
         INTEGER :: flag
-        REAL(dp) :: rstats(22)
+        REAL(dp) :: rstats(22), nsteps
         INTEGER :: istats(31)
         stats = 0
         nrec = 0
+        nsteps = 0.0d0
         CALL setup_point(p, flag)
         IF (flag .lt. 0) THEN
             ucref_model = flag
@@ -298,7 +349,11 @@ CONTAINS
             END IF
             dstep=1
             currentTime=currentTimeold
-            CALL updateChemistry(flag)
+            IF (p(P_MAXSTEPS) > 0.0d0) THEN
+                CALL bounded_update_chemistry(p(P_MAXSTEPS), nsteps, flag)
+            ELSE
+                CALL updateChemistry(flag)
+            END IF
             IF (flag .lt. 0) EXIT
             CALL GET_STATS(rstats, istats)
             stats(1) = stats(1) + istats(11)
@@ -314,6 +369,7 @@ CONTAINS
                 EXIT
             END IF
         END DO
+        stats(5) = INT(MIN(nsteps, 2.0d9))
         ucref_model = flag
     END FUNCTION ucref_model
 

@@ -275,3 +275,21 @@ def test_reference_reproduces_collapse_example(reflib):
     m = g["phase1_final"] > 1e-14
     rel = np.abs(rows[-1, 4:][m] - g["phase1_final"][m]) / g["phase1_final"][m]
     assert rel.max() < 0.01
+
+
+def test_step_bounded_reference_loop(reflib):
+    """The reference driver's step-bounded interval loop (ref_driver.f90
+    bounded_update_chemistry, the CPU baseline's work bound) restates updateChemistry
+    (chemistry.f90:145-239): with a bound that is never reached it gives the reference
+    loop's rows bit for bit; with a small bound the point stops with -5 once its steps
+    (all attempts counted) reach it."""
+    from oracle import make_params
+    p = make_params(dens=1e4, temp=10.0, finaltime=1e3)
+    rc0, rows0, st0 = reflib.model(p)
+    rc1, rows1, st1 = reflib.model(make_params(dens=1e4, temp=10.0, finaltime=1e3, maxsteps=1e12))
+    assert rc0 == 0 and rc1 == 0
+    assert np.array_equal(rows0, rows1)
+    assert st1[4] == st1[0] > 0          # one attempt per interval here: all steps = last-call steps
+    rc2, rows2, st2 = reflib.model(make_params(dens=1e4, temp=10.0, finaltime=1e3, maxsteps=200))
+    assert rc2 == -5 and st2[4] >= 200 and len(rows2) < len(rows0)
+    assert np.array_equal(rows2, rows0[:len(rows2)])

@@ -59,6 +59,19 @@ def write_network_files(verbose: bool = False) -> None:
                 print(f"wrote {out}", flush=True)
 
 
+def source_hash() -> str:
+    """Short digest of the HIP sources (csrc/*.hip, *.h): names the kernel build a
+    committed profile was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith((".hip", ".h")):
+            h.update(f.encode())
+            with open(os.path.join(CSRC, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def build(verbose: bool = False, extra=None) -> str:
     src = os.path.join(CSRC, "ucg_api.hip")
     tmp = LIB + ".tmp"

@@ -126,16 +126,22 @@ int ucg_solve_batch(int handle, int n, const double *params, const double *y_ini
 /* Same on device-resident buffers (all pointers are device pointers, stream is a
  * hipStream_t or NULL).  t_start/t_done [n] (nullable) receive the device wall clock
  * (100 MHz ticks) when each point was taken and retired.  n_slots = resident
- * workgroups (0: two per compute unit).  Asynchronous with respect to the host. */
+ * workgroups (0: two per compute unit).  Asynchronous with respect to the host.
+ * Every buffer is checked to be device memory of the library's GPU, large enough for n
+ * points (UCG_E_ARG otherwise); each point's parameter vector is checked on the device
+ * and a point that fails the checks of ucg_solve_batch reports UCG_E_PARAM unintegrated. */
 int ucg_solve_batch_device(int handle, int n, const double *params, const double *y_init, double *y_out,
                            int max_rec, double *rec, int *n_rec, int *status, int *stats,
                            double *t_start, double *t_done, int lu_bits, int n_slots, void *stream);
 
-/* Stop the running ucg_solve_batch_device call(s) on a handle: workgroups finish their
- * current integrator step, take no further points, and unfinished points report
- * UCG_E_CANCELLED.  Callable from another host thread while the solve runs.  No
- * counterpart in the reference (its grids run until every point ends); for serving
- * loops and time-bounded batches. */
+/* Stop every solve launched on a handle before this call (ucg_solve_batch_device launches,
+ * including ones queued behind other work, and the handle's request queues): workgroups
+ * finish their current integrator step and take no further points; unfinished points
+ * report UCG_E_CANCELLED and a stopped queue's unrun requests fail ucg_queue_test.
+ * Launches issued after the call are not affected.  Callable from another host thread
+ * while the solves run (not concurrently with ucg_network_free of the same handle).  No
+ * counterpart in the reference (its grids run until every point ends); for serving loops
+ * and time-bounded batches. */
 int ucg_cancel(int handle);
 
 /* ucg_solve_batch with per-point device timers [n][16]: seconds in rates, RHS, Jacobian
@@ -145,6 +151,23 @@ int ucg_cancel(int handle);
  * (no output rows).  n_slots as above. */
 int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,
                              double *timers, int lu_bits, int n_slots);
+
+/* One output interval of UCLCHEM's updateChemistry for a batch of points, each from its
+ * own state: replaces one pass of the solveAbundances loop body's chemistry call
+ * (src/fortran_src/chemistry.f90:145-239: ice bookkeeping and calculateReactionRates at
+ * the interval start, integrateODESystem -> DVODE_F90 up to 10 attempts with the
+ * reference's retry handling, chemistry.f90:241-292, dvode.f90:2084; the 1e-30 clamp).
+ * No physics update afterwards (wrap.f90:672-676 is the caller's).  Host arrays:
+ *   params [n][UCG_NPARAM] (tolerances, MXSTEP, model switches);
+ *   phys   [n][8] = density, gasTemp, dustTemp, Av, cloudSize [cm], phi (the CR-desorption
+ *          yield left by the previous rates call), abstol_factor, current time [s];
+ *   y      [n][neq] state at the interval start; t_target [n] end of the interval [s];
+ *   y_out  [n][neq]; phys_out [n][8] the same fields after the interval (phi and
+ *          abstol_factor as the attempts left them, the time reached);
+ *   status [n] (0, or UCLCHEM's error codes); stats [n][8] as for ucg_solve_batch.
+ * lu_bits as for ucg_solve_batch. */
+int ucg_interval(int handle, int n, const double *params, const double *phys, const double *y,
+                 const double *t_target, double *y_out, double *phys_out, int *status, int *stats, int lu_bits);
 
 /* Fine-grained kernels (host arrays).  phys [n][8] = density, gasTemp, dustTemp, Av,
  * cloudSize [cm], phi (the CR-desorption yield carried from the previous call),

@@ -201,5 +201,42 @@ class RefLib:
                                   ctypes.c_int(maxrec), _ptr(rec), ctypes.byref(nrec), _ptr(stats))
         return rc, rec[:nrec.value].copy(), stats
 
+    def model_until(self, params, t_stop_years: float, maxrec: int = 1000):
+        """Run the model loop until the first interval ending at or beyond t_stop_years; returns
+        (rc, rows, stats, y, phys) with phys = (density, gasTemp, dustTemp, Av, cloudSize, phi,
+        abstol_factor, time [s]), the state the next interval starts from."""
+        rec = np.zeros((maxrec, 4 + self.nspec))
+        nrec = ctypes.c_int()
+        stats = np.zeros(8, dtype=np.int32)
+        y, phys = np.zeros(self.neq), np.zeros(8)
+        rc = self.lib.ucref_model_until(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                        ctypes.c_double(t_stop_years), ctypes.c_int(maxrec), _ptr(rec),
+                                        ctypes.byref(nrec), _ptr(stats), _ptr(y), _ptr(phys))
+        return rc, rec[:nrec.value].copy(), stats, y, phys
+
+    def interval(self, params, y, phys, t_target: float, maxsteps: float = 0.0):
+        """One updateChemistry interval from the state (y, phys) to t_target seconds;
+        returns (rc, y_out, phys_out, stats) with stats = (steps, attempts, ...)."""
+        y_out, phys_out = np.zeros(self.neq), np.zeros(8)
+        stats = np.zeros(8, dtype=np.int32)
+        rc = self.lib.ucref_interval(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                     _ptr(np.ascontiguousarray(y, dtype=np.float64)),
+                                     _ptr(np.ascontiguousarray(phys, dtype=np.float64)),
+                                     ctypes.c_double(t_target), ctypes.c_double(maxsteps),
+                                     _ptr(y_out), _ptr(phys_out), _ptr(stats))
+        return rc, y_out, phys_out, stats
+
+    def update_physics(self, t_s: float) -> float:
+        """The loop's physics update after an interval ending at t_s seconds; returns gasTemp."""
+        temp = ctypes.c_double()
+        self.lib.ucref_update_physics(ctypes.c_double(t_s), ctypes.byref(temp))
+        return temp.value
+
+    def state(self):
+        """(y, phys) of the point as the next interval would start from it."""
+        y, phys = np.zeros(self.neq), np.zeros(8)
+        self.lib.ucref_get_state(_ptr(y), _ptr(phys))
+        return y, phys
+
     def network_array(self, sym, n, ctype=ctypes.c_double):
         return np.ctypeslib.as_array((ctype * n).in_dll(self.lib, sym)).copy()

@@ -128,13 +128,15 @@ CONTAINS
     ! retry handling are the reference's own (calculateReactionRates, integrateODESystem).
     ! With usepostprocess (its default, postprocess.f90:14) the reference never raises its
     ! failure counters, so they are not restated.
-    SUBROUTINE bounded_update_chemistry(maxsteps, nsteps, flag)
+    SUBROUTINE bounded_update_chemistry(maxsteps, nsteps, flag, attempts)
         REAL(dp), INTENT(IN) :: maxsteps
         REAL(dp), INTENT(INOUT) :: nsteps
         INTEGER, INTENT(OUT) :: flag
+        INTEGER(c_int), INTENT(OUT) :: attempts
         REAL(dp) :: originalTargetTime, surfCov, rstats(22)
         INTEGER :: loops, istats(31)
         loops = 0
+        attempts = 0
         flag = 0
         originalTargetTime = targetTime
         DO WHILE ((currentTime .lt. targetTime) .and. (loops .lt. 10))
@@ -163,6 +165,7 @@ CONTAINS
             WHERE(abund<MIN_ABUND) abund=MIN_ABUND
             density(dstep)=abund(NEQ,dstep)
             loops = loops + 1
+            attempts = loops
             targetTime = originalTargetTime
         END DO
     END SUBROUTINE bounded_update_chemistry
@@ -326,7 +329,95 @@ CONTAINS
         REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec)
         INTEGER(c_int), INTENT(OUT) :: nrec
         INTEGER(c_int), INTENT(OUT) :: stats(8)
+        ucref_model = run_model(p, HUGE(1.0d0), maxrec, rec, nrec, stats)
+    END FUNCTION ucref_model
+
+    ! The same loop stopped after the first interval that ends at or beyond t_stop years;
+    ! y_out and phys_out are the point's state there as the next interval starts from it
+    ! (after the physics update): phys = (density, gasTemp, dustTemp, av, cloudSize, phi,
+    ! abstol_factor, currentTime [s]).
+    INTEGER(c_int) FUNCTION ucref_model_until(p, t_stop, maxrec, rec, nrec, stats, y_out, phys_out) &
+            BIND(C, name="ucref_model_until")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        REAL(c_double), VALUE :: t_stop
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec)
+        INTEGER(c_int), INTENT(OUT) :: nrec
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
+        REAL(c_double), INTENT(OUT) :: y_out(nspec+1), phys_out(8)
+        ucref_model_until = run_model(p, t_stop, maxrec, rec, nrec, stats)
+        CALL get_state(y_out, phys_out)
+    END FUNCTION ucref_model_until
+
+    ! The point's current state in the layout of ucref_model_until (e.g. after
+    ! ucref_interval and ucref_update_physics, to continue the loop interval by interval).
+    INTEGER(c_int) FUNCTION ucref_get_state(y_out, phys_out) BIND(C, name="ucref_get_state")
+        REAL(c_double), INTENT(OUT) :: y_out(nspec+1), phys_out(8)
+        CALL get_state(y_out, phys_out)
+        ucref_get_state = 0
+    END FUNCTION ucref_get_state
+
+    SUBROUTINE get_state(y, phys)
+        REAL(c_double), INTENT(OUT) :: y(nspec+1), phys(8)
+        y = abund(:,1)
+        phys(1) = density(1)
+        phys(2) = gasTemp(1)
+        phys(3) = dustTemp(1)
+        phys(4) = av(1)
+        phys(5) = cloudSize
+        phys(6) = phi
+        phys(7) = abstol_factor
+        phys(8) = currentTime
+    END SUBROUTINE get_state
+
+    ! One output interval (updateChemistry, chemistry.f90:145-239, restated as
+    ! bounded_update_chemistry with an unreachable bound unless maxsteps > 0) of the point
+    ! with parameters p from the state (y, phys) given as ucref_model_until returns it, to
+    ! t_target seconds; no physics update afterwards.  stats = (steps of every attempt,
+    ! attempts, 0...).
+    INTEGER(c_int) FUNCTION ucref_interval(p, y, phys, t_target, maxsteps, y_out, phys_out, stats) &
+            BIND(C, name="ucref_interval")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM), y(nspec+1), phys(8)
+        REAL(c_double), VALUE :: t_target, maxsteps
+        REAL(c_double), INTENT(OUT) :: y_out(nspec+1), phys_out(8)
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
         INTEGER :: flag
+        REAL(dp) :: nsteps, bound
+        stats = 0
+        CALL setup_point(p, flag)
+        IF (flag .lt. 0) THEN
+            ucref_interval = flag
+            RETURN
+        END IF
+        abund(:,1) = y
+        density(1) = phys(1)
+        gasTemp(1) = phys(2)
+        dustTemp(1) = phys(3)
+        av(1) = phys(4)
+        cloudSize = phys(5)
+        phi = phys(6)
+        abstol_factor = phys(7)
+        currentTime = phys(8)
+        timeInYears = currentTime/SECONDS_PER_YEAR
+        targetTime = t_target
+        dstep = 1
+        nsteps = 0.0d0
+        bound = HUGE(1.0d0)
+        IF (maxsteps > 0.0d0) bound = maxsteps
+        CALL bounded_update_chemistry(bound, nsteps, flag, stats(2))
+        stats(1) = INT(MIN(nsteps, 2.0d9))
+        CALL get_state(y_out, phys_out)
+        ucref_interval = flag
+    END FUNCTION ucref_interval
+
+    INTEGER FUNCTION run_model(p, t_stop, maxrec, rec, nrec, stats)
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        REAL(c_double), INTENT(IN) :: t_stop
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec)
+        INTEGER(c_int), INTENT(OUT) :: nrec
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
+        INTEGER :: flag, attempts
         REAL(dp) :: rstats(22), nsteps
         INTEGER :: istats(31)
         stats = 0
@@ -334,11 +425,12 @@ CONTAINS
         nsteps = 0.0d0
         CALL setup_point(p, flag)
         IF (flag .lt. 0) THEN
-            ucref_model = flag
+            run_model = flag
             RETURN
         END IF
         CALL record_row(rec, maxrec, nrec)
-        DO WHILE ((flag .eq. 0) .and. (((endAtFinalDensity) .and. (density(1) < finalDens)) .or. &
+        DO WHILE ((flag .eq. 0) .and. (timeInYears < t_stop) .and. &
+            &(((endAtFinalDensity) .and. (density(1) < finalDens)) .or. &
             &((.not. endAtFinalDensity) .and. (timeInYears < finalTime))))
             currentTimeold=currentTime
             timeInYears=currentTime/SECONDS_PER_YEAR
@@ -350,7 +442,7 @@ CONTAINS
             dstep=1
             currentTime=currentTimeold
             IF (p(P_MAXSTEPS) > 0.0d0) THEN
-                CALL bounded_update_chemistry(p(P_MAXSTEPS), nsteps, flag)
+                CALL bounded_update_chemistry(p(P_MAXSTEPS), nsteps, flag, attempts)
             ELSE
                 CALL updateChemistry(flag)
             END IF
@@ -370,8 +462,8 @@ CONTAINS
             END IF
         END DO
         stats(5) = INT(MIN(nsteps, 2.0d9))
-        ucref_model = flag
-    END FUNCTION ucref_model
+        run_model = flag
+    END FUNCTION run_model
 
     SUBROUTINE record_row(rec, maxrec, nrec)
         INTEGER(c_int), VALUE :: maxrec

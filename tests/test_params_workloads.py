@@ -28,14 +28,15 @@ def test_grid_av_mapping():
 
 def test_oracle_and_product_param_layouts_agree():
     import oracle as orc
-    # the product's extra trailing slot (maxsteps, a work bound) has no oracle counterpart
+    # the trailing slot (maxsteps, a work bound with no reference counterpart) is honoured by
+    # the reference driver too, so both layouts carry it
     assert params.NAMES[-1] == "maxsteps" and params.DEFAULTS["maxsteps"] == 0.0
     assert orc.P_NAMES == [
         "dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall", "finaldens",
         "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp", "freezefactor",
-        "endatfinaldensity", "freefallfactor", "mxstep"]
-    a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0)
-    b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0)
+        "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps"]
+    a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0, maxsteps=5e4)
+    b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0, maxsteps=5e4)
     assert np.array_equal(a, b)
 
 

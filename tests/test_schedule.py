@@ -1,4 +1,6 @@
-"""Host side of the point scheduling order of Network.solve_device (CPU tensors)."""
+"""Host side of the point scheduling order of Network.solve_device.  The plumbing tests run
+on CPU tensors with a stand-in library, so they let solve_device accept CPU tensors
+(gpu.DEVICE_TYPE); with the real setting it refuses them."""
 import numpy as np
 import pytest
 
@@ -56,6 +58,7 @@ def test_ordered_solve_scatters_outputs_back(monkeypatch):
             return 0
 
     monkeypatch.setattr(gpu, "_lib", Stub())
+    monkeypatch.setattr(gpu, "DEVICE_TYPE", "cpu")
     net = gpu.Network.__new__(gpu.Network)
     net.handle, net.neq = 0, neq
     p = torch.zeros((n, nprm), dtype=torch.float64)
@@ -132,6 +135,12 @@ def test_device_solve_refuses_mismatched_buffers(monkeypatch):
     net = gpu.Network.__new__(gpu.Network)
     net.handle, net.neq, net.nspec = 0, 5, 4
     n = 3
+    host = dict(params=torch.zeros((n, 24), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
+                status=torch.zeros(n, dtype=torch.int32), stats=torch.zeros((n, 8), dtype=torch.int32))
+    with pytest.raises(ValueError):          # host tensors never reach the kernel
+        net.solve_device(**host)
+    assert not calls
+    monkeypatch.setattr(gpu, "DEVICE_TYPE", "cpu")
     good = dict(params=torch.zeros((n, 24), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
                 status=torch.zeros(n, dtype=torch.int32), stats=torch.zeros((n, 8), dtype=torch.int32))
     net.solve_device(**good)

@@ -4,6 +4,7 @@
 #include "../../include/uclchem_gpu.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -38,7 +39,9 @@ struct NetHandle {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     int *counter = nullptr;
-    int *cancel = nullptr;              // device flag read by running solves
+    int *cancel = nullptr;              // device word: highest cancelled launch generation
+    int gen = 0;                        // generation of the last launch on this handle
+    int cancel_sent = 0;                // highest generation written to *cancel
     ucg::DevNet *d_dev = nullptr;       // device copy of dev, the kernels' table descriptor
     hipStream_t cancel_stream = nullptr;
     int linear_solver = UCG_LINSOL_SPARSE;
@@ -63,17 +66,17 @@ struct Queue {
     int *h_stats = nullptr;       // [capacity][8]
     void *ws = nullptr;
     int *counter = nullptr;
-    int *cancel = nullptr;
     size_t ws_bytes = 0;
 };
 std::vector<Queue> g_queues;
+std::mutex g_cancel_mu;   // orders the cancel-word writes of concurrent ucg_cancel calls
 
 void queue_release(Queue &q)
 {
     for (void *p : {(void *)q.h_params, (void *)q.h_ctrl, (void *)q.h_done, (void *)q.h_y, (void *)q.h_status,
                     (void *)q.h_stats})
         if (p) (void)hipHostFree(p);
-    for (void *p : {q.ws, (void *)q.counter, (void *)q.cancel})
+    for (void *p : {q.ws, (void *)q.counter})
         if (p) (void)hipFree(p);
     if (q.stream) (void)hipStreamDestroy(q.stream);
     q = Queue();
@@ -199,6 +202,30 @@ int alloc_dev(DevBuf &b, size_t n)
     return 0;
 }
 
+// A caller's buffer for a device-pointer entry point: device (or managed) memory of the
+// library's GPU with at least `bytes` bytes between p and the end of its allocation.  Host
+// pointers, pinned host memory and other GPUs' memory are refused (the kernels would fault).
+int check_dev_buf(const void *p, size_t bytes, const char *name, bool nullable = false)
+{
+    if (!p) return nullable ? 0 : fail(UCG_E_ARG, std::string("null device buffer ") + name);
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(UCG_E_ARG, std::string(name) + " is not device memory");
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != g_device)
+        return fail(UCG_E_ARG, std::string(name) + " is not device memory of GPU " + std::to_string(g_device));
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) == hipSuccess) {
+        if ((const char *)p + bytes > (const char *)base + size)
+            return fail(UCG_E_ARG, std::string(name) + " is smaller than the batch needs");
+    } else {
+        (void)hipGetLastError();
+    }
+    return 0;
+}
+
 int check_params(int n, const double *params)
 {
     for (int p = 0; p < n; p++) {
@@ -249,9 +276,15 @@ int launch_kernel(const ucg::DevNet &N, const ucg::DevNet *dN, const ucg::SolveA
     return 0;
 }
 
+// Optional single-interval inputs and outputs of a solve launch (ucg_interval).
+struct IntervalArgs {
+    const double *phys_in = nullptr, *t_target = nullptr;
+    double *phys_out = nullptr;
+};
+
 int launch_solve(NetHandle &h, int n, const double *params, const double *y_init, double *y_out, int max_rec,
                  double *rec, int *n_rec, int *status, int *stats, double *t_start, double *t_done, int lu_bits,
-                 int n_slots, hipStream_t stream, double *timers = nullptr)
+                 int n_slots, hipStream_t stream, double *timers = nullptr, const IntervalArgs &iv = IntervalArgs())
 {
     const ucg::DevNet &N = h.dev;
     if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
@@ -265,8 +298,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     // reallocate between launches
     int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
     if (rc) return rc;
-    hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, h.counter, h.cancel,
-                       status, n);
+    hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, h.counter, status, n);
     ucg::SolveArgs A = {};
     A.n_points = n;
     A.params = params;
@@ -288,6 +320,10 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.ws_jac_bytes = jac;
     A.counter = h.counter;
     A.cancel = h.cancel;
+    A.gen = ++h.gen;
+    A.phys_in = iv.phys_in;
+    A.phys_out = iv.phys_out;
+    A.t_target = iv.t_target;
     return launch_kernel(N, h.d_dev, A, n_slots, lu_bits, stream);
 }
 
@@ -328,6 +364,84 @@ int ucg_init(int device)
 }
 
 static int load_tables(const ucg_network_desc *d, NetHandle &h);
+
+// Every index table of the network is range-checked on the host before anything is
+// uploaded: species indices in [0, nspec) (or -1 where a slot may be empty), reaction
+// indices in [0, nreac), pointer arrays non-decreasing from 0 to their list's length.
+static int check_tables(const ucg_network_desc *d)
+{
+    auto bad = [](const std::string &what) { return fail(UCG_E_ARG, "network table " + what + " out of range"); };
+    const long R = d->nreac, Q = d->neq, S = d->nspec;
+    if (R <= 0 || S <= 0 || d->nice < 0 || d->nsurfl < 0 || d->nbulkl < 0 || d->nfp < 0 || d->nswap < 0 ||
+        d->nloss < 0 || d->nprod < 0 || d->nnz < 0 || d->ncontrib < 0 || d->naff < 0 || d->nsw < 0)
+        return fail(UCG_E_ARG, "negative network table size");
+    auto in = [](const int *a, long n, long lo, long hi) {   // every a[i] in [lo, hi)
+        if (n > 0 && !a) return false;
+        for (long i = 0; i < n; i++)
+            if (a[i] < lo || a[i] >= hi) return false;
+        return true;
+    };
+    auto mono = [](const int *a, long n, long last) {   // a[0..n] non-decreasing from 0 to last
+        if (!a || a[0] != 0 || a[n] != last) return false;
+        for (long i = 0; i < n; i++)
+            if (a[i + 1] < a[i]) return false;
+        return true;
+    };
+    if (!in(d->rtype, R, 0, ucg::NTYPES)) return bad("rtype");
+    if (!in(d->re, 3 * R, -1, S)) return bad("re");
+    if (!in(d->pr, 4 * R, -1, S)) return bad("pr");
+    if (!in(d->ice_kind, R, 0, 3)) return bad("ice_kind");
+    if (!in(d->n_d, R, 0, ucg::MAXFAC + 1)) return bad("n_d");
+    if (!in(d->ops, (long)ucg::MAXOPS * R, ucg::OP_DIVSM, S)) return bad("ops");
+    if (!d->type_ranges) return bad("type_ranges");
+    for (int t = 0; t < ucg::NTYPES; t++) {
+        const int a = d->type_ranges[2 * t], b = d->type_ranges[2 * t + 1];
+        if (!((a == -1 && b == -1) || (0 <= a && a <= b && b < R))) return bad("type_ranges");
+        if (t == ucg::T_FREEZE && a >= 0)
+            for (int k = a; k <= b; k++)
+                if (d->re[3 * k] < 0) return bad("freeze-out reactant");
+    }
+    if (!in(d->surface_list, d->nsurfl, 0, S)) return bad("surface_list");
+    if (!in(d->bulk_list, d->nbulkl, 0, S)) return bad("bulk_list");
+    if (!in(d->bulk_partner_of_surface, Q, -1, S)) return bad("bulk_partner_of_surface");
+    if (!in(d->surface_partner_of_bulk, Q, -1, S)) return bad("surface_partner_of_bulk");
+    if (!in(d->freeze_partners, d->nfp, 0, R)) return bad("freeze_partners");
+    for (long q = 0; q < d->nfp; q++)
+        if (d->re[3 * d->freeze_partners[q]] < 0) return bad("freeze-partner reactant");
+    if (!in(d->is_surface, Q, 0, 2) || !in(d->is_bulk, Q, 0, 2)) return bad("is_surface/is_bulk");
+    if (!in(d->swap_list, d->nswap, 0, R)) return bad("swap_list");
+    if (!in(d->jnz_row, d->nnz, 0, Q) || !in(d->jnz_col, d->nnz, 0, Q) || !mono(d->jnz_ptr, d->nnz, d->ncontrib))
+        return bad("Jacobian pattern");
+    if (!in(d->jc_k, d->ncontrib, 0, R) || !d->jc_m || !d->jc_c) return bad("Jacobian contributions");
+    if (!mono(d->aff_ptr, R, d->naff) || !in(d->aff_sp, d->naff, 0, Q) || (d->naff > 0 && !d->aff_cf))
+        return bad("affected-species lists");
+    if (!mono(d->sw_ptr, Q, d->nsw) || !in(d->sw_k, d->nsw, 0, R) || (d->nsw > 0 && !d->sw_c))
+        return bad("totalSwap lists");
+    if (!mono(d->loss_ptr, S, d->nloss) || !in(d->loss_idx, d->nloss, 0, R) || !mono(d->prod_ptr, S, d->nprod) ||
+        !in(d->prod_idx, d->nprod, 0, R))
+        return bad("production/loss lists");
+    if (!in(d->nfac, R, 0, 9) || !d->fac) return bad("factor lists");
+    for (long k = 0; k < R; k++)
+        for (int m = 0; m < d->nfac[k]; m++) {
+            const int c = d->fac[ucg::MAXFAC * k + m];
+            if (c < ucg::FAC_D || c >= Q) return bad("factor code");
+        }
+    for (long q = 0; q < d->ncontrib; q++)
+        if (d->jc_m[q] < 0 || d->jc_m[q] >= d->nfac[d->jc_k[q]] || d->jc_c[q] < -8 || d->jc_c[q] > 7)
+            return bad("Jacobian contribution");
+    for (int i = 0; i < 25; i++)
+        if (d->special_species[i] < 0 || d->special_species[i] >= Q) return bad("special species index");
+    for (int i = 0; i < 9; i++)
+        if (d->special_reactions[i] < 0 || d->special_reactions[i] >= R) return bad("special reaction index");
+    // the device rate evaluator pairs ranges element-wise (rates.f90:243-279, 104-224)
+    const int *tr = d->type_ranges;
+    auto len = [&](int t) { return tr[2 * t] < 0 ? 0 : tr[2 * t + 1] - tr[2 * t] + 1; };
+    if (len(ucg::T_LH) != len(ucg::T_LHDES)) return fail(UCG_E_ARG, "LH and LHDES ranges differ in length");
+    if (len(ucg::T_ER) != len(ucg::T_ERDES)) return fail(UCG_E_ARG, "ER and ERDES ranges differ in length");
+    for (int t : {ucg::T_DESOH2, ucg::T_DESCR, ucg::T_DEUVCR, ucg::T_THERM})
+        if (len(t) > 1 && len(t) < d->nfp) return fail(UCG_E_ARG, "desorption range shorter than freeze partners");
+    return 0;
+}
 
 // Every index of the sparse Newton plan is range-checked here, on the host, so that no
 // table handed in by a caller can make the device address outside its buffers.
@@ -385,11 +499,13 @@ static int check_plan(const ucg_network_desc *d)
 int ucg_network_load(const ucg_network_desc *d, int *handle)
 {
     std::lock_guard<std::mutex> lk(g_mu);
-    int rc = init_locked(-1);
-    if (rc) return rc;
     if (!d || !handle) return fail(UCG_E_ARG, "null argument");
     if (d->neq != d->nspec + 1 || d->neq > ucg::NT * ucg::EPT)
         return fail(UCG_E_ARG, "network too large: NEQ must be <= " + std::to_string(ucg::NT * ucg::EPT));
+    // tables are validated on the host before the device is touched
+    int rc = check_tables(d);
+    if (rc || (rc = check_plan(d))) return rc;
+    if ((rc = init_locked(-1))) return rc;
     NetHandle h;
     h.used = true;
     if ((rc = load_tables(d, h))) {   // release whatever was uploaded before the failure
@@ -518,6 +634,7 @@ int ucg_network_load_file(const char *path, int *handle)
     return ucg_network_load(&d, handle);
 }
 
+// Uploads a network whose tables check_tables and check_plan have accepted.
 static int load_tables(const ucg_network_desc *d, NetHandle &h)
 {
     int rc;
@@ -618,7 +735,6 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     UP(jc_k, (size_t)d->ncontrib); UP(jc_m, (size_t)d->ncontrib); UP(jc_c, (size_t)d->ncontrib);
     UP(aff_ptr, R + 1); UP(aff_sp, (size_t)d->naff); UP(aff_cf, (size_t)d->naff);
     UP(sw_ptr, Q + 1); UP(sw_k, (size_t)d->nsw); UP(sw_c, (size_t)d->nsw);
-    if ((rc = check_plan(d))) return rc;
     {   // the Jacobian sums d totalSwap/dy per bulk species without ordering: one BULKSWAP each
         std::vector<char> seen((size_t)d->neq, 0);
         for (int q = 0; q < d->nswap; q++) {
@@ -726,22 +842,9 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     N.nf = ss[7]; N.np = ss[8]; N.nmg = ss[9]; N.nmgx = ss[10]; N.nna = ss[11]; N.nli = ss[12];
     N.npah = ss[13]; N.nsx = ss[14]; N.nsix = ss[15]; N.nclx = ss[16]; N.nd = ss[17]; N.n18o = ss[18];
     N.n15n = ss[19]; N.n13c = ss[20]; N.nelec = ss[21]; N.nco = ss[22]; N.nsurface = ss[23]; N.nbulk = ss[24];
-    for (int i = 0; i < 25; i++)
-        if (ss[i] < 0 || ss[i] >= d->neq) return fail(UCG_E_ARG, "special species index out of range");
     const int *sr = d->special_reactions;
     N.r_h2form_ct = sr[0]; N.r_h2form_er = sr[1]; N.r_h2form_erdes = sr[2]; N.r_hfreeze = sr[3];
     N.r_efreeze = sr[4]; N.r_h2freeze = sr[5]; N.r_h2_hv = sr[6]; N.r_co_hv = sr[7]; N.r_c_hv = sr[8];
-    for (int i = 0; i < 9; i++)
-        if (sr[i] < 0 || sr[i] >= d->nreac) return fail(UCG_E_ARG, "special reaction index out of range");
-    // the device rate evaluator pairs ranges element-wise (rates.f90:243-279, 104-224)
-    {
-        const int *tr = d->type_ranges;
-        auto len = [&](int t) { return tr[2 * t] < 0 ? 0 : tr[2 * t + 1] - tr[2 * t] + 1; };
-        if (len(ucg::T_LH) != len(ucg::T_LHDES)) return fail(UCG_E_ARG, "LH and LHDES ranges differ in length");
-        if (len(ucg::T_ER) != len(ucg::T_ERDES)) return fail(UCG_E_ARG, "ER and ERDES ranges differ in length");
-        for (int t : {ucg::T_DESOH2, ucg::T_DESCR, ucg::T_DEUVCR, ucg::T_THERM})
-            if (len(t) > 1 && len(t) < d->nfp) return fail(UCG_E_ARG, "desorption range shorter than freeze partners");
-    }
     HIPCHK(hipMalloc(&h.counter, sizeof(int)));
     h.allocs.push_back(h.counter);
     HIPCHK(hipMalloc(&h.cancel, sizeof(int)));
@@ -777,15 +880,24 @@ int ucg_network_free(int handle)
 
 int ucg_cancel(int handle)
 {
-    NetHandle *h;
+    // g_cancel_mu orders concurrent cancels; g_mu is held only to read the handle, so that a
+    // cancel does not wait behind a host-buffer solve that holds it for its whole run
+    std::lock_guard<std::mutex> ck(g_cancel_mu);
+    int *word;
+    hipStream_t stream;
+    int gen;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        h = get(handle);
+        NetHandle *h = get(handle);
         if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+        if (h->gen <= h->cancel_sent) return 0;   // nothing launched since the last cancel
+        word = h->cancel;
+        stream = h->cancel_stream;
+        gen = h->gen;
+        h->cancel_sent = gen;
     }
-    static const int one = 1;
-    HIPCHK(hipMemcpyAsync(h->cancel, &one, sizeof(int), hipMemcpyHostToDevice, h->cancel_stream));
-    HIPCHK(hipStreamSynchronize(h->cancel_stream));
+    HIPCHK(hipMemcpyAsync(word, &gen, sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     return 0;
 }
 
@@ -817,7 +929,22 @@ int ucg_solve_batch_device(int handle, int n, const double *params, const double
     std::lock_guard<std::mutex> lk(g_mu);
     NetHandle *h = get(handle);
     if (!h) return fail(UCG_E_HANDLE, "bad network handle");
-    if (!params || !y_out || !status || !stats) return fail(UCG_E_ARG, "null buffer");
+    if (n < 0) return fail(UCG_E_ARG, "n must not be negative");
+    if (n == 0) return 0;
+    const size_t un = (size_t)n, neq = (size_t)h->dev.neq, rw = 4 + (size_t)h->dev.nspec;
+    if (!rec) max_rec = 0;
+    if (max_rec < 0) return fail(UCG_E_ARG, "max_rec must not be negative");
+    int rc;
+    if ((rc = check_dev_buf(params, un * UCG_NPARAM * sizeof(double), "params")) ||
+        (rc = check_dev_buf(y_init, un * neq * sizeof(double), "y_init", true)) ||
+        (rc = check_dev_buf(y_out, un * neq * sizeof(double), "y_out")) ||
+        (rc = check_dev_buf(rec, un * (size_t)max_rec * rw * sizeof(double), "rec", max_rec == 0)) ||
+        (rc = check_dev_buf(n_rec, un * sizeof(int), "n_rec", true)) ||
+        (rc = check_dev_buf(status, un * sizeof(int), "status")) ||
+        (rc = check_dev_buf(stats, un * 8 * sizeof(int), "stats")) ||
+        (rc = check_dev_buf(t_start, un * sizeof(double), "t_start", true)) ||
+        (rc = check_dev_buf(t_done, un * sizeof(double), "t_done", true)))
+        return rc;
     return launch_solve(*h, n, params, y_init, y_out, max_rec, rec, n_rec, status, stats, t_start, t_done,
                         lu_bits, n_slots, (hipStream_t)stream);
 }
@@ -875,6 +1002,44 @@ int ucg_solve_batch(int handle, int n, const double *params, const double *y_ini
     HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
     if (max_rec > 0) HIPCHK(hipMemcpy(rec, drec.p, (size_t)n * max_rec * (4 + ns) * sizeof(double), hipMemcpyDeviceToHost));
     if (n_rec) HIPCHK(hipMemcpy(n_rec, dnrec.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int ucg_interval(int handle, int n, const double *params, const double *phys, const double *y,
+                 const double *t_target, double *y_out, double *phys_out, int *status, int *stats, int lu_bits)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (n <= 0 || !params || !phys || !y || !t_target || !y_out || !phys_out || !status || !stats)
+        return fail(UCG_E_ARG, "bad arguments");
+    int rc = check_params(n, params);
+    if (rc) return rc;
+    for (int p = 0; p < n; p++) {
+        const double *ph = phys + (size_t)p * 8;
+        if (!(ph[0] > 0) || !(ph[1] > 0) || !(ph[2] > 0) || !(ph[4] > 0) || !(ph[6] > 0) || !(ph[7] >= 0) ||
+            !(t_target[p] >= ph[7]) || !std::isfinite(t_target[p]))
+            return fail(UCG_E_PARAM, "invalid interval state for point " + std::to_string(p));
+    }
+    const size_t neq = h->dev.neq;
+    DevBuf dp, dph, dy0, dtt, dy, dpo, dst, dstat;
+    if ((rc = to_dev(dp, params, (size_t)n * UCG_NPARAM)) || (rc = to_dev(dph, phys, (size_t)n * 8)) ||
+        (rc = to_dev(dy0, y, (size_t)n * neq)) || (rc = to_dev(dtt, t_target, (size_t)n)) ||
+        (rc = alloc_dev<double>(dy, (size_t)n * neq)) || (rc = alloc_dev<double>(dpo, (size_t)n * 8)) ||
+        (rc = alloc_dev<int>(dst, (size_t)n)) || (rc = alloc_dev<int>(dstat, (size_t)n * 8)))
+        return rc;
+    IntervalArgs iv;
+    iv.phys_in = (const double *)dph.p;
+    iv.t_target = (const double *)dtt.p;
+    iv.phys_out = (double *)dpo.p;
+    rc = launch_solve(*h, n, (const double *)dp.p, (const double *)dy0.p, (double *)dy.p, 0, nullptr, nullptr,
+                      (int *)dst.p, (int *)dstat.p, nullptr, nullptr, lu_bits, 0, nullptr, nullptr, iv);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(phys_out, dpo.p, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
@@ -1062,12 +1227,10 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     q.ws_bytes = sl.stride * (size_t)n_slots;
     if (e == hipSuccess) e = hipMalloc(&q.ws, q.ws_bytes);
     if (e == hipSuccess) e = hipMalloc(&q.counter, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&q.cancel, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking);
-    // stream-ordered clears: other queues' kernels may be running, and nothing here may wait
+    // stream-ordered clear: other queues' kernels may be running, and nothing here may wait
     // for them (a device-wide synchronisation would wait for kernels that only end on close)
     if (e == hipSuccess) e = hipMemsetAsync(q.counter, 0, sizeof(int), q.stream);
-    if (e == hipSuccess) e = hipMemsetAsync(q.cancel, 0, sizeof(int), q.stream);
     ucg::SolveArgs A = {};
     void *d_params = nullptr, *d_ctrl = nullptr, *d_done = nullptr, *d_y = nullptr, *d_status = nullptr,
          *d_stats = nullptr;
@@ -1093,7 +1256,8 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     A.ws_mat_bytes = sl.mat;
     A.ws_jac_bytes = sl.jac;
     A.counter = q.counter;
-    A.cancel = q.cancel;
+    A.cancel = h->cancel;   // ucg_cancel on the network stops its queues too
+    A.gen = ++h->gen;
     A.q_avail = (const int *)d_ctrl;
     A.q_closed = (const int *)d_ctrl + 1;
     A.q_done = (int *)d_done;

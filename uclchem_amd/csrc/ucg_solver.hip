@@ -1682,6 +1682,14 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, const Lds L, const LUT 
 
 // ---------------------------------------------------------------- BDF ops policy
 
+// A launch of generation gen stops once the host has raised the handle's cancel word to gen
+// or beyond (launches are numbered in issue order), so a cancel also reaches launches that
+// were issued before it but have not started yet, and no reset is needed between launches.
+__device__ __forceinline__ bool cancel_requested(const int *cancel, int gen)
+{
+    return cancel && __hip_atomic_load(cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) >= gen;
+}
+
 template <typename LUT>
 struct GpuOps {
     const DevNet &N;
@@ -1695,11 +1703,9 @@ struct GpuOps {
     bool equilibrate = false;   // factor W M W^-1 (W = diag(ewt)) instead of M (dense form only)
     bool sparse = true;         // static-pattern sparse plan (else dense LU of the full matrix)
     double *Jsave = nullptr;    // saved J-form values of the sparse plan (workspace)
-    const int *cancel = nullptr;  // device flag the host may raise while the kernel runs
-    __device__ bool cancelled() const
-    {
-        return cancel && __hip_atomic_load(cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
-    }
+    const int *cancel = nullptr;  // highest cancelled launch generation, raised by the host while kernels run
+    int gen = 0;                  // this launch's generation
+    __device__ bool cancelled() const { return cancel_requested(cancel, gen); }
 
     __device__ GpuOps(const DevNet &n_, Lds &l_, LUT *m_) : N(n_), L(l_), M(m_), neq(n_.neq) {}
 
@@ -1972,8 +1978,13 @@ struct SolveArgs {
     size_t ws_mat_bytes;      // (ws_jac_bytes, sparse plan only), then rate[nreac]
     size_t ws_jac_bytes;
     int *counter;             // work queue head
-    const int *cancel;        // nullable device flag: nonzero stops the batch (points not done get
-                              // STATUS_CANCELLED)
+    const int *cancel;        // nullable: the highest cancelled launch generation (points this launch
+    int gen;                  // has not finished get STATUS_CANCELLED once it reaches gen)
+    // single-interval mode (phys_in non-null): each point is one updateChemistry call from the
+    // state phys_in/y_init to t_target, with no physics update and no output rows
+    const double *phys_in;    // [n][8]: density, Tgas, Tdust, Av, cloudSize, phi, abstol_factor, time [s]
+    double *phys_out;         // [n][8]: the same after the interval
+    const double *t_target;   // [n] seconds
     int equilibrate;          // factor the error-weight-equilibrated Newton matrix (dense form)
     int sparse;               // 1: static-pattern sparse plan, 0: dense LU
     // request-queue mode (all null/0 for a batch): n_points is the queue capacity and the
@@ -2048,14 +2059,23 @@ __device__ void lds_layout(const DevNet &N, Lds &L, double *lds_raw, double *rat
 
 // Resets the work counter and the cancel flag and marks every point "not run" before a
 // solve; a kernel rather than memsets so that the launch sequence can be graph-captured.
-__global__ void solve_init_kernel(int *counter, int *cancel, int *status, int n)
+__global__ void solve_init_kernel(int *counter, int *status, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        *counter = 0;
-        if (cancel) *cancel = 0;
-    }
+    if (i == 0) *counter = 0;
     if (i < n) status[i] = STATUS_CANCELLED;
+}
+
+// The checks the host API makes of a parameter vector, repeated on the device for callers of
+// the device-pointer entry points: a point that fails them is not integrated and reports
+// UCG_E_PARAM (its tempindx would index the hot-core tables out of bounds).
+__device__ inline bool params_valid(const LDS double *q)
+{
+    const double model = q[P_MODEL], ti = q[P_TEMPINDX];
+    bool ok = q[P_DENS] > 0.0 && q[P_TEMP] > 0.0 && q[P_RELTOL] > 0.0 && q[P_ABSMIN] >= 0.0 &&
+              q[P_MXSTEP] >= 1.0 && q[P_ROUT] > q[P_RIN] && q[P_MAXSTEPS] >= 0.0 && (model == 0.0 || model == 1.0);
+    if (model == 1.0) ok = ok && ti >= 1.0 && ti < 7.0;
+    return ok;
 }
 
 template <typename LUT>
@@ -2072,7 +2092,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
 
     for (;;) {
         if (tid() == 0) {
-            const bool stop = A.cancel && __hip_atomic_load(A.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+            const bool stop = cancel_requested(A.cancel, A.gen);
             int p = stop ? A.n_points : atomicAdd(A.counter, 1);
             if (A.q_avail && p < A.n_points) {   // request queue: wait until request p is posted
                 const uint64_t w0 = wall_clock64();
@@ -2080,8 +2100,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                     if (__hip_atomic_load(A.q_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > p) break;
                     const bool closed = __hip_atomic_load(A.q_closed, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
                     if (closed && __hip_atomic_load(A.q_avail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) > p) break;
-                    const bool cancelled =
-                        A.cancel && __hip_atomic_load(A.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0;
+                    const bool cancelled = cancel_requested(A.cancel, A.gen);
                     if (closed || cancelled || (long long)(wall_clock64() - w0) > A.q_idle_ticks) {
                         p = A.n_points;   // every path of this loop ends: post, close, cancel or idle limit
                         break;
@@ -2102,7 +2121,20 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         for (int i = tid(); i < P_NPARAM; i += NT) L.prm[i] = pin[i];
         __syncthreads();
         const LDS double *prm = L.prm;
-        if (tid() == 0) {
+        const bool interval_mode = A.phys_in != nullptr;
+        const bool valid = params_valid(prm);
+        if (interval_mode) {   // the caller's state at the interval start
+            if (tid() == 0) {
+                const double *ph = A.phys_in + (size_t)pidx * 8;
+                L.sc[S_DENS] = ph[0];
+                L.sc[S_TGAS] = ph[1];
+                L.sc[S_TDUST] = ph[2];
+                L.sc[S_AV] = ph[3];
+                L.sc[S_CS] = ph[4];
+                L.sc[S_PHI] = ph[5];
+                L.sc[S_ABSFAC] = ph[6];
+            }
+        } else if (tid() == 0) {
             double cs = (prm[P_ROUT] - prm[P_RIN]) * PC;
             L.sc[S_CS] = cs;
             L.sc[S_TGAS] = prm[P_TEMP];
@@ -2127,10 +2159,12 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         o.Jsave = (double *)(slot + A.ws_mat_bytes);
         o.cancel = A.cancel;
         o.equilibrate = !o.sparse && A.equilibrate != 0;
+        o.gen = A.gen;
         bdf::Stats st;
-        int row = 0, nint = 0, nretry = 0, flag = 0;
-        record_row(N, L, A, pidx, row++, 0.0);
-        double t = 0.0, tyears = 0.0;
+        int row = 0, nint = 0, nretry = 0, flag = valid ? 0 : -1;   // -1: UCG_E_PARAM
+        if (!interval_mode) record_row(N, L, A, pidx, row++, 0.0);
+        double t = interval_mode ? A.phys_in[(size_t)pidx * 8 + 7] : 0.0;
+        double tyears = t / SECONDS_PER_YEAR;
         const int model = (int)prm[P_MODEL];
         const double reltol = prm[P_RELTOL], absmin = prm[P_ABSMIN];
         const int mxstep = (int)prm[P_MXSTEP];
@@ -2143,14 +2177,16 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         uint64_t t_rates = 0;
         const uint64_t t_point0 = wall_clock64();
         const uint64_t c_point0 = clock64();   // shader clock, for the effective frequency
-        while ((end_dens && L.sc[S_DENS] < prm[P_FINALDENS]) || (!end_dens && tyears < prm[P_FINALTIME])) {
+        while (!flag && (interval_mode ? nint == 0
+                                       : ((end_dens && L.sc[S_DENS] < prm[P_FINALDENS]) ||
+                                          (!end_dens && tyears < prm[P_FINALTIME])))) {
             if (nint >= MAX_INTERVALS || stuck > 10) {
                 flag = -5;
                 break;
             }
             const double t_before = t;
             tyears = t / SECONDS_PER_YEAR;
-            double target = next_target_dev(model, tyears);
+            double target = interval_mode ? A.t_target[pidx] : next_target_dev(model, tyears);
             const double orig = target;
             int loops = 0;
             while (t < target && loops < 10) {   // updateChemistry retry loop (chemistry.f90:164-223)
@@ -2195,6 +2231,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
             }
             if (flag) break;
             nint++;
+            if (interval_mode) break;   // updateChemistry ends here; the physics update is the caller's
             stuck = (t > t_before) ? 0 : stuck + 1;
             tyears = target / SECONDS_PER_YEAR;
             // physics update (physics-core.f90:75-88, hotcore.f90:78-90)
@@ -2227,6 +2264,11 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         for (int i = tid(); i < neq; i += NT) A.y_out[(size_t)pidx * neq + i] = L.y[i];
         if (tid() == 0) {
             A.status[pidx] = flag;
+            if (A.phys_out) {
+                double *ph = A.phys_out + (size_t)pidx * 8;
+                ph[0] = L.sc[S_DENS]; ph[1] = L.sc[S_TGAS]; ph[2] = L.sc[S_TDUST]; ph[3] = L.sc[S_AV];
+                ph[4] = L.sc[S_CS]; ph[5] = L.sc[S_PHI]; ph[6] = L.sc[S_ABSFAC]; ph[7] = t;
+            }
             if (A.n_rec) A.n_rec[pidx] = row < A.max_rec ? row : A.max_rec;
             int *s = A.stats + (size_t)pidx * 8;
             s[0] = st.nst; s[1] = st.nfe; s[2] = st.nje; s[3] = st.nni;

@@ -18,6 +18,7 @@ MODULE uclchem_gpu
               ucg_default_params, ucg_cancel, ucg_queue_slot_bytes, ucg_queue_open, ucg_queue_submit, &
               ucg_queue_test, ucg_queue_result, ucg_queue_close
     INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 24
+    INTEGER, PARAMETER, PUBLIC :: UCG_E_ARG = -102   ! an array does not match the batch or network
     INTEGER, PARAMETER, PUBLIC :: UCGP_DENS = 1, UCGP_TEMP = 2, UCGP_ZETA = 3, UCGP_RADFIELD = 4, &
         UCGP_BASEAV = 5, UCGP_ROUT = 6, UCGP_RIN = 7, UCGP_FINALTIME = 8, UCGP_FREEFALL = 9, &
         UCGP_FINALDENS = 10, UCGP_RELTOL = 11, UCGP_ABSFAC = 12, UCGP_ABSMIN = 13, UCGP_MODEL = 14, &
@@ -134,10 +135,25 @@ CONTAINS
         REAL(C_DOUBLE), INTENT(OUT), OPTIONAL, TARGET, CONTIGUOUS :: rec(:, :, :)
         INTEGER(C_INT), INTENT(OUT), OPTIONAL :: n_rec(:)
         INTEGER, INTENT(IN), OPTIONAL :: lu_bits
-        INTEGER(C_INT) :: n, bits, max_rec
+        INTEGER(C_INT) :: n, bits, max_rec, nspec, neq, nreac
         INTEGER(C_INT), ALLOCATABLE :: nrec_local(:)
         TYPE(C_PTR) :: p_init, p_rec
         n = SIZE(params, 2)
+        ! every array must hold what the library reads or writes for n points
+        rc = ucg_network_info(INT(handle, C_INT), nspec, neq, nreac)
+        IF (rc /= 0) RETURN
+        rc = UCG_E_ARG
+        IF (SIZE(params, 1) /= UCG_NPARAM .OR. SIZE(abund_out, 1) /= neq .OR. SIZE(abund_out, 2) < n) RETURN
+        IF (SIZE(status) < n .OR. SIZE(stats, 1) /= 8 .OR. SIZE(stats, 2) < n) RETURN
+        IF (PRESENT(abund_init)) THEN
+            IF (SIZE(abund_init, 1) /= neq .OR. SIZE(abund_init, 2) < n) RETURN
+        END IF
+        IF (PRESENT(rec)) THEN
+            IF (SIZE(rec, 1) /= 4 + nspec .OR. SIZE(rec, 3) < n) RETURN
+        END IF
+        IF (PRESENT(n_rec)) THEN
+            IF (SIZE(n_rec) < n) RETURN
+        END IF
         bits = 64
         IF (PRESENT(lu_bits)) bits = lu_bits
         p_init = C_NULL_PTR

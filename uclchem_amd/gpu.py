@@ -15,6 +15,9 @@ from . import makerates_hip as mh
 from .params import INDEX, NPARAM
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libuclchem_gpu.so")
+# torch device type of the buffers solve_device hands to the kernel (host tensors would be
+# dereferenced by the GPU as device addresses)
+DEVICE_TYPE = "cuda"
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
@@ -159,6 +162,25 @@ class Network:
         _check(rc, "ucg_solve_batch")
         return dict(y=y_out, rec=rec, n_rec=n_rec, status=status, stats=stats)
 
+    def interval(self, params, phys, y, t_target, lu_bits: int = 64):
+        """One updateChemistry interval per point (ucg_interval): from the state (phys, y) —
+        phys rows (density, gasTemp, dustTemp, Av, cloudSize, phi, abstol_factor, time [s]) —
+        to t_target seconds.  Returns dict(y, phys, status, stats)."""
+        params = np.atleast_2d(params)
+        n = params.shape[0]
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        y = _rows(y, n, self.neq, "y")
+        tt = np.ascontiguousarray(np.broadcast_to(np.asarray(t_target, dtype=np.float64), (n,)))
+        y_out = np.zeros((n, self.neq))
+        phys_out = np.zeros((n, 8))
+        status = np.zeros(n, dtype=np.int32)
+        stats = np.zeros((n, 8), dtype=np.int32)
+        _check(lib().ucg_interval(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(y), _p(tt),
+                                  _p(y_out), _p(phys_out), _p(status), _p(stats), ctypes.c_int(lu_bits)),
+               "ucg_interval")
+        return dict(y=y_out, phys=phys_out, status=status, stats=stats)
+
     def solve_profiled(self, params, lu_bits=64, n_slots=0):
         params = np.atleast_2d(params)
         n = params.shape[0]
@@ -213,6 +235,8 @@ class Network:
         """The kernel indexes these buffers by point with the sizes below; a wrong shape,
         dtype or layout would make it read or write outside them, so refuse it here."""
         import torch
+        if params.device.type != DEVICE_TYPE:
+            raise ValueError(f"solve_device takes {DEVICE_TYPE} tensors, got params on {params.device}")
         if params.dim() != 2 or params.shape[1] != NPARAM:
             raise ValueError(f"params must be [n, {NPARAM}]")
         n = params.shape[0]

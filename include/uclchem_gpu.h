@@ -134,6 +134,20 @@ int ucg_solve_batch_device(int handle, int n, const double *params, const double
                            int max_rec, double *rec, int *n_rec, int *status, int *stats,
                            double *t_start, double *t_done, int lu_bits, int n_slots, void *stream);
 
+/* ucg_solve_batch_device that allocates nothing: the slot workspaces and the work counter
+ * come from a fixed device array inside the library (80 MiB per GPU, set up by ucg_init and
+ * held until the process ends), so the call makes no hipMalloc/hipFree and no allocating
+ * library call.  No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Arguments and results as ucg_solve_batch_device (results are identical); n_slots = 0 takes
+ * as many resident workgroups as fit the array, up to two per compute unit; an explicit
+ * n_slots beyond what fits, or a network one slot of which does not fit, is refused with
+ * UCG_E_ARG.  Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams or threads) race, so order them on one stream. */
+int ucg_solve_batch_device_noalloc(int handle, int n, const double *params, const double *y_init, double *y_out,
+                                   int max_rec, double *rec, int *n_rec, int *status, int *stats,
+                                   double *t_start, double *t_done, int lu_bits, int n_slots, void *stream);
+
 /* Stop every solve launched on a handle before this call (ucg_solve_batch_device launches,
  * including ones queued behind other work, and the handle's request queues): workgroups
  * finish their current integrator step and take no further points; unfinished points

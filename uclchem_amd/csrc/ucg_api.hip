@@ -12,9 +12,17 @@
 #include <string>
 #include <vector>
 
+// Fixed device workspace of ucg_solve_batch_device_noalloc: the work counter (first 256 bytes)
+// and the slot workspaces.  Each GPU holds its own copy from the library's first use on it
+// until the process ends.  Host code never uses the name itself (on the host it is only a
+// stand-in, not a device address): ucg_init records the device address of each GPU's copy.
+#define UCG_SCRATCH_BYTES (80u << 20)
+static __device__ __align__(256) unsigned char scratch[UCG_SCRATCH_BYTES];
+
 namespace {
 
 std::mutex g_mu;
+std::vector<void *> g_scratch;   // per device: the device address of that GPU's scratch
 std::string g_err;
 bool g_inited = false;
 int g_device = -1;
@@ -136,6 +144,13 @@ int init_locked(int device)
         for (int k = 0; k < 6; k++) t.ya[j][k] = sco[j + 7 * k];
     for (int j = 0; j < 7; j++) spline_host(t.nh2, t.ya[j], 6, t.y2a[j]);
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(ucg::c_photo), &t, sizeof(t)));
+    if ((int)g_scratch.size() < ndev) g_scratch.resize(ndev, nullptr);
+    if (!g_scratch[device]) {   // this device's copy of the fixed workspace (device current)
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(UCG_E_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch[device] = ptr;
+    }
     g_device = device;
     g_inited = true;
     return 0;
@@ -282,9 +297,16 @@ struct IntervalArgs {
     double *phys_out = nullptr;
 };
 
+// A caller-provided workspace for a solve launch: slot workspaces and the work counter.
+struct FixedWs {
+    void *ws = nullptr;
+    int *counter = nullptr;
+};
+
 int launch_solve(NetHandle &h, int n, const double *params, const double *y_init, double *y_out, int max_rec,
                  double *rec, int *n_rec, int *status, int *stats, double *t_start, double *t_done, int lu_bits,
-                 int n_slots, hipStream_t stream, double *timers = nullptr, const IntervalArgs &iv = IntervalArgs())
+                 int n_slots, hipStream_t stream, double *timers = nullptr, const IntervalArgs &iv = IntervalArgs(),
+                 const FixedWs &fixed = FixedWs())
 {
     const ucg::DevNet &N = h.dev;
     if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
@@ -294,11 +316,17 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     const bool sparse = h.linear_solver == UCG_LINSOL_SPARSE;
     const SlotLayout sl = slot_layout(h, lu_bits);
     const size_t mat = sl.mat, jac = sl.jac, stride = sl.stride;
-    // sized for the default slot count at least, so later calls with more points do not
-    // reallocate between launches
-    int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
-    if (rc) return rc;
-    hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, h.counter, status, n);
+    void *ws = fixed.ws;
+    int *counter = fixed.counter;
+    if (!ws) {
+        // sized for the default slot count at least, so later calls with more points do not
+        // reallocate between launches
+        int rc = ensure_ws(h, stride * (size_t)std::max(n_slots, 2 * g_num_cu));
+        if (rc) return rc;
+        ws = h.ws;
+        counter = h.counter;
+    }
+    hipLaunchKernelGGL(ucg::solve_init_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, counter, status, n);
     ucg::SolveArgs A = {};
     A.n_points = n;
     A.params = params;
@@ -314,11 +342,11 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.timers = timers;
     A.equilibrate = getenv("UCG_EQUILIBRATE") ? atoi(getenv("UCG_EQUILIBRATE")) : 0;
     A.sparse = sparse ? 1 : 0;
-    A.ws = h.ws;
+    A.ws = ws;
     A.ws_stride = stride;
     A.ws_mat_bytes = mat;
     A.ws_jac_bytes = jac;
-    A.counter = h.counter;
+    A.counter = counter;
     A.cancel = h.cancel;
     A.gen = ++h.gen;
     A.phys_in = iv.phys_in;
@@ -947,6 +975,50 @@ int ucg_solve_batch_device(int handle, int n, const double *params, const double
         return rc;
     return launch_solve(*h, n, params, y_init, y_out, max_rec, rec, n_rec, status, stats, t_start, t_done,
                         lu_bits, n_slots, (hipStream_t)stream);
+}
+
+int ucg_solve_batch_device_noalloc(int handle, int n, const double *params, const double *y_init, double *y_out,
+                                   int max_rec, double *rec, int *n_rec, int *status, int *stats, double *t_start,
+                                   double *t_done, int lu_bits, int n_slots, void *stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (n < 0) return fail(UCG_E_ARG, "n must not be negative");
+    if (n == 0) return 0;
+    if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
+    if (n_slots < 0) return fail(UCG_E_ARG, "n_slots must not be negative");
+    void *base = g_device >= 0 && g_device < (int)g_scratch.size() ? g_scratch[g_device] : nullptr;
+    if (!base) return fail(UCG_E_HIP, "the fixed workspace of this GPU was not set up");
+    const size_t stride = slot_layout(*h, lu_bits).stride;
+    const size_t cap = (UCG_SCRATCH_BYTES - 256) / stride;   // slots the fixed workspace holds
+    if (cap == 0)
+        return fail(UCG_E_ARG, "one slot of this network needs " + std::to_string(stride) + " B, more than the " +
+                                   std::to_string(UCG_SCRATCH_BYTES - 256) + " B fixed workspace");
+    const int want = std::min(n_slots > 0 ? n_slots : 2 * g_num_cu, n);
+    if (n_slots > 0 && (size_t)want > cap)
+        return fail(UCG_E_ARG, std::to_string(want) + " slots of " + std::to_string(stride) +
+                                   " B exceed the fixed workspace (at most " + std::to_string(cap) + " slots)");
+    const int slots = (int)std::min((size_t)want, cap);
+    const size_t un = (size_t)n, neq = (size_t)h->dev.neq, rw = 4 + (size_t)h->dev.nspec;
+    if (!rec) max_rec = 0;
+    if (max_rec < 0) return fail(UCG_E_ARG, "max_rec must not be negative");
+    int rc;
+    if ((rc = check_dev_buf(params, un * UCG_NPARAM * sizeof(double), "params")) ||
+        (rc = check_dev_buf(y_init, un * neq * sizeof(double), "y_init", true)) ||
+        (rc = check_dev_buf(y_out, un * neq * sizeof(double), "y_out")) ||
+        (rc = check_dev_buf(rec, un * (size_t)max_rec * rw * sizeof(double), "rec", max_rec == 0)) ||
+        (rc = check_dev_buf(n_rec, un * sizeof(int), "n_rec", true)) ||
+        (rc = check_dev_buf(status, un * sizeof(int), "status")) ||
+        (rc = check_dev_buf(stats, un * 8 * sizeof(int), "stats")) ||
+        (rc = check_dev_buf(t_start, un * sizeof(double), "t_start", true)) ||
+        (rc = check_dev_buf(t_done, un * sizeof(double), "t_done", true)))
+        return rc;
+    FixedWs fw;
+    fw.counter = (int *)base;                        // reset by solve_init_kernel on the launch stream
+    fw.ws = (unsigned char *)base + 256;
+    return launch_solve(*h, n, params, y_init, y_out, max_rec, rec, n_rec, status, stats, t_start, t_done, lu_bits,
+                        slots, (hipStream_t)stream, nullptr, IntervalArgs(), fw);
 }
 
 int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_out, int *status, int *stats,

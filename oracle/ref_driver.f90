@@ -128,11 +128,12 @@ CONTAINS
     ! retry handling are the reference's own (calculateReactionRates, integrateODESystem).
     ! With usepostprocess (its default, postprocess.f90:14) the reference never raises its
     ! failure counters, so they are not restated.
-    SUBROUTINE bounded_update_chemistry(maxsteps, nsteps, flag, attempts)
+    SUBROUTINE bounded_update_chemistry(maxsteps, nsteps, flag, attempts, work)
         REAL(dp), INTENT(IN) :: maxsteps
         REAL(dp), INTENT(INOUT) :: nsteps
         INTEGER, INTENT(OUT) :: flag
         INTEGER(c_int), INTENT(OUT) :: attempts
+        INTEGER(c_int), INTENT(INOUT), OPTIONAL :: work(6)   ! += NFE NJE NLU NNI NCFN NETF of every attempt
         REAL(dp) :: originalTargetTime, surfCov, rstats(22)
         INTEGER :: loops, istats(31)
         loops = 0
@@ -161,6 +162,7 @@ CONTAINS
             CALL integrateODESystem(flag)
             CALL GET_STATS(rstats, istats)
             nsteps = nsteps + real(istats(11), dp)
+            IF (PRESENT(work)) work = work + (/istats(12), istats(13), istats(19), istats(20), istats(21), istats(22)/)
             IF (flag .lt. 0) RETURN
             WHERE(abund<MIN_ABUND) abund=MIN_ABUND
             density(dstep)=abund(NEQ,dstep)
@@ -373,8 +375,8 @@ CONTAINS
     ! One output interval (updateChemistry, chemistry.f90:145-239, restated as
     ! bounded_update_chemistry with an unreachable bound unless maxsteps > 0) of the point
     ! with parameters p from the state (y, phys) given as ucref_model_until returns it, to
-    ! t_target seconds; no physics update afterwards.  stats = (steps of every attempt,
-    ! attempts, 0...).
+    ! t_target seconds; no physics update afterwards.  stats = (steps, attempts, and DVODE's
+    ! NFE, NJE, NLU, NNI, NCFN, NETF, each summed over the attempts).
     INTEGER(c_int) FUNCTION ucref_interval(p, y, phys, t_target, maxsteps, y_out, phys_out, stats) &
             BIND(C, name="ucref_interval")
         REAL(c_double), INTENT(IN) :: p(P_NPARAM), y(nspec+1), phys(8)
@@ -404,7 +406,7 @@ CONTAINS
         nsteps = 0.0d0
         bound = HUGE(1.0d0)
         IF (maxsteps > 0.0d0) bound = maxsteps
-        CALL bounded_update_chemistry(bound, nsteps, flag, stats(2))
+        CALL bounded_update_chemistry(bound, nsteps, flag, stats(2), stats(3:8))
         stats(1) = INT(MIN(nsteps, 2.0d9))
         CALL get_state(y_out, phys_out)
         ucref_interval = flag

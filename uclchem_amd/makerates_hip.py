@@ -530,6 +530,10 @@ def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
         ice_pos=ice_pos, gas_ice_pos=gas_ice_pos, is_bulk=is_bulk, is_surface=is_surface,
         binding_energy=fortran_real([species[i].binding_energy for i in ice]),
         formation_enthalpy=fortran_real([species[i].enthalpy for i in ice]),
+        # per surface species: solid, mono and volcanic fractions (sputtering inputs that
+        # network.f90 carries; the chemistry solve does not read them)
+        sputter_fractions=np.array([[species[i].solid_fraction, species[i].mono_fraction, species[i].volc_fraction]
+                                    for i in surface], dtype=np.float64).reshape(-1, 3),
         refractory=refractory,
         re=re, pr=pr, rtype=rtype, alpha=alpha, beta=beta, gama=gama, tmin=tmin, tmax=tmax,
         reduced_mass=redmass, extrapolate=extrap, type_ranges=ranges,

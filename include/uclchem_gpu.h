@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define UCG_NPARAM 24
+#define UCG_NPARAM 64
 /* point parameters (defaultparameters.f90 names) */
 #define UCG_P_DENS 0          /* initialDens [cm^-3] */
 #define UCG_P_TEMP 1          /* initialTemp [K] */
@@ -57,10 +57,53 @@ extern "C" {
                                  the code UCLCHEM reserves for integrations that cannot advance.
                                  No counterpart in the reference: a work bound for batch and serving
                                  callers whose grids hold a few pathologically stiff points. */
+/* initial elemental abundances (defaultparameters.f90:70-90, applied at chemistry.f90:58-105) */
+#define UCG_P_METALLICITY 21  /* scales every element heavier than He */
+#define UCG_P_ION 22          /* 0: C atomic, 1: half ionised, 2: C+ (default) */
+#define UCG_P_FH 23           /* fraction of H atomic; the rest in H2 */
+#define UCG_P_FHE 24
+#define UCG_P_FC 25
+#define UCG_P_FO 26
+#define UCG_P_FN 27
+#define UCG_P_FS 28
+#define UCG_P_FMG 29
+#define UCG_P_FSI 30
+#define UCG_P_FCL 31
+#define UCG_P_FP 32
+#define UCG_P_FFE 33          /* read by the reference's parser, not used by its start */
+#define UCG_P_FF 34
+#define UCG_P_FD 35
+#define UCG_P_FLI 36
+#define UCG_P_FNA 37
+#define UCG_P_FPAH 38
+#define UCG_P_F15N 39
+#define UCG_P_F13C 40
+#define UCG_P_F18O 41
+/* desorption and photon constants (defaultparameters.f90:113-120) */
+#define UCG_P_EBMAXH2 42
+#define UCG_P_EBMAXCR 43
+#define UCG_P_EBMAXUVCR 44
+#define UCG_P_EPSILON 45
+#define UCG_P_UV_YIELD 46
+#define UCG_P_PHI 47          /* CR-desorption yield of the first rates call */
+#define UCG_P_UVCREFF 48
+#define UCG_P_OMEGA 49
+/* switches, 0 or 1 (defaultparameters.f90:40-49) */
+#define UCG_P_DESORB 50
+#define UCG_P_H2DESORB 51
+#define UCG_P_CRDESORB 52
+#define UCG_P_UVDESORB 53
+#define UCG_P_THERMDESORB 54
+#define UCG_P_CRATTENUATION 55    /* cosmicRayAttenuation */
+#define UCG_P_IONMODEL 56         /* 0: 'L', 1: 'H' */
+#define UCG_P_IMPROVEDH2CRP 57    /* improvedH2CRPDissociation (needs UCG_P_CRATTENUATION) */
+#define UCG_P_ENFORCECHARGE 58    /* enforceChargeConservation */
+/* slots 59..63 are reserved (0) */
 
 /* status codes (UCLCHEM's own, src/fortran_src/constants.f90:22-27) */
 #define UCG_OK 0
 #define UCG_E_PARAM -1
+#define UCG_E_PHYSICS_INIT -2
 #define UCG_E_UNRECOVERABLE -4
 #define UCG_E_TOO_MANY_FAILS -5
 #define UCG_E_TIMEPOINTS -6
@@ -87,7 +130,8 @@ typedef struct ucg_network_desc {
     const int *jnz_row, *jnz_col, *jnz_ptr, *jc_k, *jc_m, *jc_c;
     const int *aff_ptr, *aff_sp, *aff_cf, *sw_ptr, *sw_k, *sw_c;
     int special_species[25];   /* nh nh2 nhe nc ncx nn no nf np nmg nmgx nna nli npah nsx nsix nclx nd n18o n15n n13c nelec nco nsurface nbulk */
-    int special_reactions[9];  /* H2Form_CT H2Form_ER H2Form_ERDes HFreeze EFreeze H2Freeze H2_hv CO_hv C_hv */
+    int special_reactions[10]; /* H2Form_CT H2Form_ER H2Form_ERDes HFreeze EFreeze H2Freeze H2_hv CO_hv C_hv,
+                                  H2_crp (-1 when the network has none) */
     /* static-pattern sparse Newton plan (uclchem_amd/newton_plan.py): sizes are
      * N ns T nsp nlev ntg npair nph nphs ngs; arrays as named there */
     int nw_sizes[10];
@@ -95,6 +139,13 @@ typedef struct ucg_network_desc {
     const int *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
     const int *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
     const int *nw_misc, *nw_nbrow, *nw_nsrow;
+    /* positive ions (species names holding '+', chemistry.f90:114-120) */
+    int nion;
+    const int *ion_list;
+    /* grain-assisted recombination parameters, 7 per GAR reaction in reaction order
+       (network.f90 garParams, rates.f90:316-332) */
+    int ngar;
+    const double *gar_params;
 } ucg_network_desc;
 
 /* Select the GPU and upload the constant photo-rate tables.  Called implicitly. */

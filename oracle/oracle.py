@@ -16,19 +16,33 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
-REF_SO = os.path.join(HERE, "_ref", "libucref.so")
+# UCREF_LIB selects another build of the reference library (e.g. one compiled on a written network)
+REF_SO = os.environ.get("UCREF_LIB", os.path.join(HERE, "_ref", "libucref.so"))
 
 # parameter vector layout (oracle/ucl_oracle.h, oracle/ref_driver.f90)
 P_NAMES = ["dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall",
            "finaldens", "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp",
-           "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps"]
-P_NPARAM = 24
-# Defaults of defaultparameters.f90:18-122.  rout is a default-real literal (0.05 -> float32).
+           "freezefactor", "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps",
+           "metallicity", "ion", "fh", "fhe", "fc", "fo", "fn", "fs", "fmg", "fsi", "fcl", "fp", "ffe", "ff",
+           "fd", "fli", "fna", "fpah", "f15n", "f13c", "f18o",
+           "ebmaxh2", "ebmaxcr", "ebmaxuvcr", "epsilon", "uv_yield", "phi", "uvcreff", "omega",
+           "desorb", "h2desorb", "crdesorb", "uvdesorb", "thermdesorb", "cosmicrayattenuation", "ionmodel",
+           "improvedh2crpdissociation", "enforcechargeconservation"]
+P_NPARAM = 64
+_F32 = lambda x: float(np.float32(x))   # noqa: E731  default-real literals
+# Defaults of defaultparameters.f90:18-122 as the compiled reference holds them.
 DEFAULTS = dict(dens=1.0e2, temp=10.0, zeta=1.0, radfield=1.0, baseav=2.0,
-                rout=float(np.float32(0.05)), rin=0.0, finaltime=5.0e6, freefall=0.0,
+                rout=_F32(0.05), rin=0.0, finaltime=5.0e6, freefall=0.0,
                 finaldens=1.0e5, reltol=1e-8, abstol_factor=1e-14, abstol_min=1e-25, model=0.0,
                 tempindx=1.0, maxtemp=300.0, freezefactor=1.0, endatfinaldensity=0.0,
-                freefallfactor=1.0, mxstep=10000.0, maxsteps=0.0)
+                freefallfactor=1.0, mxstep=10000.0, maxsteps=0.0,
+                metallicity=1.0, ion=2.0, fh=0.5, fhe=_F32(0.1), fc=1.77e-04, fo=3.34e-04, fn=6.18e-05,
+                fs=3.51e-6, fmg=2.256e-06, fsi=1.78e-06, fcl=3.39e-08, fp=7.78e-08, ffe=2.01e-7, ff=3.6e-08,
+                fd=0.0, fli=0.0, fna=0.0, fpah=0.0, f15n=0.0, f13c=0.0, f18o=0.0,
+                ebmaxh2=1.21e3, ebmaxcr=1.21e3, ebmaxuvcr=1.0e4, epsilon=_F32(0.01), uv_yield=_F32(0.03),
+                phi=1.0e5, uvcreff=1.0e-3, omega=0.5,
+                desorb=1.0, h2desorb=1.0, crdesorb=1.0, uvdesorb=1.0, thermdesorb=1.0, cosmicrayattenuation=0.0,
+                ionmodel=0.0, improvedh2crpdissociation=0.0, enforcechargeconservation=0.0)
 
 
 def make_params(**kw) -> np.ndarray:
@@ -63,14 +77,15 @@ class _Net(ctypes.Structure):
                   "nli", "npah", "nsx", "nsix", "nclx", "nd", "n18o", "n15n", "n13c", "nelec",
                   "nco", "nsurface", "nbulk", "ngn", "ngo", "ngoh",
                   "r_h2form_ct", "r_h2form_erdes", "r_h2form_er", "r_hfreeze", "r_efreeze",
-                  "r_h2freeze", "r_h2_hv", "r_co_hv", "r_c_hv"]])
+                  "r_h2freeze", "r_h2_hv", "r_co_hv", "r_c_hv", "r_h2_crp"]] +
+                [("gar_params", ctypes.c_void_p), ("ngar", ctypes.c_int32)])
 
 
 class _Point(ctypes.Structure):
     _fields_ = [("p", ctypes.c_double * P_NPARAM)] + [
         (n, ctypes.c_double) for n in ["density", "gas_temp", "dust_temp", "av", "cloud_size",
                                        "coldens", "phi", "h2col", "cocol", "ccol", "safe_mantle",
-                                       "safe_bulk", "blr"]]
+                                       "safe_bulk", "blr", "h2crp"]]
 
 
 def _ptr(a):
@@ -114,8 +129,13 @@ class Oracle:
         for fld, key in [("r_h2form_ct", "nR_H2Form_CT"), ("r_h2form_erdes", "nR_H2Form_ERDes"),
                          ("r_h2form_er", "nR_H2Form_ER"), ("r_hfreeze", "nR_HFreeze"),
                          ("r_efreeze", "nR_EFreeze"), ("r_h2freeze", "nR_H2Freeze"),
-                         ("r_h2_hv", "nR_H2_hv"), ("r_co_hv", "nR_CO_hv"), ("r_c_hv", "nR_C_hv")]:
+                         ("r_h2_hv", "nR_H2_hv"), ("r_co_hv", "nR_CO_hv"), ("r_c_hv", "nR_C_hv"),
+                         ("r_h2_crp", "nR_H2_crp")]:
             setattr(net, fld, s[key])
+        gar = np.ascontiguousarray(np.asarray(a.get("gar_params", np.zeros((0, 7))), dtype=np.float64).reshape(-1))
+        self._keep["gar_params"] = gar
+        net.gar_params = gar.ctypes.data if len(gar) else None
+        net.ngar = len(gar) // 7
         self.net = net
         L = self.lib
         L.ucl_next_target.restype = ctypes.c_double
@@ -126,6 +146,8 @@ class Oracle:
         L.ucl_update_physics.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
 
     def init_point(self, params):
+        if len(params) != P_NPARAM:
+            raise ValueError(f"parameter vector must have {P_NPARAM} entries")
         pt = _Point()
         y = np.zeros(self.t.neq)
         self.lib.ucl_point_init(ctypes.byref(self.net), _ptr(np.ascontiguousarray(params, dtype=np.float64)),
@@ -199,6 +221,16 @@ class RefLib:
         stats = np.zeros(8, dtype=np.int32)
         rc = self.lib.ucref_model(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
                                   ctypes.c_int(maxrec), _ptr(rec), ctypes.byref(nrec), _ptr(stats))
+        return rc, rec[:nrec.value].copy(), stats
+
+    def model_from(self, params, y0, maxrec: int = 1000):
+        """The model from given start abundances y0 [neq] (the reference's abundanceStart)."""
+        rec = np.zeros((maxrec, 4 + self.nspec))
+        nrec = ctypes.c_int()
+        stats = np.zeros(8, dtype=np.int32)
+        rc = self.lib.ucref_model_from(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                       _ptr(np.ascontiguousarray(y0, dtype=np.float64)), ctypes.c_int(maxrec),
+                                       _ptr(rec), ctypes.byref(nrec), _ptr(stats))
         return rc, rec[:nrec.value].copy(), stats
 
     def model_until(self, params, t_stop_years: float, maxrec: int = 1000):

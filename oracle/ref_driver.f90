@@ -35,12 +35,20 @@ MODULE ucref_driver
     REAL(dp), PARAMETER :: tempb(nMasses)=(/0.5339,0.6255,0.8395,1.085,1.289,1.98/)
     INTEGER :: modelKind = 0, tempIndx = 1
     REAL(dp) :: maxTemp = 300.0d0
+    ! given start abundances of the next model (wrap.f90:636-640, abundanceStart)
+    LOGICAL :: have_start = .false.
+    REAL(dp), ALLOCATABLE :: start_abund(:)
 
     ! parameter vector layout; mirrored by oracle/oracle.py and uclchem_amd/params.py
     INTEGER, PARAMETER :: P_DENS=1, P_TEMP=2, P_ZETA=3, P_RADFIELD=4, P_BASEAV=5, P_ROUT=6, P_RIN=7, &
         & P_FINALTIME=8, P_FREEFALL=9, P_FINALDENS=10, P_RELTOL=11, P_ABSFAC=12, P_ABSMIN=13, &
         & P_MODEL=14, P_TEMPINDX=15, P_MAXTEMP=16, P_FREEZEFAC=17, P_ENDATFINALDENS=18, P_FREEFALLFAC=19, &
-        & P_MXSTEP=20, P_MAXSTEPS=21, P_NPARAM=24
+        & P_MXSTEP=20, P_MAXSTEPS=21, P_METALLICITY=22, P_ION=23, P_FH=24, P_FHE=25, P_FC=26, P_FO=27, &
+        & P_FN=28, P_FS=29, P_FMG=30, P_FSI=31, P_FCL=32, P_FP=33, P_FFE=34, P_FF=35, P_FD=36, P_FLI=37, &
+        & P_FNA=38, P_FPAH=39, P_F15N=40, P_F13C=41, P_F18O=42, P_EBMAXH2=43, P_EBMAXCR=44, P_EBMAXUVCR=45, &
+        & P_EPSILON=46, P_UV_YIELD=47, P_PHI=48, P_UVCREFF=49, P_OMEGA=50, P_DESORB=51, P_H2DESORB=52, &
+        & P_CRDESORB=53, P_UVDESORB=54, P_THERMDESORB=55, P_CRATTEN=56, P_IONMODEL=57, P_IMPROVEDH2CRP=58, &
+        & P_ENFORCECHARGE=59, P_NPARAM=64
 CONTAINS
 
     SUBROUTINE reset_globals()
@@ -59,6 +67,7 @@ CONTAINS
         uvcreff=1.0d-3; omega=0.5
         usepostprocess=.true.; lusecoldens=.false.
         modelKind=0; tempIndx=1; maxTemp=300.0d0
+        h2CRPRate=0.0d0   ! module state the physics update sets; each point starts from zero
     END SUBROUTINE reset_globals
 
     SUBROUTINE apply_params(p)
@@ -70,6 +79,21 @@ CONTAINS
         modelKind=NINT(p(P_MODEL)); tempIndx=NINT(p(P_TEMPINDX)); maxTemp=p(P_MAXTEMP)
         freezeFactor=p(P_FREEZEFAC); endAtFinalDensity=(p(P_ENDATFINALDENS) .ne. 0.0d0)
         freefallFactor=p(P_FREEFALLFAC); MXSTEP=NINT(p(P_MXSTEP))
+        metallicity=p(P_METALLICITY); ion=NINT(p(P_ION), dp); fh=p(P_FH); fhe=p(P_FHE); fc=p(P_FC); fo=p(P_FO)
+        fn=p(P_FN); fs=p(P_FS); fmg=p(P_FMG); fsi=p(P_FSI); fcl=p(P_FCL); fp=p(P_FP); ffe=p(P_FFE); ff=p(P_FF)
+        fd=p(P_FD); fli=p(P_FLI); fna=p(P_FNA); fpah=p(P_FPAH); f15n=p(P_F15N); f13c=p(P_F13C); f18O=p(P_F18O)
+        ebmaxh2=p(P_EBMAXH2); ebmaxcr=p(P_EBMAXCR); ebmaxuvcr=p(P_EBMAXUVCR); epsilon=p(P_EPSILON)
+        uv_yield=p(P_UV_YIELD); phi=p(P_PHI); uvcreff=p(P_UVCREFF); omega=p(P_OMEGA)
+        desorb=(p(P_DESORB) .ne. 0.0d0); h2desorb=(p(P_H2DESORB) .ne. 0.0d0); crdesorb=(p(P_CRDESORB) .ne. 0.0d0)
+        uvdesorb=(p(P_UVDESORB) .ne. 0.0d0); thermdesorb=(p(P_THERMDESORB) .ne. 0.0d0)
+        cosmicRayAttenuation=(p(P_CRATTEN) .ne. 0.0d0)
+        IF (NINT(p(P_IONMODEL)) .eq. 1) THEN
+            ionModel='H'
+        ELSE
+            ionModel='L'
+        END IF
+        improvedH2CRPDissociation=(p(P_IMPROVEDH2CRP) .ne. 0.0d0)
+        enforceChargeConservation=(p(P_ENFORCECHARGE) .ne. 0.0d0)
     END SUBROUTINE apply_params
 
     SUBROUTINE hotcoreUpdateTargetTime
@@ -334,6 +358,22 @@ CONTAINS
         ucref_model = run_model(p, HUGE(1.0d0), maxrec, rec, nrec, stats)
     END FUNCTION ucref_model
 
+    ! The model loop from given start abundances y0(1:nspec+1) instead of the elemental start
+    ! (the abundanceStart path of wrap.f90:636-640: set after initializeChemistry).
+    INTEGER(c_int) FUNCTION ucref_model_from(p, y0, maxrec, rec, nrec, stats) BIND(C, name="ucref_model_from")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM), y0(nspec+1)
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec)
+        INTEGER(c_int), INTENT(OUT) :: nrec
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
+        IF (ALLOCATED(start_abund)) DEALLOCATE(start_abund)
+        ALLOCATE(start_abund(nspec+1))
+        start_abund = y0
+        have_start = .true.
+        ucref_model_from = run_model(p, HUGE(1.0d0), maxrec, rec, nrec, stats)
+        have_start = .false.
+    END FUNCTION ucref_model_from
+
     ! The same loop stopped after the first interval that ends at or beyond t_stop years;
     ! y_out and phys_out are the point's state there as the next interval starts from it
     ! (after the physics update): phys = (density, gasTemp, dustTemp, av, cloudSize, phi,
@@ -430,6 +470,7 @@ CONTAINS
             run_model = flag
             RETURN
         END IF
+        IF (have_start) abund(:nspec+1,1) = start_abund(:nspec+1)
         CALL record_row(rec, maxrec, nrec)
         DO WHILE ((flag .eq. 0) .and. (timeInYears < t_stop) .and. &
             &(((endAtFinalDensity) .and. (density(1) < finalDens)) .or. &

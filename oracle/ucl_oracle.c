@@ -43,13 +43,6 @@ static double vdiff_prefactor(void) { return 2.0 * K_BOLTZ * SURFACE_SITE_DENSIT
 static double num_sites_per_grain(void) { return GRAIN_RADIUS * GRAIN_RADIUS * SURFACE_SITE_DENSITY * 4.0 * PI; }
 
 /* defaultparameters.f90 "here be dragons" values; epsilon and uv_yield are default-real */
-#define EBMAXH2 1.21e3
-#define EBMAXCR 1.21e3
-#define EBMAXUVCR 1.0e4
-#define EPSILON_H2 ((double)0.01f)
-#define UV_YIELD ((double)0.03f)
-#define UVCREFF 1.0e-3
-#define OMEGA 0.5
 #define ICE_GAS_RATIO ((double)0.3f)      /* photoreactions.f90:41 */
 #define TURB_VEL 1.0                      /* rates.f90:17 */
 
@@ -349,34 +342,38 @@ int ucl_point_init(const ucl_net *n, const double *params, ucl_point *pt, double
     pt->coldens = 1.0 * pt->cloud_size / 1.0 * pt->p[P_DENS];   /* real(points-dstep+1)*cloudSize/real(points)*initialDens */
     pt->av = pt->p[P_BASEAV] + pt->coldens / 1.6e21;
     if (pt->p[P_FREEFALL] != 0.0) pt->density = (double)1.001f * pt->p[P_DENS];
-    pt->phi = 1.0e5;
-    /* elemental abundances, defaultparameters.f90:68-90 (fhe is a default-real literal) */
-    const double fh = 0.5, fhe = (double)0.1f, fc = 1.77e-04, fo = 3.34e-04, fn = 6.18e-05, fs = 3.51e-6,
-                 fmg = 2.256e-06, fsi = 1.78e-06, fcl = 3.39e-08, fp = 7.78e-08, ff = 3.6e-08;
+    pt->phi = pt->p[P_PHI];
+    pt->h2crp = 0.0;
+    /* elemental abundances (chemistry.f90:58-105) from the parameter vector */
+    const double *q = pt->p;
     int neq = n->neq;
     for (int i = 0; i < neq; i++) y[i] = 1.0e-30;
-    y[n->no] = fo;
-    y[n->nn] = fn;
-    y[n->nmg] = fmg;
-    y[n->np] = fp;
-    y[n->nf] = ff;
-    y[n->nna] = 0.0;
-    y[n->nli] = 0.0;
-    y[n->npah] = 0.0;
-    y[n->nsx] = fs;
-    y[n->nsix] = fsi;
-    y[n->nclx] = fcl;
-    y[n->nc] = 1.0e-10;       /* ion=2: carbon fully ionised */
-    y[n->ncx] = fc;
-    y[n->n18o] = 0.0;
-    y[n->n15n] = 0.0;
-    y[n->n13c] = 0.0;
+    y[n->no] = q[P_FO];
+    y[n->nn] = q[P_FN];
+    y[n->nmg] = q[P_FMG];
+    y[n->np] = q[P_FP];
+    y[n->nf] = q[P_FF];
+    y[n->nna] = q[P_FNA];
+    y[n->nli] = q[P_FLI];
+    y[n->npah] = q[P_FPAH];
+    y[n->nsx] = q[P_FS];
+    y[n->nsix] = q[P_FSI];
+    y[n->nclx] = q[P_FCL];
+    switch ((int)q[P_ION]) {   /* how much carbon starts ionised */
+    case 0: y[n->nc] = q[P_FC]; y[n->ncx] = 1.0e-10; break;
+    case 1: y[n->nc] = q[P_FC] * 0.5; y[n->ncx] = q[P_FC] * 0.5; break;
+    case 2: y[n->nc] = 1.0e-10; y[n->ncx] = q[P_FC]; break;
+    default: break;
+    }
+    y[n->n18o] = q[P_F18O];
+    y[n->n15n] = q[P_F15N];
+    y[n->n13c] = q[P_F13C];
     y[n->nelec] = y[n->ncx] + y[n->nsix] + y[n->nsx] + y[n->nclx] + y[n->nmgx];
-    for (int i = 0; i < neq; i++) y[i] = y[i] * 1.0;   /* metallicity */
-    y[n->nh] = fh;
-    y[n->nh2] = 0.5 * (1.0 - fh);
-    y[n->nd] = 0.0;
-    y[n->nhe] = fhe;
+    for (int i = 0; i < neq; i++) y[i] = y[i] * q[P_METALLICITY];
+    y[n->nh] = q[P_FH];
+    y[n->nh2] = 0.5 * (1.0 - q[P_FH]);
+    y[n->nd] = q[P_FD];
+    y[n->nhe] = q[P_FHE];
     y[neq - 1] = pt->density;
     return 0;
 }
@@ -409,9 +406,12 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
 
     for (int k = 0; k < n->nreac; k++) rate[k] = 0.0;
 
+    const double *q = pt->p;
+    const int desorb = q[P_DESORB] != 0.0;
     range_of(n, T_CRP, &a, &b);
     if (a != b)
         for (int j = a; j <= b; j++) rate[j] = n->alpha[j] * zeta;
+    if (q[P_IMPROVEDH2CRP] != 0.0 && n->r_h2_crp >= 0) rate[n->r_h2_crp] = pt->h2crp;
 
     const double photo_bulk_pow = pow((double)(1.0f - 0.007f), 1.0 + 0.5 / blr);
     range_of(n, T_PHOTON, &a, &b);
@@ -426,7 +426,7 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
     range_of(n, T_CRPHOT, &a, &b);
     if (a != b) {
         for (int j = a; j <= b; j++) {
-            rate[j] = n->alpha[j] * n->gama[j] * 1.0 / (1.0 - OMEGA) * zeta * pow(T / 300, n->beta[j]);
+            rate[j] = n->alpha[j] * n->gama[j] * 1.0 / (1.0 - q[P_OMEGA]) * zeta * pow(T / 300, n->beta[j]);
             int r1 = n->re[3 * j];
             if (in_list(n->bulk_list, n->nbulkl, r1)) rate[j] = rate[j] * ICE_GAS_RATIO * photo_bulk_pow;
             else if (in_list(n->surface_list, n->nsurfl, r1)) rate[j] = rate[j] * ICE_GAS_RATIO;
@@ -442,9 +442,9 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
     /* non-thermal desorption (rates.f90:91-155) with freeze-out switch-off */
     range_of(n, T_DESOH2, &a, &b);
     if (a != b) {
-        if (sm > MIN_SURFACE_ABUND) {
-            double v = EPSILON_H2 * ucl_h2_form_efficiency(T, Td);
-            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > EBMAXH2 ? 0.0 : v;
+        if (desorb && q[P_H2DESORB] != 0.0 && sm > MIN_SURFACE_ABUND) {
+            double v = q[P_EPSILON] * ucl_h2_form_efficiency(T, Td);
+            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > q[P_EBMAXH2] ? 0.0 : v;
         }
         for (int q = 0; q < n->nfp; q++) {
             int f = n->freeze_partners[q];
@@ -453,9 +453,9 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
     }
     range_of(n, T_DESCR, &a, &b);
     if (a != b) {
-        if (sm > MIN_SURFACE_ABUND) {
+        if (desorb && q[P_CRDESORB] != 0.0 && sm > MIN_SURFACE_ABUND) {
             double v = 4.0 * PI * zeta * 1.64e-4 * (GRAIN_SURFACEAREA_PER_H) * pt->phi;
-            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > EBMAXCR ? 0.0 : v;
+            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > q[P_EBMAXCR] ? 0.0 : v;
         }
         for (int q = 0; q < n->nfp; q++) {
             int f = n->freeze_partners[q];
@@ -464,19 +464,35 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
     }
     range_of(n, T_DEUVCR, &a, &b);
     if (a != b) {
-        if (sm > MIN_SURFACE_ABUND && zeta > 0) {
-            double v = GRAIN_CROSSSECTION_PER_H * UV_YIELD * 4.875e3 * zeta;
-            v = v * (1 + (radfield / UVCREFF) * (1.0 / zeta) * exp((double)-1.8f * av));
-            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > EBMAXUVCR ? 0.0 : v;
+        if (desorb && q[P_UVDESORB] != 0.0 && sm > MIN_SURFACE_ABUND && zeta > 0) {
+            double v = GRAIN_CROSSSECTION_PER_H * q[P_UV_YIELD] * 4.875e3 * zeta;
+            v = v * (1 + (radfield / q[P_UVCREFF]) * (1.0 / zeta) * exp((double)-1.8f * av));
+            for (int j = a; j <= b; j++) rate[j] = n->gama[j] > q[P_EBMAXUVCR] ? 0.0 : v;
         }
         for (int q = 0; q < n->nfp; q++) {
             int f = n->freeze_partners[q];
             if (rate[f] * y[n->re[3 * f]] * dens < MIN_SURFACE_ABUND * rate[a + q]) rate[f] = 0.0;
         }
     }
+    /* excited-species reactions (rates.f90:158-197): CR excitation, relaxation at the
+     * reactant's diffusion frequency, and solid-phase reactions of excited species */
+    range_of(n, T_CRS, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++)
+            rate[j] = n->alpha[j] * (n->beta[j] * (n->gama[j] / 100) * ((double)8.6f * zeta * (double)1.3f));
+    range_of(n, T_EXRELAX, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++) rate[j] = vdiff_of(n, ice_index(n, n->re[3 * j]));
+    range_of(n, T_EXSOLID, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++) {
+            const double va = vdiff_of(n, ice_index(n, n->re[3 * j])), vb = vdiff_of(n, ice_index(n, n->re[3 * j + 1]));
+            rate[j] = (vb + va) / (SURFACE_SITE_DENSITY * 1.8e-8);
+            rate[j] = n->alpha[j] * rate[j];
+        }
     /* continuous thermal desorption (rates.f90:200-229) */
     range_of(n, T_THERM, &a, &b);
-    if (a != b) {
+    if (a != b && q[P_THERMDESORB] != 0.0) {   /* off: the range stays zero, no switch-off */
         for (int j = a; j <= b; j++) {
             int p = ice_index(n, n->re[3 * j]);
             if (p >= 0) rate[j] = vdiff_of(n, p) * exp(-n->gama[j] / Td);
@@ -547,9 +563,26 @@ int ucl_rates(const ucl_net *n, ucl_point *pt, double *y, double *rate, double *
     if (a >= 0)
         for (int j = a; j <= b; j++)
             rate[j] = n->alpha[j] * pow(T / 300.0, n->beta[j]) * exp(-n->gama[j] / T);
-    /* phi update used by the next call's CR desorption (rates.f90:320-326) */
+    /* ion-polar reactions (rates.f90:301-313, KIDA formulas) */
+    range_of(n, T_IONOPOL1, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++)
+            rate[j] = n->alpha[j] * n->beta[j] * (0.62 + 0.4767 * n->gama[j] * sqrt(300.0 / T));
+    range_of(n, T_IONOPOL2, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++)
+            rate[j] = n->alpha[j] * n->beta[j] *
+                      (1.0 + 0.0967 * n->gama[j] * sqrt(300.0 / T) + n->gama[j] * n->gama[j] * 300.0 / ((double)10.526f * T));
+    /* phi: used by GAR now and by the next call's CR desorption (rates.f90:320-326) */
     double phi = radfield * exp(-2.5 * av) * sqrt(T) / (y[neq - 1] * y[n->nelec]);
     pt->phi = fmin(fmax(phi, 1e2), 1e6);
+    range_of(n, T_GAR, &a, &b);
+    if (a != b)
+        for (int j = a; j <= b; j++) {
+            const double *g = n->gar_params + 7 * (j - a), ph = pt->phi;
+            rate[j] = (double)0.6f * n->alpha[j] * g[0] /
+                      (1. + g[1] * pow(ph, g[2]) * (1. + g[3] * pow(T, g[4]) * pow(ph, -g[5] - g[6] * log(T))));
+        }
     /* temperature-range masks (rates.f90:335-337) */
     for (int k = 0; k < n->nreac; k++) {
         if (!n->extrapolate[k] && T < n->tmin[k]) rate[k] = 0.0;
@@ -682,6 +715,24 @@ void ucl_update_physics(const ucl_net *n, ucl_point *pt, double current_time_s)
     pt->coldens = pt->cloud_size / 1.0 * pt->density;
     pt->av = pt->p[P_BASEAV] + pt->coldens / 1.6e21;
     pt->dust_temp = pt->gas_temp;
+    if (pt->p[P_CRATTEN] != 0.0 && pt->p[P_IMPROVEDH2CRP] != 0.0) {
+        /* physics-core.f90:117-155: H2 dissociation by attenuated cosmic rays, polynomial in
+         * log10 of the column density (Padovani et al. 2018 fits, L or H model) */
+        static const double ck_l[10] = {1.582911005330e7, -6.465722684896e6, 1.172189025424e6, -1.237950798073e5,
+                                        8.393404654312e3, -3.788811358130e2, 1.138688455029e1, -2.197136304567e-1,
+                                        2.469841278950e-3, -1.232393620924e-5};
+        static const double ck_h[10] = {1.217227462831e7, -4.989649250304e6, 9.079152156645e5, -9.624890825395e4,
+                                        6.551161486120e3, -2.968976216187e2, 8.959037875226e0, -1.735757324445e-1,
+                                        1.959267277734e-3, -9.816996707980e-6};
+        const double *ck = pt->p[P_IONMODEL] != 0.0 ? ck_h : ck_l;
+        const double lc = log10(pt->coldens);
+        double sum = 0.0, pw = 1.0;
+        for (int k = 0; k < 10; k++) {
+            sum += ck[k] * pw;
+            pw *= lc;
+        }
+        pt->h2crp = pow(10.0, sum) * pt->p[P_ZETA];
+    }
     if ((int)pt->p[P_MODEL] == 1) {
         static const double tempa[6] = {1.927e-1, 4.8560e-2, 7.8470e-3, 9.6966e-4, 1.706e-4, 4.74e-7};
         static const float tempb[6] = {0.5339f, 0.6255f, 0.8395f, 1.085f, 1.289f, 1.98f};

@@ -22,7 +22,15 @@ enum { T_PHOTON, T_CRP, T_CRPHOT, T_FREEZE, T_DESORB, T_THERM, T_DESOH2, T_DESCR
 /* point-parameter vector (same layout as oracle/ref_driver.f90) */
 enum { P_DENS, P_TEMP, P_ZETA, P_RADFIELD, P_BASEAV, P_ROUT, P_RIN, P_FINALTIME, P_FREEFALL,
        P_FINALDENS, P_RELTOL, P_ABSFAC, P_ABSMIN, P_MODEL, P_TEMPINDX, P_MAXTEMP, P_FREEZEFAC,
-       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_NPARAM = 24 };
+       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_MAXSTEPS,
+       /* initial elemental abundances (defaultparameters.f90:70-90) */
+       P_METALLICITY, P_ION, P_FH, P_FHE, P_FC, P_FO, P_FN, P_FS, P_FMG, P_FSI, P_FCL, P_FP, P_FFE, P_FF,
+       P_FD, P_FLI, P_FNA, P_FPAH, P_F15N, P_F13C, P_F18O,
+       /* desorption and photon constants (defaultparameters.f90:113-120) */
+       P_EBMAXH2, P_EBMAXCR, P_EBMAXUVCR, P_EPSILON, P_UV_YIELD, P_PHI, P_UVCREFF, P_OMEGA,
+       /* switches (defaultparameters.f90:40-49) */
+       P_DESORB, P_H2DESORB, P_CRDESORB, P_UVDESORB, P_THERMDESORB, P_CRATTEN, P_IONMODEL, P_IMPROVEDH2CRP,
+       P_ENFORCECHARGE, P_NPARAM = 64 };
 
 typedef struct {
     int32_t nspec, neq, nreac, nice, nsurfl, nbulkl, nfp, nswap;   /* list lengths */
@@ -44,6 +52,9 @@ typedef struct {
     /* special reactions (0-based) */
     int32_t r_h2form_ct, r_h2form_erdes, r_h2form_er, r_hfreeze, r_efreeze, r_h2freeze,
         r_h2_hv, r_co_hv, r_c_hv;
+    int32_t r_h2_crp;           /* H2 + CRP -> H + H, or -1 */
+    const double *gar_params;   /* [ngar*7], one row per GAR reaction */
+    int32_t ngar;
 } ucl_net;
 
 /* per-point physical state carried between intervals */
@@ -53,6 +64,7 @@ typedef struct {
     double phi;                 /* CR desorption yield; leaks between calls (rates.f90:118,322-326) */
     double h2col, cocol, ccol;  /* set by ucl_rates, reused by ucl_rhs */
     double safe_mantle, safe_bulk, blr;
+    double h2crp;               /* H2 CR dissociation rate the physics update sets (physics-core.f90:143-155) */
 } ucl_point;
 
 int ucl_point_init(const ucl_net *net, const double *params, ucl_point *pt, double *y0);

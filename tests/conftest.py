@@ -36,3 +36,16 @@ def reflib():
 def gpu_net(tables):
     from uclchem_amd import gpu
     return gpu.Network(tables)
+
+
+@pytest.fixture(scope="session")
+def oracle_of():
+    """Oracle (C restatement) bound to a given network's tables, one per network."""
+    from oracle import Oracle
+    cache = {}
+
+    def get(t):
+        if t.name not in cache:
+            cache[t.name] = Oracle(t)
+        return cache[t.name]
+    return get

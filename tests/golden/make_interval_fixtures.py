@@ -1,0 +1,137 @@
+"""Write single-interval reference fixtures (tests/golden/ref_intervals.npz) from the reference
+Fortran hot path (oracle/_ref/libucref.so).
+
+Run in the build container (needs the reference build):
+    python tests/golden/make_interval_fixtures.py [--procs 6] [--budget 2400]
+
+Each row is one updateChemistry interval of the reference's model loop
+(src/fortran_src/chemistry.f90:145-239): the point's parameter vector, the state the interval
+starts from (abundances; density, gasTemp, dustTemp, Av, cloudSize, phi, abstol_factor, time),
+the interval's end, and the reference's result (state, time reached, steps, attempts).  The
+states come from the reference's own trajectories walked interval by interval
+(ucref_interval + ucref_update_physics, which continue the loop of ucref_model bit for bit):
+
+* "hot": hot-core warm-up points (temp_indx 4, maxTemp 300 K: the hot_core_grid workload)
+  at n_H near 1e5, 1e6, 1e7 and 1e8 cm^-3, walked from t = 0 for a wall-time budget;
+* "tail": points 594, 380, 1203 and 2549 of the 4096-point static-cloud grid, whose late
+  intervals need ~1e5 reference steps.
+
+Rows kept per trajectory: the hardest intervals the reference completes in one attempt;
+where its ten attempts end short of the interval's end, the stretch its first attempt covers
+(the target is then the time that attempt reached); and, for hot-core points, the last
+walked state with its temperature raised to 20, 50, 100 and 300 K ("jump" rows: the
+desorption conditions of the warm-up, one interval each, from an icy state the walk reached).
+"""
+from __future__ import annotations
+
+import argparse
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from uclchem_amd import workloads  # noqa: E402
+from uclchem_amd.params import INDEX, SECONDS_PER_YEAR as SPY  # noqa: E402
+
+TAIL_POINTS = (594, 380, 1203, 2549)
+HOT_TARGETS = (1e5, 1e6, 1e7, 1e8)
+JUMP_TEMPS = (20.0, 50.0, 100.0, 300.0)
+
+
+def hot_points():
+    """Indices of the hot-core workload whose n_H is nearest each of HOT_TARGETS (first 256)."""
+    g = workloads.hot_core_grid(256)
+    ln = np.log10(g[:, INDEX["initialdens"]])
+    return [int(np.argmin(np.abs(ln - np.log10(t)))) for t in HOT_TARGETS]
+
+
+def walk(job):
+    """Walk one trajectory interval by interval; returns its candidate rows."""
+    kind, idx, budget = job
+    from oracle import RefLib, Oracle
+    from uclchem_amd import makerates_hip as mh
+    ref = RefLib()
+    orc = Oracle(mh.load_network(mh.network_path("default")))
+    p = (workloads.hot_core_grid(256) if kind == "hot" else workloads.static_cloud_grid(4096))[idx]
+    model = int(p[INDEX["model"]])
+    mxstep = float(p[INDEX["mxstep"]])
+    t_wall = time.time()
+    rc, _, _, y, phys = ref.model_until(p, 1.0)
+    rows = []
+    while rc == 0 and phys[7] / SPY < 1e6 and time.time() - t_wall < budget:
+        t1 = orc.next_target(model, phys[7] / SPY)
+        ts = time.time()
+        rc, y1, ph1, st = ref.interval(p, y, phys, t1)
+        wall = time.time() - ts
+        reached = ph1[7] >= t1
+        if reached and st[1] == 1:
+            rows.append(dict(kind=kind, idx=idx, how="interval", params=p, y_in=y, phys_in=phys, target=t1,
+                             y_ref=y1, phys_ref=ph1, rc=rc, steps=st[0], attempts=st[1], wall=wall))
+        elif not reached:   # the stretch the first attempt covers
+            rc1, y2, ph2, st2 = ref.interval(p, y, phys, t1, maxsteps=mxstep)
+            if ph2[7] > phys[7]:
+                rows.append(dict(kind=kind, idx=idx, how="first-attempt", params=p, y_in=y, phys_in=phys,
+                                 target=ph2[7], y_ref=y2, phys_ref=ph2, rc=0, steps=st2[0], attempts=st2[1],
+                                 wall=wall))
+        ref.update_physics(ph1[7])
+        y, phys = ref.state()
+        print(f"{kind} {idx}: t={phys[7] / SPY:.4g} yr T={phys[1]:.1f} K steps {st[0]} attempts {st[1]} "
+              f"{wall:.1f}s", flush=True)
+    if kind == "hot":   # temperature jumps from the last walked state
+        for T in JUMP_TEMPS:
+            ph = phys.copy()
+            ph[1] = ph[2] = T
+            t1 = orc.next_target(model, ph[7] / SPY)
+            ts = time.time()
+            rc, y1, ph1, st = ref.interval(p, y, ph, t1)
+            rows.append(dict(kind=kind, idx=idx, how=f"jump {T:.0f} K", params=p, y_in=y, phys_in=ph, target=t1,
+                             y_ref=y1, phys_ref=ph1, rc=rc, steps=st[0], attempts=st[1], wall=time.time() - ts))
+            print(f"{kind} {idx}: jump to {T} K: steps {st[0]} attempts {st[1]} reached "
+                  f"{ph1[7] >= t1}", flush=True)
+    return rows
+
+
+def select(rows, n_hard=3, n_first=2):
+    """Per trajectory: the n_hard single-attempt intervals with the most steps, the n_first
+    first-attempt rows with the most steps, every jump row whose interval the reference ends."""
+    out = []
+    keys = sorted({(r["kind"], r["idx"]) for r in rows})
+    for key in keys:
+        mine = [r for r in rows if (r["kind"], r["idx"]) == key]
+        hard = sorted([r for r in mine if r["how"] == "interval"], key=lambda r: -r["steps"])[:n_hard]
+        first = sorted([r for r in mine if r["how"] == "first-attempt"], key=lambda r: -r["steps"])[:n_first]
+        jump = [r for r in mine if r["how"].startswith("jump") and r["phys_ref"][7] >= r["target"]]
+        out += hard + first + jump
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--procs", type=int, default=6)
+    ap.add_argument("--budget", type=float, default=2400.0, help="wall seconds per hot-core trajectory walk")
+    ap.add_argument("--tail-budget", type=float, default=900.0, help="wall seconds per tail-point walk")
+    a = ap.parse_args()
+    jobs = [("hot", i, a.budget) for i in hot_points()] + [("tail", i, a.tail_budget) for i in TAIL_POINTS]
+    with mp.get_context("fork").Pool(a.procs) as pool:
+        rows = [r for part in pool.map(walk, jobs, chunksize=1) for r in part]
+    sel = select(rows)
+    cols = dict(kind=np.array([r["kind"] for r in sel]), idx=np.array([r["idx"] for r in sel]),
+                how=np.array([r["how"] for r in sel]))
+    for k in ("params", "y_in", "phys_in", "target", "y_ref", "phys_ref", "rc", "steps", "attempts", "wall"):
+        cols[k] = np.array([r[k] for r in sel])
+    np.savez_compressed(os.path.join(HERE, "ref_intervals.npz"), **cols)
+    for r in sel:
+        print(f"{r['kind']:4s} {r['idx']:5d} {r['how']:14s} T={r['phys_in'][1]:6.1f} K "
+              f"t={r['phys_in'][7] / SPY:10.4g} -> {r['target'] / SPY:10.4g} yr steps {r['steps']:6d} "
+              f"attempts {r['attempts']} rc {r['rc']} ({r['wall']:.1f}s)")
+
+
+if __name__ == "__main__":
+    main()

@@ -10,6 +10,12 @@ Outputs (small .npz files next to this script):
   ref_hot_core.npz       the reference model for seeded hot-core points
   example_collapse_hotcore.npz  examples/example-output/phase1-full.dat, startcollapse.dat,
                          phase2-full.dat (the collapse and hot-core example models)
+  ref_collapse500.npz    the reference compiled on the 500/6000 synthetic network (written by
+                         uclchem_amd/reference_format.py: oracle/_ref/net_synthetic_500_6000)
+                         for 16 points of the 8192-point free-fall grid, one per density stratum
+  ref_serving.npz        the reference, compiled on each request's network, for the seeded
+                         24-request stream over the default and synthetic networks
+                         (t_end log-uniform 1e3..1e7 yr)
 """
 from __future__ import annotations
 
@@ -31,19 +37,31 @@ from uclchem_amd import workloads  # noqa: E402
 REF_EXAMPLE = "/root/reference/examples/example-output/static-full.dat"
 
 
-def _run_one(p):
+SERVING_NETS = ["synthetic_150_1500", "default", "synthetic_500_6000"]
+
+
+def _lib_path(net):
+    if net is None:
+        return None
+    return os.path.join(ROOT, "oracle", "_ref", f"net_{net}", "libucref.so")
+
+
+def _run_one(job):
+    p, net = job if isinstance(job, tuple) else (job, None)
     from oracle import RefLib
-    ref = RefLib()
+    ref = RefLib() if net is None else RefLib(_lib_path(net))
     t0 = time.time()
-    rc, rows, stats = ref.model(p, maxrec=2000)
+    rc, rows, stats = ref.model(p, maxrec=4000)
     return rc, rows[-1].copy(), rows[:, 0].copy(), stats, time.time() - t0
 
 
-def run_models(params, procs):
+def run_models(params, procs, nets=None):
+    jobs = list(params) if nets is None else list(zip(params, nets))
     with mp.get_context("fork").Pool(procs) as pool:
-        out = pool.map(_run_one, list(params))
+        out = pool.map(_run_one, jobs, chunksize=1)
     rc = np.array([o[0] for o in out])
-    final = np.stack([o[1] for o in out])
+    rows = [o[1] for o in out]
+    final = np.stack(rows) if len({len(r) for r in rows}) == 1 else rows   # networks of different sizes
     stats = np.stack([o[3] for o in out])
     wall = np.array([o[4] for o in out])
     nrows = np.array([len(o[2]) for o in out])
@@ -74,13 +92,158 @@ def example_models():
                         phase2_final=r2[-1, 8:])
 
 
+def _model_capped(args):
+    """One reference model in a child process with a wall-time cap; None when it is exceeded."""
+    import subprocess
+    import tempfile
+    p, cap = args
+    with tempfile.NamedTemporaryFile(suffix=".npz") as tmp:
+        code = ("import sys,time,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
+                "p=np.array([float(x) for x in sys.argv[1].split(',')]); t=time.time(); "
+                "rc,rows,st=RefLib().model(p,maxrec=2000); "
+                "np.savez(sys.argv[2],rc=rc,final=rows[-1],nrows=len(rows),stats=st,wall=time.time()-t)"
+                ) % os.path.join(ROOT, "oracle")
+        try:
+            subprocess.run([sys.executable, "-c", code, ",".join(repr(float(x)) for x in p), tmp.name],
+                           timeout=cap, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        except subprocess.TimeoutExpired:
+            return None
+        z = np.load(tmp.name)
+        return int(z["rc"]), z["final"].copy(), int(z["nrows"]), z["stats"].copy(), float(z["wall"])
+
+
+def grid4096_sample(n, procs, cap):
+    """n points of the benchmarked 4096-point static-cloud grid, one per initial-density
+    stratum (seeded draw), each run through the reference; a point whose reference run
+    exceeds `cap` seconds (the grid's pathologically stiff tail, covered interval by interval
+    by ref_intervals.npz) is replaced by the next draw from its stratum."""
+    from uclchem_amd.params import INDEX
+    g = workloads.static_cloud_grid(4096)
+    order = np.argsort(g[:, INDEX["initialdens"]], kind="stable")
+    rng = np.random.Generator(np.random.PCG64(5))
+    strata = [list(rng.permutation(s)) for s in np.array_split(order, n)]
+    chosen = [None] * n
+    with mp.get_context("fork").Pool(procs) as pool:
+        while any(c is None for c in chosen):
+            todo = [k for k in range(n) if chosen[k] is None]
+            cand = [int(strata[k].pop(0)) for k in todo]
+            res = pool.map(_model_capped, [(g[i], cap) for i in cand], chunksize=1)
+            for k, i, r in zip(todo, cand, res):
+                if r is not None:
+                    chosen[k] = (i, r)
+                else:
+                    print(f"grid point {i} exceeded {cap:.0f} s in the reference; drawing again", flush=True)
+    idx = np.array([c[0] for c in chosen])
+    res = [c[1] for c in chosen]
+    return idx, g[idx], np.array([r[0] for r in res]), np.stack([r[1] for r in res]), \
+        np.stack([r[3] for r in res]), np.array([r[4] for r in res]), np.array([r[2] for r in res])
+
+
+def nondefault_params(n=8, seed=9):
+    """n static-cloud points to 1e5 yr whose initial-abundance, desorption-constant and switch
+    parameters are all away from their defaults (seeded), in the product's parameter layout."""
+    from uclchem_amd.params import INDEX, make_params
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        sw = rng.integers(0, 2, 6)
+        out.append(make_params(
+            initialdens=10 ** rng.uniform(2, 6), initialtemp=rng.uniform(10, 60), zeta=10 ** rng.uniform(0, 1.5),
+            radfield=10 ** rng.uniform(0, 2), baseav=rng.uniform(1, 5), finaltime=1e5,
+            metallicity=rng.uniform(0.5, 1.5), ion=int(i % 3), fh=rng.uniform(0.2, 0.8), fhe=rng.uniform(0.08, 0.12),
+            fc=rng.uniform(1.2e-4, 2.4e-4), fo=rng.uniform(2.5e-4, 4.5e-4), fn=rng.uniform(4e-5, 8e-5),
+            fs=rng.uniform(2e-6, 6e-6), ebmaxh2=rng.uniform(800, 2000), ebmaxcr=rng.uniform(800, 2000),
+            ebmaxuvcr=rng.uniform(5e3, 2e4), epsilon=rng.uniform(0.005, 0.03), uv_yield=rng.uniform(0.01, 0.06),
+            phi=10 ** rng.uniform(4, 6), uvcreff=10 ** rng.uniform(-3.5, -2.5), omega=rng.uniform(0.3, 0.7),
+            desorb=float(sw[0] or i % 2), h2desorb=float(sw[1]), crdesorb=float(sw[2]), uvdesorb=float(sw[3]),
+            cosmicrayattenuation=float(sw[5]), ionmodel=float(i % 2), improvedh2crpdissociation=float(sw[5]),
+            enforcechargeconservation=float(sw[4])))
+        out[-1][INDEX["thermdesorb"]] = float(i % 2)       # the slot itself (the dict key sets uvdesorb)
+    return np.array(out)
+
+
+def collapse500_sample(n=16):
+    """Indices of n points of the 8192-point free-fall grid, one drawn per initial-density
+    stratum (seeded)."""
+    from uclchem_amd.params import INDEX
+    g = workloads.collapse_grid(8192)
+    order = np.argsort(g[:, INDEX["initialdens"]], kind="stable")
+    rng = np.random.Generator(np.random.PCG64(11))
+    return np.array(sorted(int(rng.choice(s)) for s in np.array_split(order, n)))
+
+
+def serving_stream():
+    """The seeded 24-request stream of the serving parity test."""
+    from uclchem_amd import serving
+    return serving.poisson_requests(24, rate_per_s=40.0, networks=SERVING_NETS, seed=7, t_end_range=(1e3, 1e7))
+
+
+def synthetic_fixtures(procs):
+    idx = collapse500_sample()
+    params = workloads.collapse_grid(8192)[idx]
+    t0 = time.time()
+    rc, final, stats, wall, nrows = run_models(params, procs, ["synthetic_500_6000"] * len(params))
+    np.savez_compressed(os.path.join(HERE, "ref_collapse500.npz"), index=idx, params=params, rc=rc, final=final,
+                        stats=stats, wall=wall, nrows=nrows)
+    print(f"ref_collapse500: {len(params)} points in {time.time() - t0:.1f}s, per-point wall median "
+          f"{np.median(wall):.2f}s max {wall.max():.2f}s, rc {set(rc.tolist())}", flush=True)
+    reqs = serving_stream()
+    params = np.array([p for _, _, p in reqs])
+    nets = [n for _, n, _ in reqs]
+    t0 = time.time()
+    rc, final, stats, wall, nrows = run_models(params, procs, nets)
+    width = max(len(f) for f in final)
+    fin = np.zeros((len(final), width))
+    for i, f in enumerate(final):
+        fin[i, :len(f)] = f
+    np.savez_compressed(os.path.join(HERE, "ref_serving.npz"), params=params, network=np.array(nets),
+                        arrival=np.array([a for a, _, _ in reqs]), rc=rc, final=fin, stats=stats, wall=wall,
+                        nrows=nrows)
+    print(f"ref_serving: {len(params)} requests in {time.time() - t0:.1f}s, per-request wall median "
+          f"{np.median(wall):.2f}s max {wall.max():.2f}s, rc {set(rc.tolist())}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=24)
     ap.add_argument("--hot", type=int, default=4)
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--examples-only", action="store_true", help="only convert the reference's example outputs")
+    ap.add_argument("--synthetic-only", action="store_true", help="only the synthetic-network fixtures")
+    ap.add_argument("--grid-only", action="store_true", help="only the 4096-grid sample (ref_cloud_grid.npz)")
+    ap.add_argument("--params-only", action="store_true", help="only the non-default-parameter points")
+    ap.add_argument("--alltypes-only", action="store_true", help="only the all-reaction-types network points")
+    ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     a = ap.parse_args()
+    if a.synthetic_only:
+        synthetic_fixtures(a.procs)
+        return
+    if a.alltypes_only:
+        # static clouds on the all-types check network (net_alltypes.npz, written by
+        # make_alltypes_network.py), through the reference compiled on it
+        params = workloads.static_cloud_grid(6, seed=31, final_time=1e5)
+        t0 = time.time()
+        rc, final, stats, wall, nrows = run_models(params, a.procs, ["alltypes"] * len(params))
+        np.savez_compressed(os.path.join(HERE, "ref_alltypes.npz"), params=params, rc=rc, final=final, stats=stats,
+                            wall=wall, nrows=nrows)
+        print(f"ref_alltypes: {len(params)} points in {time.time() - t0:.1f}s, rc {set(rc.tolist())}", flush=True)
+        return
+    if a.params_only:
+        params = nondefault_params()
+        t0 = time.time()
+        rc, final, stats, wall, nrows = run_models(params, a.procs)
+        np.savez_compressed(os.path.join(HERE, "ref_params.npz"), params=params, rc=rc, final=final, stats=stats,
+                            wall=wall, nrows=nrows)
+        print(f"ref_params: {len(params)} points in {time.time() - t0:.1f}s, rc {set(rc.tolist())}", flush=True)
+        return
+    if a.grid_only:
+        t0 = time.time()
+        idx, params, rc, final, stats, wall, nrows = grid4096_sample(a.grid, a.procs, a.cap)
+        np.savez_compressed(os.path.join(HERE, "ref_cloud_grid.npz"), index=idx, params=params, rc=rc, final=final,
+                            stats=stats, wall=wall, nrows=nrows)
+        print(f"ref_cloud_grid: {len(params)} points of the 4096-point grid in {time.time() - t0:.1f}s, per-point "
+              f"wall median {np.median(wall):.2f}s max {wall.max():.2f}s, rc {set(rc.tolist())}", flush=True)
+        return
     # 1. the reference's own example output at t = 1e6 yr
     lines = open(REF_EXAMPLE).read().splitlines()
     hdr = [h.strip() for h in lines[0].split(",")]

@@ -1,0 +1,88 @@
+"""GPU parity of single output intervals (ucg_interval, one updateChemistry call of
+src/fortran_src/chemistry.f90:145-239) against the reference Fortran on the same states.
+
+The fixture (tests/golden/ref_intervals.npz, written by make_interval_fixtures.py from
+oracle/_ref) holds states of the reference's own trajectories: hot-core warm-up points
+(temp_indx 4, n_H ~1e5..1e8) including their desorption temperatures, and the hard late
+intervals of the 4096-point static-cloud grid's tail points (594, 380, 1203, 2549), where the
+reference needs up to ~1e5 steps.  Gates: the interval's end is reached, every abundance above
+1e-14 within 1% of the reference, element totals conserved to 1e-10 over the interval.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import conserved_elements, element_totals
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _fixture():
+    path = os.path.join(GOLD, "ref_intervals.npz")
+    if not os.path.exists(path):
+        pytest.skip("ref_intervals.npz not generated")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("lu_bits", [64, 32])
+def test_intervals_match_reference(gpu_net, tables, lu_bits):
+    f = _fixture()
+    n = len(f["target"])
+    for q in range(n):
+        print(f"row {q}: {f['kind'][q]} point {f['idx'][q]} {f['how'][q]} T={f['phys_in'][q, 1]:.1f} K "
+              f"t={f['phys_in'][q, 7] / 3.16e7:.4g} -> {f['target'][q] / 3.16e7:.4g} yr "
+              f"(reference: {f['steps'][q]} steps, {f['attempts'][q]} attempts)", flush=True)
+    from uclchem_amd.params import widen
+    out = gpu_net.interval(widen(f["params"]), f["phys_in"], f["y_in"], f["target"], lu_bits=lu_bits)
+    keep = [i for i in conserved_elements(tables)]
+    bad = []
+    for q in range(n):
+        assert out["status"][q] == 0, (q, out["status"][q])
+        assert out["phys"][q, 7] >= f["target"][q] * (1 - 1e-12), (q, out["phys"][q, 7], f["target"][q])
+        ref = f["y_ref"][q, :tables.nspec]
+        m = ref > 1e-14
+        rel = np.abs(out["y"][q, :tables.nspec][m] - ref[m]) / ref[m]
+        e0, e1 = element_totals(tables, f["y_in"][q]), element_totals(tables, out["y"][q])
+        k = [i for i in keep if e0[i] > 0]
+        cons = np.abs(e1[k] - e0[k]) / e0[k]
+        if rel.max() > 0.01 or cons.max() > 1e-10:
+            bad.append((q, str(f["how"][q]), float(rel.max()), float(cons.max())))
+        # phi and abstol_factor leave the interval as the reference's do
+        assert out["phys"][q, 5] == pytest.approx(f["phys_ref"][q, 5], rel=1e-12)
+        assert out["phys"][q, 6] == f["phys_ref"][q, 6] or f["how"][q] == "first-attempt"
+    assert not bad, bad
+
+
+def test_interval_continues_the_full_solve(gpu_net, tables):
+    """Chaining ucg_interval with the loop's physics update reproduces ucg_solve_batch's
+    own intervals (cloud schedule to 1e3 yr): the same kernel path either way."""
+    from uclchem_amd import workloads
+    from uclchem_amd.params import INDEX
+    p = workloads.static_cloud_grid(4)
+    p[:, INDEX["finaltime"]] = 1e3
+    full = gpu_net.solve(p, max_rec=40)
+    # restart every point from its output row k (time t_k) and integrate to row k+1
+    for k in (3, 8):
+        rows = full["rec"][:, k]
+        nxt = full["rec"][:, k + 1]
+        y = np.concatenate([rows[:, 4:], rows[:, 1:2]], axis=1)
+        phys = np.zeros((len(p), 8))
+        phys[:, 0] = rows[:, 1]
+        phys[:, 1] = phys[:, 2] = rows[:, 2]
+        phys[:, 3] = rows[:, 3]
+        phys[:, 4] = (p[:, INDEX["rout"]] - p[:, INDEX["rin"]]) * 3.086e18
+        # phi as the previous interval's rates left it (rates.f90:321-325, from that interval's
+        # start state: row k-1; every interval here ends in one attempt)
+        prev = full["rec"][:, k - 1]
+        ne = prev[:, 4 + tables.scalars["nelec"]]
+        phi = p[:, INDEX["radfield"]] * np.exp(-2.5 * prev[:, 3]) * np.sqrt(prev[:, 2]) / (prev[:, 1] * ne)
+        phys[:, 5] = np.clip(phi, 1e2, 1e6)
+        phys[:, 6] = p[:, INDEX["abstol_factor"]]
+        phys[:, 7] = rows[:, 0] * 3.16e7
+        out = gpu_net.interval(p, phys, y, nxt[:, 0] * 3.16e7)
+        assert np.all(out["status"] == 0)
+        m = nxt[:, 4:] > 1e-14
+        rel = np.abs(out["y"][:, :tables.nspec][m] - nxt[:, 4:][m]) / nxt[:, 4:][m]
+        assert rel.max() < 1e-6, rel.max()

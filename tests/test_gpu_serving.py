@@ -1,6 +1,7 @@
 """GPU tests of the serving loop (request queues, continuous batching over three networks).
 
-Each result is the same alone and in the mix: a request answered by a queue while other
+Served results match the reference Fortran on every network of the mix (a seeded stream with
+t_end up to 1e7 yr).  Each result is the same alone and in the mix: a request answered by a queue while other
 requests of three networks arrive (Poisson stream) and run beside it is bit-identical to
 the same point solved alone by ucg_solve_batch (the per-point computation never depends
 on which workgroup runs it or on what else is resident).  Error paths are forced from the
@@ -45,6 +46,40 @@ def test_results_same_alone_and_in_mix(nets):
         assert np.array_equal(t.y, alone["y"][0]), name
         assert np.array_equal(t.stats, alone["stats"][0]), name
         assert t.t_done >= t.t_arrival
+
+
+def test_served_results_match_the_reference(nets):
+    """The seeded 24-request stream of tests/golden/ref_serving.npz (t_end log-uniform
+    1e3..1e7 yr over the three networks) served with continuous admission: every result within
+    1% of the reference Fortran compiled on the request's network (the default network's
+    shipped sources; the synthetic ones written by uclchem_amd/reference_format.py) on every
+    abundance above 1e-14."""
+    import os
+    from uclchem_amd import serving
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_serving.npz")
+    if not os.path.exists(path):
+        pytest.skip("ref_serving.npz not generated")
+    f = np.load(path)
+    from uclchem_amd.params import widen
+    reqs = [(float(a), str(n), p) for a, n, p in zip(f["arrival"], f["network"], widen(f["params"]))]
+    slots = serving.plan_slots(nets, n_cu=64, hbm_budget_bytes=8e9)
+    srv = serving.Server(nets, slots, capacity=64)
+    try:
+        done = serving.run_stream(srv, reqs, timeout_s=500.0,
+                                  progress=lambda d, s, t: print(f"[serving] {d}/{s} done at {t:.0f}s", flush=True))
+    finally:
+        srv.close()
+    done = sorted(done, key=lambda x: x.t_arrival)
+    bad = []
+    for q, ((arr, name, p), t) in enumerate(zip(reqs, done)):
+        assert t.network == name and t.status == 0 and f["rc"][q] == 0
+        ns = nets[name].nspec
+        ref = f["final"][q, 4:4 + ns]
+        m = ref > 1e-14
+        r = np.abs(t.y[:ns][m] - ref[m]) / ref[m]
+        if r.max() > 0.01:
+            bad.append((q, name, float(p[7]), float(r.max())))
+    assert not bad, bad
 
 
 def test_queue_errors_are_reported(nets):

@@ -122,6 +122,48 @@ def test_oracle_rates_rhs_bit_exact_vs_reference(reflib, oracle, tables, seed):
             y[-1] = y0[-1]
 
 
+def _nondefault(rng):
+    """A seeded parameter vector away from every default the elemental start, the
+    desorption constants and the switches read."""
+    from oracle import make_params
+    sw = rng.integers(0, 2, 6)
+    return make_params(dens=10 ** rng.uniform(2, 7), temp=rng.uniform(10, 200), zeta=10 ** rng.uniform(0, 2),
+                       radfield=10 ** rng.uniform(0, 3), baseav=rng.uniform(0, 5), finaltime=1e5,
+                       metallicity=rng.uniform(0.3, 2.0), ion=int(rng.integers(0, 3)), fh=rng.uniform(0.1, 0.9),
+                       fhe=rng.uniform(0.05, 0.15), fc=rng.uniform(1e-4, 3e-4), fo=rng.uniform(2e-4, 5e-4),
+                       fn=rng.uniform(3e-5, 8e-5), fs=rng.uniform(1e-6, 1e-5), fmg=rng.uniform(1e-6, 4e-6),
+                       fsi=rng.uniform(1e-6, 3e-6), fcl=rng.uniform(1e-8, 5e-8), fp=rng.uniform(5e-8, 1e-7),
+                       ff=rng.uniform(1e-8, 5e-8), ebmaxh2=rng.uniform(500, 3000), ebmaxcr=rng.uniform(500, 3000),
+                       ebmaxuvcr=rng.uniform(2e3, 2e4), epsilon=rng.uniform(0.005, 0.05),
+                       uv_yield=rng.uniform(0.01, 0.1), phi=10 ** rng.uniform(3, 6), uvcreff=10 ** rng.uniform(-4, -2),
+                       omega=rng.uniform(0.2, 0.8), desorb=float(sw[0] or sw[1]), h2desorb=float(sw[1]),
+                       crdesorb=float(sw[2]), uvdesorb=float(sw[3]), thermdesorb=float(sw[4]),
+                       cosmicrayattenuation=float(sw[5]), ionmodel=float(rng.integers(0, 2)),
+                       improvedh2crpdissociation=float(sw[5]))
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_oracle_nondefault_parameters_bit_exact_vs_reference(reflib, oracle, tables, seed):
+    """Elemental start and rates with every initial-abundance, desorption-constant and
+    switch parameter away from its default: the C restatement bit-exact against the compiled
+    reference (driven through its parameter globals)."""
+    rng = np.random.default_rng(seed)
+    for trial in range(8):
+        p = _nondefault(rng)
+        rc, y0r = reflib.init(p)
+        pt, y0 = oracle.init_point(p)
+        assert rc == 0 and np.array_equal(y0r, y0), trial
+        _, y = random_state(tables, oracle, rng, p, bare_grains=(trial % 4 == 0))
+        for call in range(2):
+            rr, yr, sr = reflib.rates(y)
+            ro, yo, so = oracle.rates(pt, y)
+            assert np.array_equal(rr, ro), (trial, int(np.argmax(np.abs(rr - ro))))
+            assert np.array_equal(sr, so)
+            assert np.array_equal(reflib.rhs(yr), oracle.rhs(pt, ro, yr)[0])
+            y = np.abs(y * (1 + 0.1 * rng.standard_normal(tables.neq)))
+            y[-1] = y0[-1]
+
+
 def test_photo_functions_vs_reference_procedures(reflib, oracle):
     """The reference's own H2/CO photodissociation and shielding procedures, called directly."""
     L = reflib.lib

@@ -28,16 +28,38 @@ def test_grid_av_mapping():
 
 def test_oracle_and_product_param_layouts_agree():
     import oracle as orc
-    # the trailing slot (maxsteps, a work bound with no reference counterpart) is honoured by
-    # the reference driver too, so both layouts carry it
-    assert params.NAMES[-1] == "maxsteps" and params.DEFAULTS["maxsteps"] == 0.0
-    assert orc.P_NAMES == [
-        "dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finaltime", "freefall", "finaldens",
-        "reltol", "abstol_factor", "abstol_min", "model", "tempindx", "maxtemp", "freezefactor",
-        "endatfinaldensity", "freefallfactor", "mxstep", "maxsteps"]
-    a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0, maxsteps=5e4)
-    b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0, maxsteps=5e4)
+    # the same 64 slots in the same order (the oracle spells the first two dens/temp); the
+    # trailing work bound (maxsteps, no reference counterpart) is honoured by the reference
+    # driver too
+    assert len(params.NAMES) <= params.NPARAM == orc.P_NPARAM == 64
+    assert orc.P_NAMES[:2] == ["dens", "temp"] and orc.P_NAMES[2:] == params.NAMES[2:]
+    for name in params.NAMES[2:]:
+        assert orc.DEFAULTS[name] == params.DEFAULTS[name], name
+    a = orc.make_params(dens=3e3, temp=20.0, zeta=2.0, maxsteps=5e4, metallicity=0.5, ion=1, phi=3e4)
+    b = params.make_params(initialdens=3e3, initialtemp=20.0, zeta=2.0, maxsteps=5e4, metallicity=0.5, ion=1,
+                           phi=3e4)
     assert np.array_equal(a, b)
+
+
+def test_parameter_dict_quirks_of_the_reference_parser():
+    """dictionaryParser's thermdesorb branch sets uvdesorb (wrap.f90:802-803), and ionModel
+    is the character 'L' or 'H'; default-real literals keep their float32 values."""
+    p = params.make_params({"thermdesorb": False, "ionModel": "H"})
+    assert p[params.INDEX["uvdesorb"]] == 0.0 and p[params.INDEX["thermdesorb"]] == 1.0
+    assert p[params.INDEX["ionmodel"]] == 1.0
+    d = params.make_params()
+    assert d[params.INDEX["epsilon"]] == float(np.float32(0.01))
+    assert d[params.INDEX["fhe"]] == float(np.float32(0.1))
+    import pytest
+    with pytest.raises(ValueError):
+        params.make_params(ionModel="X")
+
+
+def test_widen_keeps_the_early_layout():
+    old = params.make_params(initialdens=5e3, maxsteps=7.0)[:params.NPARAM_V1]
+    w = params.widen(old)
+    assert w.shape == (1, params.NPARAM)
+    assert np.array_equal(w[0], params.make_params(initialdens=5e3, maxsteps=7.0))
 
 
 def test_collapse_and_hot_core_workloads():

@@ -22,7 +22,7 @@ def test_densest_first_is_a_stable_descending_permutation():
 
 def test_explicit_order_checked():
     from uclchem_amd.gpu import schedule_order
-    p = torch.zeros((5, 24), dtype=torch.float64)
+    p = torch.zeros((5, 64), dtype=torch.float64)
     assert schedule_order(p, torch.tensor([4, 3, 2, 1, 0])).tolist() == [4, 3, 2, 1, 0]
     for bad in ([0, 1, 2, 3], [0, 1, 2, 3, 3], [0, 1, 2, 3, 5]):
         with pytest.raises(ValueError):
@@ -40,7 +40,7 @@ def test_ordered_solve_scatters_outputs_back(monkeypatch):
     import ctypes
     from uclchem_amd import gpu
     from uclchem_amd.params import INDEX
-    n, neq, nprm = 6, 4, 24
+    n, neq, nprm = 6, 4, 64
     seen = {}
 
     def arr(ptr, count, ctype):
@@ -84,7 +84,7 @@ def test_host_solve_order_round_trip(monkeypatch):
     import ctypes
     from uclchem_amd import gpu
     from uclchem_amd.params import INDEX
-    n, neq, nprm, max_rec, nspec = 5, 4, 24, 2, 3
+    n, neq, nprm, max_rec, nspec = 5, 4, 64, 2, 3
     seen = {}
 
     def arr(ptr, count, ctype):
@@ -135,17 +135,17 @@ def test_device_solve_refuses_mismatched_buffers(monkeypatch):
     net = gpu.Network.__new__(gpu.Network)
     net.handle, net.neq, net.nspec = 0, 5, 4
     n = 3
-    host = dict(params=torch.zeros((n, 24), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
+    host = dict(params=torch.zeros((n, 64), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
                 status=torch.zeros(n, dtype=torch.int32), stats=torch.zeros((n, 8), dtype=torch.int32))
     with pytest.raises(ValueError):          # host tensors never reach the kernel
         net.solve_device(**host)
     assert not calls
     monkeypatch.setattr(gpu, "DEVICE_TYPE", "cpu")
-    good = dict(params=torch.zeros((n, 24), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
+    good = dict(params=torch.zeros((n, 64), dtype=torch.float64), y_out=torch.zeros((n, 5), dtype=torch.float64),
                 status=torch.zeros(n, dtype=torch.int32), stats=torch.zeros((n, 8), dtype=torch.int32))
     net.solve_device(**good)
     assert len(calls) == 1
-    bad = [("params", torch.zeros((n, 24), dtype=torch.float32)), ("params", torch.zeros((n, 23), dtype=torch.float64)),
+    bad = [("params", torch.zeros((n, 64), dtype=torch.float32)), ("params", torch.zeros((n, 24), dtype=torch.float64)),
            ("y_out", torch.zeros((n, 4), dtype=torch.float64)), ("status", torch.zeros(n, dtype=torch.int64)),
            ("stats", torch.zeros((n, 8), dtype=torch.int32).t().contiguous().t()),
            ("stats", torch.zeros((n + 1, 8), dtype=torch.int32))]
@@ -170,8 +170,8 @@ def test_piece_wrappers_check_widths(monkeypatch):
     monkeypatch.setattr(gpu, "_lib", object())        # any call into the library would fail loudly
     net = gpu.Network.__new__(gpu.Network)
     net.handle, net.neq, net.nspec, net.nreac = 0, 5, 4, 7
-    p, ph, y, r = np.zeros((2, 24)), np.zeros((2, 8)), np.zeros((2, 5)), np.zeros((2, 7))
-    for call in (lambda: net.rates(np.zeros((2, 23)), ph, y), lambda: net.rates(p, np.zeros((2, 7)), y),
+    p, ph, y, r = np.zeros((2, 64)), np.zeros((2, 8)), np.zeros((2, 5)), np.zeros((2, 7))
+    for call in (lambda: net.rates(np.zeros((2, 24)), ph, y), lambda: net.rates(p, np.zeros((2, 7)), y),
                  lambda: net.rates(p, ph, np.zeros((2, 4))), lambda: net.rhs(p, ph, np.zeros((2, 6)), y),
                  lambda: net.jacobian(p, ph, r, np.zeros((3, 5))),
                  lambda: net.newton_solve(p, ph, y, 1.0, np.zeros((2, 6)))):

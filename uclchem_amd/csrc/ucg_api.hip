@@ -243,6 +243,7 @@ int check_dev_buf(const void *p, size_t bytes, const char *name, bool nullable =
 
 int check_params(int n, const double *params)
 {
+    static_assert(UCG_NPARAM == ucg::P_NPARAM, "parameter layouts of the header and the kernels differ");
     for (int p = 0; p < n; p++) {
         const double *q = params + (size_t)p * UCG_NPARAM;
         if (!(q[UCG_P_DENS] > 0) || !(q[UCG_P_TEMP] > 0) || !(q[UCG_P_RELTOL] > 0) || !(q[UCG_P_ABSMIN] >= 0) ||
@@ -252,6 +253,12 @@ int check_params(int n, const double *params)
         if (model != 0 && model != 1) return fail(UCG_E_PARAM, "model must be 0 (cloud) or 1 (hot core)");
         if (model == 1 && ((int)q[UCG_P_TEMPINDX] < 1 || (int)q[UCG_P_TEMPINDX] > 6))
             return fail(UCG_E_PARAM, "tempindx must be 1..6");
+        if (q[UCG_P_IONMODEL] != 0.0 && q[UCG_P_IONMODEL] != 1.0)
+            return fail(UCG_E_PARAM, "ionmodel must be 0 ('L') or 1 ('H')");
+        if (q[UCG_P_OMEGA] == 1.0 || q[UCG_P_UVCREFF] == 0.0)
+            return fail(UCG_E_PARAM, "omega must differ from 1 and uvcreff from 0");
+        // improvedH2CRPDissociation without cosmicRayAttenuation is the reference's physics
+        // initialisation error, reported per point (UCG_E_PHYSICS_INIT), not refused here
     }
     return 0;
 }
@@ -459,8 +466,15 @@ static int check_tables(const ucg_network_desc *d)
             return bad("Jacobian contribution");
     for (int i = 0; i < 25; i++)
         if (d->special_species[i] < 0 || d->special_species[i] >= Q) return bad("special species index");
-    for (int i = 0; i < 9; i++)
-        if (d->special_reactions[i] < 0 || d->special_reactions[i] >= R) return bad("special reaction index");
+    for (int i = 0; i < 10; i++)   // the last, H2 + CRP, may be absent (-1)
+        if (d->special_reactions[i] < (i == 9 ? -1 : 0) || d->special_reactions[i] >= R)
+            return bad("special reaction index");
+    if (d->nion < 0 || !in(d->ion_list, d->nion, 0, S)) return bad("ion_list");
+    {   // one parameter row per GAR reaction, paired by position in the GAR range
+        const int a = d->type_ranges[2 * ucg::T_GAR], b = d->type_ranges[2 * ucg::T_GAR + 1];
+        const long ngar_needed = a < 0 ? 0 : b - a + 1;
+        if (d->ngar != ngar_needed || (d->ngar > 0 && !d->gar_params)) return bad("gar_params");
+    }
     // the device rate evaluator pairs ranges element-wise (rates.f90:243-279, 104-224)
     const int *tr = d->type_ranges;
     auto len = [&](int t) { return tr[2 * t] < 0 ? 0 : tr[2 * t + 1] - tr[2 * t] + 1; };
@@ -642,8 +656,16 @@ int ucg_network_load_file(const char *path, int *handle)
     d.extrapolate = keep_c.back().data();
     if ((rc = find("special_species", 1, 25, &p))) return rc;
     memcpy(d.special_species, p, sizeof d.special_species);
-    if ((rc = find("special_reactions", 1, 9, &p))) return rc;
+    if ((rc = find("special_reactions", 1, 10, &p))) return rc;
     memcpy(d.special_reactions, p, sizeof d.special_reactions);
+    if ((rc = find("nion", 1, 1, &p))) return rc;
+    memcpy(&d.nion, p, sizeof d.nion);
+    if (d.nion < 0) return fail(UCG_E_ARG, "network file record nion is negative");
+    if ((rc = geti("ion_list", d.nion, &d.ion_list))) return rc;
+    if ((rc = find("ngar", 1, 1, &p))) return rc;
+    memcpy(&d.ngar, p, sizeof d.ngar);
+    if (d.ngar < 0) return fail(UCG_E_ARG, "network file record ngar is negative");
+    if ((rc = getd("gar_params", 7 * (int64_t)d.ngar, &d.gar_params))) return rc;
     if ((rc = find("nw_sizes", 1, 10, &p))) return rc;
     memcpy(d.nw_sizes, p, sizeof d.nw_sizes);
     {
@@ -873,6 +895,11 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
     const int *sr = d->special_reactions;
     N.r_h2form_ct = sr[0]; N.r_h2form_er = sr[1]; N.r_h2form_erdes = sr[2]; N.r_hfreeze = sr[3];
     N.r_efreeze = sr[4]; N.r_h2freeze = sr[5]; N.r_h2_hv = sr[6]; N.r_co_hv = sr[7]; N.r_c_hv = sr[8];
+    N.r_h2_crp = sr[9];
+    N.nion = d->nion;
+    if ((rc = upload(h, d->ion_list, (size_t)d->nion, N.ion_list))) return rc;
+    N.ngar = d->ngar;
+    if ((rc = upload(h, d->gar_params, 7 * (size_t)d->ngar, N.gar))) return rc;
     HIPCHK(hipMalloc(&h.counter, sizeof(int)));
     h.allocs.push_back(h.counter);
     HIPCHK(hipMalloc(&h.cancel, sizeof(int)));

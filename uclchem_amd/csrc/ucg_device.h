@@ -53,7 +53,12 @@ enum { OP_NONE = -1, OP_BLR = -2, OP_SWAP = -3, OP_DIVSM = -4, FAC_D = -5 };
 // point-parameter vector layout (shared with include/uclchem_gpu.h)
 enum { P_DENS, P_TEMP, P_ZETA, P_RADFIELD, P_BASEAV, P_ROUT, P_RIN, P_FINALTIME, P_FREEFALL,
        P_FINALDENS, P_RELTOL, P_ABSFAC, P_ABSMIN, P_MODEL, P_TEMPINDX, P_MAXTEMP, P_FREEZEFAC,
-       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_MAXSTEPS, P_NPARAM = 24 };
+       P_ENDATFINALDENS, P_FREEFALLFAC, P_MXSTEP, P_MAXSTEPS,
+       P_METALLICITY, P_ION, P_FH, P_FHE, P_FC, P_FO, P_FN, P_FS, P_FMG, P_FSI, P_FCL, P_FP, P_FFE, P_FF,
+       P_FD, P_FLI, P_FNA, P_FPAH, P_F15N, P_F13C, P_F18O,
+       P_EBMAXH2, P_EBMAXCR, P_EBMAXUVCR, P_EPSILON, P_UV_YIELD, P_PHI, P_UVCREFF, P_OMEGA,
+       P_DESORB, P_H2DESORB, P_CRDESORB, P_UVDESORB, P_THERMDESORB, P_CRATTEN, P_IONMODEL, P_IMPROVEDH2CRP,
+       P_ENFORCECHARGE, P_NPARAM = 64 };
 
 // physical constants (constants.f90, surfacereactions.f90:33-51)
 constexpr double K_BOLTZ = 1.38065040e-16;
@@ -95,7 +100,11 @@ struct DevNet {
     int32_t nh, nh2, nhe, nc, ncx, nn, no, nf, np, nmg, nmgx, nna, nli, npah, nsx, nsix, nclx, nd,
         n18o, n15n, n13c, nelec, nco, nsurface, nbulk;
     int32_t r_h2form_ct, r_h2form_er, r_h2form_erdes, r_hfreeze, r_efreeze, r_h2freeze, r_h2_hv, r_co_hv,
-        r_c_hv;
+        r_c_hv, r_h2_crp;   // r_h2_crp: H2 + CRP -> H + H, or -1
+    int32_t nion;             // positive ions (names holding '+'), for enforceChargeConservation
+    const int32_t *ion_list;
+    int32_t ngar;             // GAR reactions; gar [ngar*7] their Weingartner & Draine parameters
+    const double *gar;
     // static-pattern sparse Newton plan (augmented order of size nw_N = neq + 2)
     int32_t nw_N, nw_ns, nw_T, nw_nsp, nw_nlev, nw_nph, nw_nval;
     int32_t nw_stage;            // 1: factors and solve program are staged in LDS; 0: they stay in the workspace

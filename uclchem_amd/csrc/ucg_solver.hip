@@ -21,7 +21,7 @@ __constant__ PhotoTables c_photo;
 
 // LDS scalar slots
 enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI, S_DENS, S_TGAS, S_TDUST,
-       S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_NSCAL = 24 };
+       S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_H2CRP, S_NSCAL = 24 };
 
 // Large collective routines can be kept out of line to bound the kernel's register
 // allocation (UCG_BIG=__attribute__((noinline))); inlined by default.
@@ -134,17 +134,26 @@ UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
     }
     if (tid() == 64) sc[S_SCAT] = scatter_dev(1000.0, av);
     __syncthreads();
-    const double sm = sc[S_SM], blr = sc[S_BLR], phi = sc[S_PHI];
+    const double sm = sc[S_SM], blr = sc[S_BLR], phi = sc[S_PHI];   // phi: the previous call's (rates.f90:118)
+    // this call's grain-assisted-recombination phi = G0 e^(-2.5 Av) T^0.5 / n_e (rates.f90:321-325):
+    // the GAR rates use it, CR desorption above still the previous one; stored for the next call
+    const double phi_new = fmin(fmax(radfield * exp(-2.5 * av) * sqrt(T) / (y[neq - 1] * y[N.nelec]), 1e2), 1e6);
     const double photo_bulk_pow = pow((double)(1.0f - 0.007f), 1.0 + 0.5 / blr);
     const double ice_ratio = (double)0.3f;
     const bool act_crp = type_active(N, T_CRP), act_ph = type_active(N, T_PHOTON),
                act_crph = type_active(N, T_CRPHOT), act_fr = type_active(N, T_FREEZE),
                act_h2d = type_active(N, T_DESOH2), act_cr = type_active(N, T_DESCR),
-               act_uv = type_active(N, T_DEUVCR), act_th = type_active(N, T_THERM),
-               act_lh = type_active(N, T_LH), act_er = type_active(N, T_ER);
+               act_uv = type_active(N, T_DEUVCR),
+               act_th = type_active(N, T_THERM) && prm[P_THERMDESORB] != 0.0,   // off: zero, no switch-off
+               act_lh = type_active(N, T_LH), act_er = type_active(N, T_ER),
+               act_io1 = type_active(N, T_IONOPOL1), act_io2 = type_active(N, T_IONOPOL2),
+               act_crs = type_active(N, T_CRS), act_exr = type_active(N, T_EXRELAX),
+               act_exs = type_active(N, T_EXSOLID), act_gar = type_active(N, T_GAR);
+    const int gar0 = N.type_ranges[2 * T_GAR];
     const int lh0 = N.type_ranges[2 * T_LH], lhd0 = N.type_ranges[2 * T_LHDES];
     const int er0 = N.type_ranges[2 * T_ER], erd0 = N.type_ranges[2 * T_ERDES];
     const double h2eff = h2_form_efficiency_dev(T, Td);
+    const bool desorb = prm[P_DESORB] != 0.0;
     // phase 1: base rates (every entry written)
     for (int k = tid(); k < N.nreac; k += NT) {
         const int t = N.rtype[k];
@@ -152,12 +161,13 @@ UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
         switch (t) {
         case T_CRP:
             if (act_crp) r = N.alpha[k] * zeta;
+            if (k == N.r_h2_crp && prm[P_IMPROVEDH2CRP] != 0.0) r = sc[S_H2CRP];   // rates.f90:39-41
             break;
         case T_PHOTON:
         case T_CRPHOT:
             if ((t == T_PHOTON && act_ph) || (t == T_CRPHOT && act_crph)) {
                 if (t == T_PHOTON) r = N.alpha[k] * exp(-N.gama[k] * av) * radfield / (double)1.7f;
-                else r = N.alpha[k] * N.gama[k] * 1.0 / (1.0 - 0.5) * zeta * pow(T / 300, N.beta[k]);
+                else r = N.alpha[k] * N.gama[k] * 1.0 / (1.0 - prm[P_OMEGA]) * zeta * pow(T / 300, N.beta[k]);
                 if (N.ice_kind[k] == 2) r = r * ice_ratio * photo_bulk_pow;
                 else if (N.ice_kind[k] == 1) r = r * ice_ratio;
             }
@@ -166,17 +176,18 @@ UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
             if (act_fr) r = freeze_out(N, prm, k, T, Td);
             break;
         case T_DESOH2:
-            if (act_h2d && sm > MIN_SURFACE_ABUND) r = N.gama[k] > 1.21e3 ? 0.0 : (double)0.01f * h2eff;
+            if (act_h2d && desorb && prm[P_H2DESORB] != 0.0 && sm > MIN_SURFACE_ABUND)
+                r = N.gama[k] > prm[P_EBMAXH2] ? 0.0 : prm[P_EPSILON] * h2eff;
             break;
         case T_DESCR:
-            if (act_cr && sm > MIN_SURFACE_ABUND)
-                r = N.gama[k] > 1.21e3 ? 0.0 : 4.0 * PI_F * zeta * 1.64e-4 * (GRAIN_SURFACEAREA_PER_H) * phi;
+            if (act_cr && desorb && prm[P_CRDESORB] != 0.0 && sm > MIN_SURFACE_ABUND)
+                r = N.gama[k] > prm[P_EBMAXCR] ? 0.0 : 4.0 * PI_F * zeta * 1.64e-4 * (GRAIN_SURFACEAREA_PER_H) * phi;
             break;
         case T_DEUVCR:
-            if (act_uv && sm > MIN_SURFACE_ABUND && zeta > 0) {
-                double v = GRAIN_CROSSSECTION_PER_H * (double)0.03f * 4.875e3 * zeta;
-                v = v * (1 + (radfield / 1.0e-3) * (1.0 / zeta) * exp((double)-1.8f * av));
-                r = N.gama[k] > 1.0e4 ? 0.0 : v;
+            if (act_uv && desorb && prm[P_UVDESORB] != 0.0 && sm > MIN_SURFACE_ABUND && zeta > 0) {
+                double v = GRAIN_CROSSSECTION_PER_H * prm[P_UV_YIELD] * 4.875e3 * zeta;
+                v = v * (1 + (radfield / prm[P_UVCREFF]) * (1.0 / zeta) * exp((double)-1.8f * av));
+                r = N.gama[k] > prm[P_EBMAXUVCR] ? 0.0 : v;
             }
             break;
         case T_THERM:
@@ -230,6 +241,30 @@ UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
         case T_TWOBODY:
             r = N.alpha[k] * pow(T / 300.0, N.beta[k]) * exp(-N.gama[k] / T);
             break;
+        case T_IONOPOL1:   // rates.f90:301-305 (KIDA ion-polar formula 1)
+            if (act_io1) r = N.alpha[k] * N.beta[k] * (0.62 + 0.4767 * N.gama[k] * sqrt(300.0 / T));
+            break;
+        case T_IONOPOL2:   // rates.f90:307-313 (formula 2; 300.0 and 10.526 are default-real literals)
+            if (act_io2)
+                r = N.alpha[k] * N.beta[k] *
+                    (1.0 + 0.0967 * N.gama[k] * sqrt(300.0 / T) + N.gama[k] * N.gama[k] * 300.0 / ((double)10.526f * T));
+            break;
+        case T_CRS:        // rates.f90:158-165: excitation by cosmic rays (Shingledecker et al. 2018)
+            if (act_crs) r = N.alpha[k] * (N.beta[k] * (N.gama[k] / 100) * ((double)8.6f * zeta * (double)1.3f));
+            break;
+        case T_EXRELAX:    // rates.f90:168-179: relaxation at the reactant's diffusion frequency
+            if (act_exr) r = N.rstat[5 * k];
+            break;
+        case T_EXSOLID:    // rates.f90:182-197: alpha (vA + vB) / (site density 1.8e-8)
+            if (act_exs) r = N.alpha[k] * N.rstat[5 * k];
+            break;
+        case T_GAR:        // rates.f90:316-332: grain-assisted recombination (Weingartner & Draine 2001)
+            if (act_gar) {
+                const double *g = N.gar + 7 * (k - gar0);
+                r = (double)0.6f * N.alpha[k] * g[0] /
+                    (1. + g[1] * pow(phi_new, g[2]) * (1. + g[3] * pow(T, g[4]) * pow(phi_new, -g[5] - g[6] * log(T))));
+            }
+            break;
         default:
             break;
         }
@@ -271,8 +306,7 @@ UCG_BIG __device__ void block_rates(const DevNet &N, const Lds L)
     // photo overrides and the phi update (rates.f90:316-342)
     if (tid() == 0) {
         rate[N.r_h2_hv] = h2_photodiss_dev(sc[S_H2COL], radfield, sc[S_SCAT]);
-        double ph = radfield * exp(-2.5 * av) * sqrt(T) / (y[neq - 1] * y[N.nelec]);
-        sc[S_PHI] = fmin(fmax(ph, 1e2), 1e6);
+        sc[S_PHI] = phi_new;
     }
     if (tid() == 64) rate[N.r_co_hv] = co_photodiss_dev(sc[S_H2COL], sc[S_COCOL], radfield, av);
     if (tid() == 128)
@@ -1996,35 +2030,71 @@ struct SolveArgs {
 };
 
 __device__ void elemental_start(const DevNet &N, const LDS double *prm, LDS double *y, double density)
-{   // chemistry.f90:63-105 with the default elemental abundances (defaultparameters.f90:68-90)
+{   // chemistry.f90:58-105: elemental abundances from the point's parameters, in the
+    // reference's order (species a network lacks share the density slot, restored last)
     const int neq = N.neq;
     for (int i = tid(); i < neq; i += NT) y[i] = MIN_ABUND;
     __syncthreads();
     if (tid() == 0) {
-        y[N.no] = 3.34e-04;
-        y[N.nn] = 6.18e-05;
-        y[N.nmg] = 2.256e-06;
-        y[N.np] = 7.78e-08;
-        y[N.nf] = 3.6e-08;
-        y[N.nna] = 0.0;
-        y[N.nli] = 0.0;
-        y[N.npah] = 0.0;
-        y[N.nsx] = 3.51e-6;
-        y[N.nsix] = 1.78e-06;
-        y[N.nclx] = 3.39e-08;
-        y[N.nc] = 1.0e-10;
-        y[N.ncx] = 1.77e-04;
-        y[N.n18o] = 0.0;
-        y[N.n15n] = 0.0;
-        y[N.n13c] = 0.0;
+        y[N.no] = prm[P_FO];
+        y[N.nn] = prm[P_FN];
+        y[N.nmg] = prm[P_FMG];
+        y[N.np] = prm[P_FP];
+        y[N.nf] = prm[P_FF];
+        y[N.nna] = prm[P_FNA];
+        y[N.nli] = prm[P_FLI];
+        y[N.npah] = prm[P_FPAH];
+        y[N.nsx] = prm[P_FS];
+        y[N.nsix] = prm[P_FSI];
+        y[N.nclx] = prm[P_FCL];
+        const double ion = prm[P_ION];      // how much carbon starts ionised
+        if (ion == 0.0) {
+            y[N.nc] = prm[P_FC];
+            y[N.ncx] = 1.0e-10;
+        } else if (ion == 1.0) {
+            y[N.nc] = prm[P_FC] * 0.5;
+            y[N.ncx] = prm[P_FC] * 0.5;
+        } else if (ion == 2.0) {
+            y[N.nc] = 1.0e-10;
+            y[N.ncx] = prm[P_FC];
+        }
+        y[N.n18o] = prm[P_F18O];
+        y[N.n15n] = prm[P_F15N];
+        y[N.n13c] = prm[P_F13C];
         y[N.nelec] = y[N.ncx] + y[N.nsix] + y[N.nsx] + y[N.nclx] + y[N.nmgx];
-        y[N.nh] = 0.5;
-        y[N.nh2] = 0.5 * (1.0 - 0.5);
-        y[N.nd] = 0.0;
-        y[N.nhe] = (double)0.1f;
+    }
+    __syncthreads();
+    const double met = prm[P_METALLICITY];
+    for (int i = tid(); i < neq; i += NT) y[i] = y[i] * met;
+    __syncthreads();
+    if (tid() == 0) {
+        y[N.nh] = prm[P_FH];
+        y[N.nh2] = 0.5 * (1.0 - prm[P_FH]);
+        y[N.nd] = prm[P_FD];
+        y[N.nhe] = prm[P_FHE];
         y[neq - 1] = density;
     }
     __syncthreads();
+}
+
+// H2 dissociation by cosmic rays attenuated by the column (physics-core.f90:117-155, the
+// improvedH2CRPDissociation rate the physics update sets): a degree-9 polynomial in
+// log10(coldens), coefficients of the L or H model
+__device__ inline double h2_crp_rate_dev(double coldens, double zeta, bool high)
+{
+    const double ck_l[10] = {1.582911005330e7, -6.465722684896e6, 1.172189025424e6, -1.237950798073e5,
+                             8.393404654312e3, -3.788811358130e2, 1.138688455029e1, -2.197136304567e-1,
+                             2.469841278950e-3, -1.232393620924e-5};
+    const double ck_h[10] = {1.217227462831e7, -4.989649250304e6, 9.079152156645e5, -9.624890825395e4,
+                             6.551161486120e3, -2.968976216187e2, 8.959037875226e0, -1.735757324445e-1,
+                             1.959267277734e-3, -9.816996707980e-6};
+    const double lc = log10(coldens);
+    double sum = 0.0, pw = 1.0;
+    for (int k = 0; k < 10; k++) {
+        sum += (high ? ck_h[k] : ck_l[k]) * pw;
+        pw *= lc;
+    }
+    return pow(10.0, sum) * zeta;
 }
 
 __device__ void record_row(const DevNet &N, const Lds L, const SolveArgs &A, int pidx, int row, double tyears)
@@ -2067,15 +2137,20 @@ __global__ void solve_init_kernel(int *counter, int *status, int n)
 }
 
 // The checks the host API makes of a parameter vector, repeated on the device for callers of
-// the device-pointer entry points: a point that fails them is not integrated and reports
-// UCG_E_PARAM (its tempindx would index the hot-core tables out of bounds).
-__device__ inline bool params_valid(const LDS double *q)
+// the device-pointer entry points: 0, or the status of a point that is not integrated:
+// -1 (PARAMETER_READ_ERROR: a value the kernel cannot use, e.g. a tempindx that would index
+// the hot-core tables out of bounds) or -2 (PHYSICS_INIT_ERROR: improvedH2CRPDissociation
+// without cosmicRayAttenuation, physics-core.f90:60-64).
+__device__ inline int params_status(const LDS double *q)
 {
-    const double model = q[P_MODEL], ti = q[P_TEMPINDX];
+    const double model = q[P_MODEL], ti = q[P_TEMPINDX], im = q[P_IONMODEL];
     bool ok = q[P_DENS] > 0.0 && q[P_TEMP] > 0.0 && q[P_RELTOL] > 0.0 && q[P_ABSMIN] >= 0.0 &&
-              q[P_MXSTEP] >= 1.0 && q[P_ROUT] > q[P_RIN] && q[P_MAXSTEPS] >= 0.0 && (model == 0.0 || model == 1.0);
+              q[P_MXSTEP] >= 1.0 && q[P_ROUT] > q[P_RIN] && q[P_MAXSTEPS] >= 0.0 && (model == 0.0 || model == 1.0) &&
+              (im == 0.0 || im == 1.0) && q[P_OMEGA] != 1.0 && q[P_UVCREFF] != 0.0;
     if (model == 1.0) ok = ok && ti >= 1.0 && ti < 7.0;
-    return ok;
+    if (!ok) return -1;
+    if (q[P_IMPROVEDH2CRP] != 0.0 && q[P_CRATTEN] == 0.0) return -2;
+    return 0;
 }
 
 template <typename LUT>
@@ -2122,7 +2197,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         __syncthreads();
         const LDS double *prm = L.prm;
         const bool interval_mode = A.phys_in != nullptr;
-        const bool valid = params_valid(prm);
+        const int pstatus = params_status(prm);
         if (interval_mode) {   // the caller's state at the interval start
             if (tid() == 0) {
                 const double *ph = A.phys_in + (size_t)pidx * 8;
@@ -2133,6 +2208,9 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                 L.sc[S_CS] = ph[4];
                 L.sc[S_PHI] = ph[5];
                 L.sc[S_ABSFAC] = ph[6];
+                // the rate the last physics update set from the column at this density
+                L.sc[S_H2CRP] = (ph[7] > 0.0 && prm[P_CRATTEN] != 0.0 && prm[P_IMPROVEDH2CRP] != 0.0)
+                                    ? h2_crp_rate_dev(ph[4] * ph[0], prm[P_ZETA], prm[P_IONMODEL] != 0.0) : 0.0;
             }
         } else if (tid() == 0) {
             double cs = (prm[P_ROUT] - prm[P_RIN]) * PC;
@@ -2144,8 +2222,9 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
             L.sc[S_AV] = prm[P_BASEAV] + L.sc[S_COLDENS] / 1.6e21;
             if (prm[P_FREEFALL] != 0.0) dens = (double)1.001f * prm[P_DENS];
             L.sc[S_DENS] = dens;
-            L.sc[S_PHI] = 1.0e5;
+            L.sc[S_PHI] = prm[P_PHI];
             L.sc[S_ABSFAC] = prm[P_ABSFAC];
+            L.sc[S_H2CRP] = 0.0;   // set by the physics update (physics-core.f90:87,143-155)
         }
         __syncthreads();
         if (A.y_init) {
@@ -2161,7 +2240,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         o.equilibrate = !o.sparse && A.equilibrate != 0;
         o.gen = A.gen;
         bdf::Stats st;
-        int row = 0, nint = 0, nretry = 0, flag = valid ? 0 : -1;   // -1: UCG_E_PARAM
+        int row = 0, nint = 0, nretry = 0, flag = pstatus;
         if (!interval_mode) record_row(N, L, A, pidx, row++, 0.0);
         double t = interval_mode ? A.phys_in[(size_t)pidx * 8 + 7] : 0.0;
         double tyears = t / SECONDS_PER_YEAR;
@@ -2214,7 +2293,22 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                     flag = -4;   // INT_UNRECOVERABLE_ERROR (constants.f90:25)
                 }
                 if (is == -7) flag = STATUS_CANCELLED;
-                // clamp (chemistry.f90:215) and density bookkeeping (:216)
+                // enforceChargeConservation (chemistry.f90:288-291: electrons = sum of the
+                // positive ions after the integrator call), then the clamp (:215) and the
+                // density bookkeeping (:216)
+                if (prm[P_ENFORCECHARGE] != 0.0 && is != -3 && is != -7) {
+#pragma unroll
+                    for (int e = 0; e < EPT; e++)
+                        if (o.own(e)) L.y[o.idx(e)] = o.z[0][e];
+                    __syncthreads();
+                    double ions = 0.0;
+                    for (int q = tid(); q < N.nion; q += NT) ions += L.y[N.ion_list[q]];
+                    ions = block_sum(ions, L.red);
+#pragma unroll
+                    for (int e = 0; e < EPT; e++)
+                        if (o.own(e) && o.idx(e) == N.nelec) o.z[0][e] = ions;
+                    __syncthreads();
+                }
 #pragma unroll
                 for (int e = 0; e < EPT; e++)
                     if (o.own(e)) {
@@ -2240,6 +2334,8 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                 L.sc[S_COLDENS] = cs / 1.0 * L.sc[S_DENS];
                 L.sc[S_AV] = prm[P_BASEAV] + L.sc[S_COLDENS] / 1.6e21;
                 L.sc[S_TDUST] = L.sc[S_TGAS];
+                if (prm[P_CRATTEN] != 0.0 && prm[P_IMPROVEDH2CRP] != 0.0)
+                    L.sc[S_H2CRP] = h2_crp_rate_dev(L.sc[S_COLDENS], prm[P_ZETA], prm[P_IONMODEL] != 0.0);
                 if (model == 1) {
                     const double tempa[6] = {1.927e-1, 4.8560e-2, 7.8470e-3, 9.6966e-4, 1.706e-4, 4.74e-7};
                     const float tempb[6] = {0.5339f, 0.6255f, 0.8395f, 1.085f, 1.289f, 1.98f};
@@ -2321,6 +2417,7 @@ __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
 {
     for (int i = tid(); i < P_NPARAM; i += NT) L.prm[i] = A.params[(size_t)p * P_NPARAM + i];
     for (int i = tid(); i < N.neq; i += NT) L.y[i] = A.y[(size_t)p * N.neq + i];
+    __syncthreads();
     if (tid() == 0) {
         const double *ph = A.phys + (size_t)p * 8;
         L.sc[S_DENS] = ph[0];
@@ -2330,6 +2427,9 @@ __device__ void piece_setup(const DevNet &N, Lds &L, const PieceArgs &A, int p)
         L.sc[S_CS] = ph[4];
         L.sc[S_PHI] = ph[5];
         L.sc[S_ABSFAC] = ph[6];
+        const LDS double *prm = L.prm;
+        L.sc[S_H2CRP] = (ph[7] > 0.0 && prm[P_CRATTEN] != 0.0 && prm[P_IMPROVEDH2CRP] != 0.0)
+                            ? h2_crp_rate_dev(ph[4] * ph[0], prm[P_ZETA], prm[P_IONMODEL] != 0.0) : 0.0;
     }
     if (A.rate_in)
         for (int k = tid(); k < N.nreac; k += NT) L.rate[k] = A.rate_in[(size_t)p * N.nreac + k];

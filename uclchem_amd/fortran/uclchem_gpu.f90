@@ -17,13 +17,20 @@ MODULE uclchem_gpu
     PUBLIC :: ucg_init, ucg_network_load_file_f, ucg_network_info, ucg_solve_batch_f, ucg_error_message, &
               ucg_default_params, ucg_cancel, ucg_queue_slot_bytes, ucg_queue_open, ucg_queue_submit, &
               ucg_queue_test, ucg_queue_result, ucg_queue_close
-    INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 24
+    INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 64
     INTEGER, PARAMETER, PUBLIC :: UCG_E_ARG = -102   ! an array does not match the batch or network
     INTEGER, PARAMETER, PUBLIC :: UCGP_DENS = 1, UCGP_TEMP = 2, UCGP_ZETA = 3, UCGP_RADFIELD = 4, &
         UCGP_BASEAV = 5, UCGP_ROUT = 6, UCGP_RIN = 7, UCGP_FINALTIME = 8, UCGP_FREEFALL = 9, &
         UCGP_FINALDENS = 10, UCGP_RELTOL = 11, UCGP_ABSFAC = 12, UCGP_ABSMIN = 13, UCGP_MODEL = 14, &
         UCGP_TEMPINDX = 15, UCGP_MAXTEMP = 16, UCGP_FREEZEFAC = 17, UCGP_ENDATFINALDENS = 18, &
-        UCGP_FREEFALLFAC = 19, UCGP_MXSTEP = 20, UCGP_MAXSTEPS = 21
+        UCGP_FREEFALLFAC = 19, UCGP_MXSTEP = 20, UCGP_MAXSTEPS = 21, UCGP_METALLICITY = 22, UCGP_ION = 23, &
+        UCGP_FH = 24, UCGP_FHE = 25, UCGP_FC = 26, UCGP_FO = 27, UCGP_FN = 28, UCGP_FS = 29, UCGP_FMG = 30, &
+        UCGP_FSI = 31, UCGP_FCL = 32, UCGP_FP = 33, UCGP_FFE = 34, UCGP_FF = 35, UCGP_FD = 36, UCGP_FLI = 37, &
+        UCGP_FNA = 38, UCGP_FPAH = 39, UCGP_F15N = 40, UCGP_F13C = 41, UCGP_F18O = 42, UCGP_EBMAXH2 = 43, &
+        UCGP_EBMAXCR = 44, UCGP_EBMAXUVCR = 45, UCGP_EPSILON = 46, UCGP_UV_YIELD = 47, UCGP_PHI = 48, &
+        UCGP_UVCREFF = 49, UCGP_OMEGA = 50, UCGP_DESORB = 51, UCGP_H2DESORB = 52, UCGP_CRDESORB = 53, &
+        UCGP_UVDESORB = 54, UCGP_THERMDESORB = 55, UCGP_CRATTENUATION = 56, UCGP_IONMODEL = 57, &
+        UCGP_IMPROVEDH2CRP = 58, UCGP_ENFORCECHARGE = 59
 
     INTERFACE
         INTEGER(C_INT) FUNCTION ucg_init(device) BIND(C, name="ucg_init")
@@ -210,6 +217,33 @@ CONTAINS
         p(UCGP_FREEFALLFAC) = 1.0D0
         p(UCGP_MXSTEP) = 10000.0D0
         p(UCGP_MAXSTEPS) = 0.0D0
+        p(UCGP_METALLICITY) = 1.0D0
+        p(UCGP_ION) = 2.0D0
+        p(UCGP_FH) = 0.5D0
+        p(UCGP_FHE) = REAL(0.1, C_DOUBLE)
+        p(UCGP_FC) = 1.77D-04
+        p(UCGP_FO) = 3.34D-04
+        p(UCGP_FN) = 6.18D-05
+        p(UCGP_FS) = 3.51D-6
+        p(UCGP_FMG) = 2.256D-06
+        p(UCGP_FSI) = 1.78D-06
+        p(UCGP_FCL) = 3.39D-08
+        p(UCGP_FP) = 7.78D-08
+        p(UCGP_FFE) = 2.01D-7
+        p(UCGP_FF) = 3.6D-08
+        p(UCGP_EBMAXH2) = 1.21D3
+        p(UCGP_EBMAXCR) = 1.21D3
+        p(UCGP_EBMAXUVCR) = 1.0D4
+        p(UCGP_EPSILON) = REAL(0.01, C_DOUBLE)
+        p(UCGP_UV_YIELD) = REAL(0.03, C_DOUBLE)
+        p(UCGP_PHI) = 1.0D5
+        p(UCGP_UVCREFF) = 1.0D-3
+        p(UCGP_OMEGA) = 0.5D0
+        p(UCGP_DESORB) = 1.0D0
+        p(UCGP_H2DESORB) = 1.0D0
+        p(UCGP_CRDESORB) = 1.0D0
+        p(UCGP_UVDESORB) = 1.0D0
+        p(UCGP_THERMDESORB) = 1.0D0
     END SUBROUTINE ucg_default_params
 
 END MODULE uclchem_gpu

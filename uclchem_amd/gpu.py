@@ -34,8 +34,10 @@ class _Desc(ctypes.Structure):
                   "bulk_partner_of_surface", "surface_partner_of_bulk", "freeze_partners",
                   "is_surface", "is_bulk", "jnz_row", "jnz_col", "jnz_ptr", "jc_k", "jc_m", "jc_c",
                   "aff_ptr", "aff_sp", "aff_cf", "sw_ptr", "sw_k", "sw_c"]] +
-                [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 9)] +
-                [("nw_sizes", ctypes.c_int * 10)] + [(n, ctypes.c_void_p) for n in mh.DESC_NW])
+                [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 10)] +
+                [("nw_sizes", ctypes.c_int * 10)] + [(n, ctypes.c_void_p) for n in mh.DESC_NW] +
+                [("nion", ctypes.c_int), ("ion_list", ctypes.c_void_p), ("ngar", ctypes.c_int),
+                 ("gar_params", ctypes.c_void_p)])
 
 
 class GpuError(RuntimeError):
@@ -116,7 +118,7 @@ class Network:
             setattr(d, k, _p(v).value)
         for i in range(25):
             d.special_species[i] = int(ssp[i])
-        for i in range(9):
+        for i in range(10):
             d.special_reactions[i] = int(srx[i])
         h = ctypes.c_int(-1)
         _check(lib().ucg_network_load(ctypes.byref(d), ctypes.byref(h)), "ucg_network_load")

@@ -243,11 +243,22 @@ class NetworkTables:
 
 
 def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
-                 name: str = "network") -> NetworkTables:
+                 name: str = "network", gar_database: Dict[str, Sequence[float]] | None = None) -> NetworkTables:
+    """gar_database: the grain-assisted recombination parameters (Weingartner & Draine 2001,
+    seven per ion) keyed by ion name, needed when the network has GAR reactions; each GAR
+    reaction takes its first reactant's row, in reaction order (makerates.py:107-123)."""
     names = [s.name for s in species]
     for r in reactions:
         if r.rtype in UNSUPPORTED_TYPES:
             raise ValueError(f"reaction type {r.rtype} is not supported by the device rate evaluator")
+    gar_rows = []
+    for r in reactions:
+        if r.rtype == "GAR":
+            ion = r.reactants[0]
+            if not gar_database or ion not in gar_database:
+                raise ValueError(f"GAR reaction of {ion} has no grain-assisted recombination parameters")
+            gar_rows.append([float(v) for v in gar_database[ion]])
+    gar_params = fortran_real(np.array(gar_rows, dtype=np.float64).reshape(-1, 7))
     nspec = len(names)
     neq = nspec + 1
     nreac = len(reactions)
@@ -537,7 +548,7 @@ def build_tables(species: Sequence[Species], reactions: Sequence[Reaction],
         refractory=refractory,
         re=re, pr=pr, rtype=rtype, alpha=alpha, beta=beta, gama=gama, tmin=tmin, tmax=tmax,
         reduced_mass=redmass, extrapolate=extrap, type_ranges=ranges,
-        freeze_partners=freeze_partners, gar_params=np.zeros((0, 7)),
+        freeze_partners=freeze_partners, gar_params=gar_params,
         n_d=n_d, ops=ops, loss_ptr=loss_ptr, loss_idx=loss_idx, prod_ptr=prod_ptr,
         prod_idx=prod_idx, swap_list=np.array(swap_list, dtype=np.int32),
     )
@@ -661,10 +672,21 @@ def _static_rate_constants(names, species, reactions, re, pr, rtype, alpha, gama
         elif t in ("LHDES", "ERDES"):
             rstat[k, 0] = desorption_fraction(k)
             rstat[k, 1] = 1.0 if r1 in bulk_set else 0.0
+        elif t in ("EXRELAX", "EXSOLID"):
+            # diffusion frequencies of the ice reactants (rates.f90:170-197 take the last
+            # iceList match; a reactant outside the ice list would leave them undefined)
+            p1 = ice_pos(r1)
+            p2 = ice_pos(int(re[k, 1])) if t == "EXSOLID" else 0
+            if p1 < 0 or p2 < 0:
+                raise ValueError(f"{t} reaction {k} has a reactant outside the ice list")
+            if t == "EXRELAX":
+                rstat[k, 0] = vdiff[p1]
+            else:
+                rstat[k, 0] = (vdiff[p2] + vdiff[p1]) / (SURFACE_SITE_DENSITY * 1.8e-8)
     return rstat, ice_kind
 
 
-UNSUPPORTED_TYPES = ("IONOPOL1", "IONOPOL2", "CRS", "EXSOLID", "EXRELAX", "GAR", "DESORB")
+UNSUPPORTED_TYPES: tuple = ()   # every reaction type of rates.f90 has a device rate law
 
 
 # ---------------------------------------------------------------- persistence
@@ -693,7 +715,7 @@ SPECIAL_SPECIES_ORDER = ["nh", "nh2", "nhe", "nc", "ncx", "nn", "no", "nf", "np"
                          "nli", "npah", "nsx", "nsix", "nclx", "nd", "n18o", "n15n", "n13c", "nelec", "nco",
                          "nsurface", "nbulk"]
 SPECIAL_REACTION_ORDER = ["nR_H2Form_CT", "nR_H2Form_ER", "nR_H2Form_ERDes", "nR_HFreeze", "nR_EFreeze",
-                          "nR_H2Freeze", "nR_H2_hv", "nR_CO_hv", "nR_C_hv"]
+                          "nR_H2Freeze", "nR_H2_hv", "nR_CO_hv", "nR_C_hv", "nR_H2_crp"]
 DESC_SIZES = ["nspec", "neq", "nreac", "nice", "nsurfl", "nbulkl", "nfp", "nswap",
               "nloss", "nprod", "nnz", "ncontrib", "naff", "nsw"]
 DESC_F64 = ["alpha", "beta", "gama", "tmin", "tmax", "mass", "rstat"]
@@ -746,9 +768,20 @@ def device_descriptor(t: NetworkTables):
                  nswap=len(a["swap_list"]), nloss=len(a["loss_idx"]), nprod=len(a["prod_idx"]),
                  nnz=len(a["jnz_row"]), ncontrib=len(a["jc_k"]), naff=len(a["aff_sp"]), nsw=len(a["sw_k"]))
     special_species = np.array([t.scalars[k] for k in SPECIAL_SPECIES_ORDER], dtype=np.int32)
-    special_reactions = np.array([t.scalars[k] for k in SPECIAL_REACTION_ORDER], dtype=np.int32)
+    special_reactions = np.array([t.scalars.get(k, -1) for k in SPECIAL_REACTION_ORDER], dtype=np.int32)
     sizes["nw_sizes"] = np.ascontiguousarray(a["nw_sizes"], dtype=np.int32)
+    arrays["ion_list"] = ion_list(t)
+    sizes["nion"] = len(arrays["ion_list"])
+    gar = np.ascontiguousarray(np.asarray(a.get("gar_params", np.zeros((0, 7))), dtype=np.float64).reshape(-1, 7))
+    arrays["gar_params"] = gar.reshape(-1)
+    sizes["ngar"] = len(gar)
     return sizes, arrays, special_species, special_reactions
+
+
+def ion_list(t: "NetworkTables") -> np.ndarray:
+    """0-based indices of the positive ions: species whose name holds '+'
+    (initializeChemistry's ionlist, chemistry.f90:114-120)."""
+    return np.array([i for i, n in enumerate(t.species_names) if "+" in n], dtype=np.int32)
 
 
 BIN_MAGIC = b"UCGNET01"
@@ -764,8 +797,9 @@ def save_network_bin(t: NetworkTables, path: str) -> None:
     and zero padding to a multiple of 8 bytes."""
     sizes, arrays, ssp, srx = device_descriptor(t)
     recs = [("sizes", np.array([sizes[k] for k in DESC_SIZES], dtype=np.int32)),
-            ("special_species", ssp), ("special_reactions", srx), ("nw_sizes", sizes["nw_sizes"])]
-    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8 + DESC_NW]
+            ("special_species", ssp), ("special_reactions", srx), ("nw_sizes", sizes["nw_sizes"]),
+            ("nion", np.array([sizes["nion"]], dtype=np.int32)), ("ngar", np.array([sizes["ngar"]], dtype=np.int32))]
+    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8 + DESC_NW + ["ion_list", "gar_params"]]
     import struct
     with open(path, "wb") as f:
         f.write(BIN_MAGIC)

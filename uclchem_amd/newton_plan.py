@@ -313,7 +313,8 @@ def lds_bytes(tables, staged: bool = True) -> int:
                 (stage(nsp, 8) + 7) // 8)
     else:
         c = Npad
-    return 8 * (max(b, c) + 4 * npad + 16 + 24 + 24)
+    from .params import NPARAM
+    return 8 * (max(b, c) + 4 * npad + 16 + 24 + NPARAM)   # + reduction slots, scalars, parameters
 
 
 def schur_sources(tables) -> int:

@@ -232,7 +232,8 @@ def write_reference_sources(t: mh.NetworkTables, outdir: str) -> List[str]:
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="write reference-format network sources for a network")
-    ap.add_argument("network", help="network name (uclchem_amd/networks/<name>.npz)")
+    ap.add_argument("network", help="network name (uclchem_amd/networks/<name>.npz) or path of a .npz")
     ap.add_argument("outdir")
     args = ap.parse_args()
-    print("\n".join(write_reference_sources(mh.load_network(mh.network_path(args.network)), args.outdir)))
+    path = args.network if args.network.endswith(".npz") else mh.network_path(args.network)
+    print("\n".join(write_reference_sources(mh.load_network(path), args.outdir)))

@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .params import base_av_for, make_params
+from .params import NPARAM, base_av_for, make_params
 
 SEED = 20261202
 
@@ -30,7 +30,7 @@ def static_cloud_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.n
     logzeta = rng.uniform(0.0, 2.0, n)
     logg0 = rng.uniform(0.0, 3.0, n)
     av = rng.uniform(1.0, 20.0, n)
-    out = np.zeros((n, 24))
+    out = np.zeros((n, NPARAM))
     for i in range(n):
         nh = 10.0 ** lognh[i]
         out[i] = make_params(initialdens=nh, initialtemp=temp[i], zeta=10.0 ** logzeta[i],
@@ -41,7 +41,7 @@ def static_cloud_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.n
 def hot_core_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed + 5))
     lognh = rng.uniform(5.0, 8.0, n)
-    out = np.zeros((n, 24))
+    out = np.zeros((n, NPARAM))
     for i in range(n):
         out[i] = make_params(initialdens=10.0 ** lognh[i], initialtemp=10.0, model=1, tempindx=4, maxtemp=300.0,
                              finaltime=final_time)
@@ -51,7 +51,7 @@ def hot_core_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.ndarr
 def collapse_grid(n: int, seed: int = SEED, final_time: float = 1e6) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed + 3))
     lognh = rng.uniform(2.0, 6.0, n)
-    out = np.zeros((n, 24))
+    out = np.zeros((n, NPARAM))
     for i in range(n):
         out[i] = make_params(initialdens=10.0 ** lognh[i], initialtemp=10.0, freefall=1, finaldens=1e6,
                              finaltime=final_time)

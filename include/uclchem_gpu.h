@@ -146,6 +146,11 @@ typedef struct ucg_network_desc {
        (network.f90 garParams, rates.f90:316-332) */
     int ngar;
     const double *gar_params;
+    /* charge row of the sparse plan, used by points with enforceChargeConservation
+       (chemistry.f90:333-335): the nw_nerow plan slots of the electron row, cleared, and
+       the nion slots of its positive-ion entries (ion_list order) */
+    int nw_nerow;
+    const int *nw_erow, *nw_eion;
 } ucg_network_desc;
 
 /* Select the GPU and upload the constant photo-rate tables.  Called implicitly. */
@@ -173,6 +178,16 @@ int ucg_network_free(int handle);
  * queue's kernel runs (ucg_queue_open) use ucg_solve_batch_device on a stream instead. */
 int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out,
                     int max_rec, double *rec, int *n_rec, int *status, int *stats, int lu_bits);
+
+/* ucg_solve_batch that also returns the rate coefficients of every output row: the
+ * reference's returnRates with returnArray (ratesarray(dtime, dstep, :) = rate(:nreac),
+ * src/fortran_src/io.f90:90-93; wrap.f90:15-80).  rates_rec [n][max_rec][nreac]: row r
+ * holds the rates of the interval that ended at row r (the last attempt's coefficients,
+ * with the H2 and CO photodissociation rates of its last RHS evaluation); row 0, written
+ * before any interval, is zero.  rec and max_rec > 0 are required. */
+int ucg_solve_batch_rates(int handle, int n, const double *params, const double *y_init, double *y_out,
+                          int max_rec, double *rec, double *rates_rec, int *n_rec, int *status, int *stats,
+                          int lu_bits);
 
 /* Same on device-resident buffers (all pointers are device pointers, stream is a
  * hipStream_t or NULL).  t_start/t_done [n] (nullable) receive the device wall clock

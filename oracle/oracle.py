@@ -78,7 +78,8 @@ class _Net(ctypes.Structure):
                   "nco", "nsurface", "nbulk", "ngn", "ngo", "ngoh",
                   "r_h2form_ct", "r_h2form_erdes", "r_h2form_er", "r_hfreeze", "r_efreeze",
                   "r_h2freeze", "r_h2_hv", "r_co_hv", "r_c_hv", "r_h2_crp"]] +
-                [("gar_params", ctypes.c_void_p), ("ngar", ctypes.c_int32)])
+                [("gar_params", ctypes.c_void_p), ("ngar", ctypes.c_int32),
+                 ("ion_list", ctypes.c_void_p), ("nion", ctypes.c_int32)])
 
 
 class _Point(ctypes.Structure):
@@ -136,6 +137,11 @@ class Oracle:
         self._keep["gar_params"] = gar
         net.gar_params = gar.ctypes.data if len(gar) else None
         net.ngar = len(gar) // 7
+        # positive ions: every species whose name holds '+' (chemistry.f90:114-120)
+        ions = np.array([i for i, nm in enumerate(tables.species_names) if "+" in nm], dtype=np.int32)
+        self._keep["ion_list"] = ions
+        net.ion_list = ions.ctypes.data if len(ions) else None
+        net.nion = len(ions)
         self.net = net
         L = self.lib
         L.ucl_next_target.restype = ctypes.c_double
@@ -222,6 +228,23 @@ class RefLib:
         rc = self.lib.ucref_model(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
                                   ctypes.c_int(maxrec), _ptr(rec), ctypes.byref(nrec), _ptr(stats))
         return rc, rec[:nrec.value].copy(), stats
+
+    def model_rates(self, params, maxrec: int = 1000):
+        """ucref_model with the rate array of every output row: (rc, rows, stats, rates[nrec, nreac])."""
+        rec = np.zeros((maxrec, 4 + self.nspec))
+        rates = np.zeros((maxrec, self.nreac))
+        nrec = ctypes.c_int()
+        stats = np.zeros(8, dtype=np.int32)
+        rc = self.lib.ucref_model_rates(_ptr(np.ascontiguousarray(params, dtype=np.float64)), ctypes.c_int(maxrec),
+                                        _ptr(rec), ctypes.byref(nrec), _ptr(stats), _ptr(rates))
+        return rc, rec[:nrec.value].copy(), stats, rates[:nrec.value].copy()
+
+    def set_coefficient(self, which: str, k: int, v: float) -> None:
+        """alpha/beta/gamma of reaction k (1-based) in the reference's module arrays."""
+        code = {"alpha": 0, "beta": 1, "gamma": 2}[which]
+        self.lib.ucref_set_coefficient.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        if self.lib.ucref_set_coefficient(code, int(k), float(v)) != 0:
+            raise ValueError(f"ucref_set_coefficient({which}, {k}) failed")
 
     def model_from(self, params, y0, maxrec: int = 1000):
         """The model from given start abundances y0 [neq] (the reference's abundanceStart)."""

@@ -38,6 +38,9 @@ MODULE ucref_driver
     ! given start abundances of the next model (wrap.f90:636-640, abundanceStart)
     LOGICAL :: have_start = .false.
     REAL(dp), ALLOCATABLE :: start_abund(:)
+    ! rate coefficients per output row (returnRates, io.f90:90-93), while ucref_model_rates runs
+    LOGICAL :: want_rates = .false.
+    REAL(dp), ALLOCATABLE :: rates_buf(:,:)
 
     ! parameter vector layout; mirrored by oracle/oracle.py and uclchem_amd/params.py
     INTEGER, PARAMETER :: P_DENS=1, P_TEMP=2, P_ZETA=3, P_RADFIELD=4, P_BASEAV=5, P_ROUT=6, P_RIN=7, &
@@ -358,6 +361,45 @@ CONTAINS
         ucref_model = run_model(p, HUGE(1.0d0), maxrec, rec, nrec, stats)
     END FUNCTION ucref_model
 
+    ! ucref_model that also returns the rate array as each row is written: the reference's
+    ! ratesarray(dtime, dstep, :) = rate(:nreac) (io.f90:90-93).  Row 1 holds whatever the
+    ! module's rate array held before the first interval.
+    INTEGER(c_int) FUNCTION ucref_model_rates(p, maxrec, rec, nrec, stats, rates) BIND(C, name="ucref_model_rates")
+        REAL(c_double), INTENT(IN) :: p(P_NPARAM)
+        INTEGER(c_int), VALUE :: maxrec
+        REAL(c_double), INTENT(OUT) :: rec(4+nspec, maxrec), rates(nreac, maxrec)
+        INTEGER(c_int), INTENT(OUT) :: nrec
+        INTEGER(c_int), INTENT(OUT) :: stats(8)
+        IF (ALLOCATED(rates_buf)) DEALLOCATE(rates_buf)
+        ALLOCATE(rates_buf(nreac, maxrec))
+        rates_buf = 0.0d0
+        want_rates = .true.
+        ucref_model_rates = run_model(p, HUGE(1.0d0), maxrec, rec, nrec, stats)
+        want_rates = .false.
+        rates = rates_buf
+    END FUNCTION ucref_model_rates
+
+    ! A reaction coefficient replaced as the alpha/beta/gamma dictionary keys do
+    ! (wrap.f90:744-761, coefficientParser :985-1019): which 0 alpha, 1 beta, 2 gama; k 1-based.
+    ! Like the reference's, the change stays in the module arrays until set back.
+    INTEGER(c_int) FUNCTION ucref_set_coefficient(which, k, v) BIND(C, name="ucref_set_coefficient")
+        INTEGER(c_int), VALUE :: which, k
+        REAL(c_double), VALUE :: v
+        ucref_set_coefficient = -1
+        IF (k < 1 .or. k > nreac) RETURN
+        SELECT CASE (which)
+            CASE (0)
+                alpha(k) = v
+            CASE (1)
+                beta(k) = v
+            CASE (2)
+                gama(k) = v
+            CASE DEFAULT
+                RETURN
+        END SELECT
+        ucref_set_coefficient = 0
+    END FUNCTION ucref_set_coefficient
+
     ! The model loop from given start abundances y0(1:nspec+1) instead of the elemental start
     ! (the abundanceStart path of wrap.f90:636-640: set after initializeChemistry).
     INTEGER(c_int) FUNCTION ucref_model_from(p, y0, maxrec, rec, nrec, stats) BIND(C, name="ucref_model_from")
@@ -488,6 +530,7 @@ CONTAINS
                 CALL bounded_update_chemistry(p(P_MAXSTEPS), nsteps, flag, attempts)
             ELSE
                 CALL updateChemistry(flag)
+                attempts = loopCounter
             END IF
             IF (flag .lt. 0) EXIT
             CALL GET_STATS(rstats, istats)
@@ -495,6 +538,9 @@ CONTAINS
             stats(2) = stats(2) + istats(12)
             stats(3) = stats(3) + istats(13)
             stats(4) = stats(4) + 1
+            ! integrator calls per interval: each further one recomputes the rates mid-interval
+            stats(6) = MAX(stats(6), attempts)
+            IF (attempts > 1) stats(7) = stats(7) + 1
             timeInYears= targetTime/SECONDS_PER_YEAR
             CALL coreUpdatePhysics
             IF (modelKind .eq. 1) CALL hotcoreUpdatePhysics
@@ -519,6 +565,7 @@ CONTAINS
         rec(3, nrec) = gasTemp(1)
         rec(4, nrec) = av(1)
         rec(5:, nrec) = abund(1:nspec, 1)
+        IF (want_rates) rates_buf(:, nrec) = rate(1:nreac)
     END SUBROUTINE record_row
 
 END MODULE ucref_driver

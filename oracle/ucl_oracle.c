@@ -677,6 +677,13 @@ int ucl_rhs(const ucl_net *n, const ucl_point *pt, double *rate, const double *y
     for (int i = 0; i < n->nsurfl; i++) sum_s += ydot[n->surface_list[i]];
     ydot[n->nbulk] = sum_b;
     ydot[n->nsurface] = sum_s;
+    /* enforceChargeConservation: the electron row is the sum of the positive-ion rows
+       (chemistry.f90:333-335) */
+    if (pt->p[P_ENFORCECHARGE] != 0.0) {
+        double e = 0.0;
+        for (int q = 0; q < n->nion; q++) e += ydot[n->ion_list[q]];
+        ydot[n->nelec] = e;
+    }
     /* density (physics-core.f90:90-103) */
     double dd = 0.0;
     double d = y[neq - 1];

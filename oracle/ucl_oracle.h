@@ -55,6 +55,8 @@ typedef struct {
     int32_t r_h2_crp;           /* H2 + CRP -> H + H, or -1 */
     const double *gar_params;   /* [ngar*7], one row per GAR reaction */
     int32_t ngar;
+    const int32_t *ion_list;    /* [nion] positive ions, names holding '+' (chemistry.f90:114-120) */
+    int32_t nion;
 } ucl_net;
 
 /* per-point physical state carried between intervals */

@@ -21,6 +21,13 @@ where its ten attempts end short of the interval's end, the stretch its first at
 (the target is then the time that attempt reached); and, for hot-core points, the last
 walked state with its temperature raised to 20, 50, 100 and 300 K ("jump" rows: the
 desorption conditions of the warm-up, one interval each, from an icy state the walk reached).
+
+On the first-attempt stretches the reference's DVODE spends its whole step budget, and its
+answer for the smallest abundances depends on its tolerance.  For those rows y_tight holds
+the reference run over the same stretch at 1000 times tighter relative and absolute
+tolerances (reltol and abstol_factor / 1000, no step bound), so that a test can tell the
+reference's own integration error from a difference of method (other rows: NaN).
+    python tests/golden/make_interval_fixtures.py --tight-only   # add y_tight to the fixture
 """
 from __future__ import annotations
 
@@ -112,12 +119,47 @@ def select(rows, n_hard=3, n_first=2):
     return out
 
 
+def tight(job):
+    """The reference over one first-attempt stretch at 1000x tighter tolerances."""
+    p, y, phys, target = job
+    from oracle import RefLib
+    p = p.copy()
+    ph = phys.copy()
+    p[INDEX["reltol"]] *= 1e-3
+    p[INDEX["mxstep"]] = 1e7
+    ph[6] *= 1e-3
+    rc, y1, ph1, st = RefLib().interval(p, y, ph, target)
+    ok = rc == 0 and ph1[7] >= target * (1 - 1e-12)
+    print(f"tight: {st[0]} steps, rc {rc}, reached {ok}", flush=True)
+    return y1 if ok else np.full_like(y1, np.nan)
+
+
+def add_tight(procs):
+    from uclchem_amd.params import widen
+    path = os.path.join(HERE, "ref_intervals.npz")
+    f = dict(np.load(path))
+    rows = [q for q in range(len(f["target"])) if f["how"][q] == "first-attempt"]
+    params = widen(f["params"])
+    jobs = [(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q])) for q in rows]
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(tight, jobs, chunksize=1)
+    yt = np.full_like(f["y_ref"], np.nan)
+    for q, y1 in zip(rows, res):
+        yt[q] = y1
+    f["y_tight"] = yt
+    np.savez_compressed(path, **f)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--tight-only", action="store_true", help="add y_tight to the existing fixture")
     ap.add_argument("--procs", type=int, default=6)
     ap.add_argument("--budget", type=float, default=2400.0, help="wall seconds per hot-core trajectory walk")
     ap.add_argument("--tail-budget", type=float, default=900.0, help="wall seconds per tail-point walk")
     a = ap.parse_args()
+    if a.tight_only:
+        add_tight(a.procs)
+        return
     jobs = [("hot", i, a.budget) for i in hot_points()] + [("tail", i, a.tail_budget) for i in TAIL_POINTS]
     with mp.get_context("fork").Pool(a.procs) as pool:
         rows = [r for part in pool.map(walk, jobs, chunksize=1) for r in part]

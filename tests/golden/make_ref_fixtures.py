@@ -16,6 +16,8 @@ Outputs (small .npz files next to this script):
   ref_serving.npz        the reference, compiled on each request's network, for the seeded
                          24-request stream over the default and synthetic networks
                          (t_end log-uniform 1e3..1e7 yr)
+  ref_coeff.npz          the static cloud to 1e5 yr with reaction coefficients changed (the
+                         alpha/beta/gamma keys) and the rate array at five output rows
 """
 from __future__ import annotations
 
@@ -139,6 +141,80 @@ def grid4096_sample(n, procs, cap):
         np.stack([r[3] for r in res]), np.array([r[4] for r in res]), np.array([r[2] for r in res])
 
 
+def grid_tight(procs, cap):
+    """Add to ref_cloud_grid.npz the reference's answer at tight tolerances (reltol 1e-11,
+    abstol_factor 1e-17, abstol_min 1e-30) for each of its points: final_tight (NaN where the
+    run exceeds `cap` seconds).  |final - final_tight| is the reference's own integration
+    noise on that point: on the grid's stiff tail it includes the different places where
+    its integrator runs out of steps and the rates are recomputed mid-interval
+    (chemistry.f90:164-221)."""
+    from uclchem_amd.params import INDEX, widen
+    path = os.path.join(HERE, "ref_cloud_grid.npz")
+    f = dict(np.load(path))
+    params = widen(f["params"])
+    tight = params.copy()
+    tight[:, INDEX["reltol"]] = 1e-11
+    tight[:, INDEX["abstol_factor"]] = 1e-17
+    tight[:, INDEX["abstol_min"]] = 1e-30
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_model_capped, [(p, cap) for p in tight], chunksize=1)
+    ft = np.full_like(f["final"], np.nan)
+    wall = np.full(len(params), np.nan)
+    for q, r in enumerate(res):
+        if r is not None and r[0] == 0 and r[2] == f["nrows"][q]:
+            ft[q] = r[1]
+            wall[q] = r[4]
+        print(f"grid point {f['index'][q]}: tight {'capped' if r is None else (r[0], r[2], round(r[4], 1))}",
+              flush=True)
+    f["final_tight"] = ft
+    f["tight_wall"] = wall
+    np.savez_compressed(path, **f)
+
+
+def coeff_changes(tables):
+    """The coefficient changes of ref_coeff.npz: H3+ + CO (alpha, beta), the first CRP
+    reaction (alpha), the first LH and LHDES reactions (gamma); 1-based indices."""
+    from uclchem_amd import makerates_hip as mh
+    names = list(tables.species_names)
+    first = lambda kind: next(k for k in range(tables.nreac) if mh.ALL_TYPES[int(tables.rtype[k])] == kind)
+    two = next(k for k in range(tables.nreac) if mh.ALL_TYPES[int(tables.rtype[k])] == "TWOBODY"
+               and sorted(names[i] for i in tables.re[k][:2] if 0 <= i < tables.nspec) == ["CO", "H3+"])
+    return dict(alpha={two + 1: 3.3e-9, first("CRP") + 1: 0.7}, beta={two + 1: -0.25},
+                gamma={first("LH") + 1: 600.0, first("LHDES") + 1: 50.0})
+
+
+RATE_ROWS = (1, 5, 10, 20, -1)
+
+
+def coeff_fixture():
+    """ref_coeff.npz: the static cloud (n_H 1e4, 10 K) to 1e5 yr with the coefficient changes
+    of coeff_changes, through the reference with its module arrays changed the same way
+    (ucref_set_coefficient), recording the rate array at output rows RATE_ROWS
+    (returnRates, io.f90:90-93)."""
+    from oracle import RefLib
+    from uclchem_amd import makerates_hip as mh
+    from uclchem_amd.params import make_params
+    t = mh.load_network(mh.network_path("default"))
+    ch = coeff_changes(t)
+    ref = RefLib()
+    old = {w: {k: float(t.arrays["gama" if w == "gamma" else w][k - 1]) for k in c} for w, c in ch.items()}
+    for w, c in ch.items():
+        for k, v in c.items():
+            ref.set_coefficient(w, k, v)
+    p = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e5)
+    rc, rows, stats, rates = ref.model_rates(p, maxrec=200)
+    for w, c in old.items():
+        for k, v in c.items():
+            ref.set_coefficient(w, k, v)
+    which = np.array([w for w, c in ch.items() for _ in c])
+    idx = np.array([k for c in ch.values() for k in c])
+    val = np.array([v for c in ch.values() for v in c.values()])
+    sel = np.array([r % len(rows) for r in RATE_ROWS])
+    np.savez_compressed(os.path.join(HERE, "ref_coeff.npz"), params=p, which=which, index=idx, value=val, rc=rc,
+                        rows=rows, stats=stats, rate_rows=sel, rates=rates[sel])
+    print(f"ref_coeff: rc {rc}, {len(rows)} rows, rates at rows {sel.tolist()}", flush=True)
+
+
 def nondefault_params(n=8, seed=9):
     """n static-cloud points to 1e5 yr whose initial-abundance, desorption-constant and switch
     parameters are all away from their defaults (seeded), in the product's parameter layout."""
@@ -213,8 +289,16 @@ def main():
     ap.add_argument("--grid-only", action="store_true", help="only the 4096-grid sample (ref_cloud_grid.npz)")
     ap.add_argument("--params-only", action="store_true", help="only the non-default-parameter points")
     ap.add_argument("--alltypes-only", action="store_true", help="only the all-reaction-types network points")
+    ap.add_argument("--grid-tight", action="store_true", help="add tight-tolerance answers to ref_cloud_grid.npz")
+    ap.add_argument("--coeff-only", action="store_true", help="only the coefficient-change / rates-output model")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     a = ap.parse_args()
+    if a.grid_tight:
+        grid_tight(a.procs, a.cap)
+        return
+    if a.coeff_only:
+        coeff_fixture()
+        return
     if a.synthetic_only:
         synthetic_fixtures(a.procs)
         return

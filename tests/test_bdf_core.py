@@ -296,3 +296,33 @@ def test_hot_core_intervals_like_reference_dvode(bdf_lib):
         y = np.maximum(y_ref, 1e-30)
         y[-1] = y_ref[-1]
         t_yr = t1 / 3.16e7
+
+
+def test_abrupt_warm_up_interval_like_reference_dvode(bdf_lib):
+    """An icy hot-core state (n_H ~1e6) suddenly at 300 K (tests/golden/ref_intervals.npz, a
+    "jump 300 K" row): the first steps need more than seven consecutive error-test failures
+    before the step size fits the desorption time scale.  DVODE_F90 allows fifteen
+    (dvode.f90:1865-1876) and completes the interval in one attempt; so must the core, with
+    the reference's F, reaching its answer within 1e-5."""
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+    from oracle import REF_SO, RefLib
+    if not os.path.exists(REF_SO):
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    from uclchem_amd.params import INDEX, widen
+    f = np.load(os.path.join(HERE, "golden", "ref_intervals.npz"))
+    rows = [q for q in range(len(f["target"])) if f["how"][q] == "jump 300 K" and f["steps"][q] > 1000]
+    assert rows
+    q = rows[len(rows) // 2]
+    ref = RefLib()
+    rhs = ctypes.cast(ref.lib.ucref_rhs, ctypes.c_void_p)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    p = widen(f["params"][q:q + 1])[0]
+    y, ph, t1 = f["y_in"][q].copy(), f["phys_in"][q].copy(), float(f["target"][q])
+    ref.interval(p, y, ph, t1, maxsteps=1.0)          # leaves the interval's rates in the reference
+    atol = np.maximum(ph[6] * y, p[INDEX["abstol_min"]])
+    st = np.zeros(7, dtype=np.int32)
+    r = bdf_lib.bdf_host_solve_fn(len(y), rhs, ph[7], t1, p[INDEX["reltol"]], ptr(atol), ptr(y), 10000, ptr(st))
+    assert r == 2, (q, r, st.tolist())
+    m = f["y_ref"][q, :-1] > 1e-14
+    assert np.all(np.abs(y[:-1][m] - f["y_ref"][q, :-1][m]) <= 1e-5 * f["y_ref"][q, :-1][m])

@@ -7,6 +7,11 @@ oracle/_ref) holds states of the reference's own trajectories: hot-core warm-up 
 intervals of the 4096-point static-cloud grid's tail points (594, 380, 1203, 2549), where the
 reference needs up to ~1e5 steps.  Gates: the interval's end is reached, every abundance above
 1e-14 within 1% of the reference, element totals conserved to 1e-10 over the interval.
+
+On the "first-attempt" rows (stretches where the reference's DVODE spends its whole step budget)
+the reference's smallest abundances move by tens of percent when its own tolerances are
+tightened 1000-fold (y_tight, the reference rerun that way); there an abundance passes within
+1% of the reference plus twice the reference's own change, |y_ref - y_tight|.
 """
 import os
 
@@ -43,12 +48,16 @@ def test_intervals_match_reference(gpu_net, tables, lu_bits):
         assert out["phys"][q, 7] >= f["target"][q] * (1 - 1e-12), (q, out["phys"][q, 7], f["target"][q])
         ref = f["y_ref"][q, :tables.nspec]
         m = ref > 1e-14
-        rel = np.abs(out["y"][q, :tables.nspec][m] - ref[m]) / ref[m]
+        allow = 0.01 * ref[m]
+        if f["how"][q] == "first-attempt":
+            assert "y_tight" in f.files and np.all(np.isfinite(f["y_tight"][q])), q
+            allow = allow + 2.0 * np.abs(ref[m] - f["y_tight"][q, :tables.nspec][m])
+        excess = np.abs(out["y"][q, :tables.nspec][m] - ref[m]) / allow   # <= 1 passes
         e0, e1 = element_totals(tables, f["y_in"][q]), element_totals(tables, out["y"][q])
         k = [i for i in keep if e0[i] > 0]
         cons = np.abs(e1[k] - e0[k]) / e0[k]
-        if rel.max() > 0.01 or cons.max() > 1e-10:
-            bad.append((q, str(f["how"][q]), float(rel.max()), float(cons.max())))
+        if excess.max() > 1.0 or cons.max() > 1e-10:
+            bad.append((q, str(f["how"][q]), float(excess.max()), float(cons.max())))
         # phi and abstol_factor leave the interval as the reference's do
         assert out["phys"][q, 5] == pytest.approx(f["phys_ref"][q, 5], rel=1e-12)
         assert out["phys"][q, 6] == f["phys_ref"][q, 6] or f["how"][q] == "first-attempt"

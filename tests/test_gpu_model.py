@@ -80,3 +80,34 @@ def test_get_rates_and_odes_match_oracle(tables, oracle):
         out = model.network().solve(np.stack([model.make_params(dict(d), finaltime=1e-7)]), y_init=y0[None])
         fo, r2 = oracle.rhs(pt, ro, out["y"][0])
         assert np.all(np.abs(odes[q] - fo) <= 1e-12 * term_magnitude(tables, r2, out["y"][0]))
+
+
+def test_coefficient_changes_and_rates_output_match_reference():
+    """The alpha/beta/gamma dictionary keys (wrap.f90:744-761) and return_rates
+    (ratesarray, io.f90:90-93) against the reference run with its module arrays changed the
+    same way (tests/golden/ref_coeff.npz): the static cloud to 1e5 yr, every abundance above
+    1e-14 within 1%, and the rate array at five output rows within 1e-3 relative (the
+    coefficients follow the state each interval starts from, which the two integrators reach
+    within their tolerances)."""
+    import os
+    from uclchem_amd import model
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_coeff.npz"))
+    d = dict(initialdens=1e4, initialtemp=10.0, finaltime=1e5)
+    for w in ("alpha", "beta", "gamma"):
+        sel = f["which"] == w
+        d[w] = {int(k): float(v) for k, v in zip(f["index"][sel], f["value"][sel])}
+    phys, chem, rates, start, flags = model.cloud_grid([d], return_array=True, return_rates=True)
+    assert flags[0] == 0 and chem.shape[0] == len(f["rows"])
+    ref = f["rows"][-1, 4:]
+    m = ref > 1e-14
+    assert (np.abs(chem[-1, 0][m] - ref[m]) / ref[m]).max() < 0.01
+    plain = model.cloud_grid([dict(initialdens=1e4, initialtemp=10.0, finaltime=1e5)], return_array=True)
+    assert (np.abs(plain[1][-1, 0][m] - ref[m]) / ref[m]).max() > 0.05    # the changes matter
+    assert plain[2] is None
+    got = rates[f["rate_rows"], 0, :]
+    want = f["rates"]
+    nz = np.abs(want) > 1e-30
+    assert np.all(got[~nz] == 0.0) or np.abs(got[~nz]).max() < 1e-30
+    rel = np.abs(got[nz] - want[nz]) / np.abs(want[nz])
+    assert rel.max() < 1e-3, rel.max()
+    assert np.all(rates[0, 0] == 0.0)

@@ -20,7 +20,7 @@ def _states(tables, oracle, seed, n=8):
     for trial in range(n):
         p = make_params(initialdens=10 ** rng.uniform(2, 8), initialtemp=rng.uniform(10, 300),
                         zeta=10 ** rng.uniform(0, 2), radfield=10 ** rng.uniform(0, 3), baseav=rng.uniform(0, 10),
-                        finaltime=1e6)
+                        finaltime=1e6, enforcechargeconservation=bool(trial % 2))
         pt, y = random_state(tables, oracle, rng, p, bare_grains=(trial % 3 == 0))
         P.append(p)
         PH.append(phys_of(pt))
@@ -51,9 +51,16 @@ def test_rhs_matches_oracle(gpu_net, tables, oracle):
         Yr.append(y)
         YD.append(yd)
         MAG.append(term_magnitude(tables, r2, y))
+    from uclchem_amd.makerates_hip import ion_list
+    from uclchem_amd.params import INDEX
+    ions, ne = ion_list(tables), tables.scalars["nelec"]
+    for q in range(len(P)):
+        if P[q, INDEX["enforcechargeconservation"]]:   # the electron row is the sum of the ion rows
+            MAG[q][ne] = MAG[q][ions].sum()
     ydg, rtg = gpu_net.rhs(P, PH, np.array(R), np.array(Yr))
     err = np.abs(ydg - np.array(YD))
-    assert np.all(err <= 1e-12 * np.array(MAG))
+    bad = np.argwhere(err > 1e-12 * np.array(MAG))
+    assert len(bad) == 0, [(int(q), int(i), float(err[q, i]), float(MAG[q][i])) for q, i in bad[:8]]
 
 
 def test_jacobian_directional_derivatives(gpu_net, tables, oracle):
@@ -79,26 +86,28 @@ def test_jacobian_directional_derivatives(gpu_net, tables, oracle):
             assert (best / scale).max() <= 1e-6, (q, rep)
 
 
+@pytest.mark.parametrize("enforce", [False, True])
 @pytest.mark.parametrize("kind", ["sparse", "dense"])
 @pytest.mark.parametrize("lu_bits,tol", [(64, 1e-12), (32, 1e-5)])
-def test_newton_solve_backward_error(gpu_net, tables, lu_bits, tol, kind):
-    """(I - gamma J) x = b on states along a real trajectory (config 1 rows), J from the
-    dense Jacobian kernel: checks the sparse plan's assembly, bordering, sum-row
-    replacement, factorisation and solve against the plain matrix."""
+def test_newton_solve_backward_error(gpu_net, tables, lu_bits, tol, kind, enforce):
+    """(I - gamma J) x = b on states along a real trajectory (the static cloud's rows), J
+    from the dense Jacobian kernel: checks the sparse plan's assembly, bordering, sum-row
+    replacement (BULK, SURFACE and, with charge enforcement, the electron row),
+    factorisation and solve against the plain matrix."""
     from uclchem_amd import workloads
     gpu_net.set_linear_solver(gpu_net.SPARSE if kind == "sparse" else gpu_net.DENSE)
     try:
-        _newton_backward_error(gpu_net, tables, lu_bits, tol)
+        _newton_backward_error(gpu_net, tables, lu_bits, tol, enforce)
     finally:
         gpu_net.set_linear_solver(gpu_net.SPARSE)
 
 
-def _newton_backward_error(gpu_net, tables, lu_bits, tol):
+def _newton_backward_error(gpu_net, tables, lu_bits, tol, enforce):
     from uclchem_amd import workloads
     out = gpu_net.solve(workloads.static_cloud_point(), max_rec=60)
     rows = out["rec"][0, :out["n_rec"][0]]
     from uclchem_amd.params import make_params
-    p = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e6)
+    p = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e6, enforcechargeconservation=enforce)
     cs = (float(np.float32(0.05))) * 3.086e18
     rng = np.random.default_rng(4)
     for k in (10, 25, 40):

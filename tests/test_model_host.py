@@ -16,16 +16,20 @@ class _Tables:
     species_names = ["H", "H2", "CO", "#CO"]
 
 
+NREAC = 3
+
+
 class _FakeNet:
-    nspec, neq = NSPEC, NSPEC + 1
+    nspec, neq, nreac = NSPEC, NSPEC + 1, NREAC
     t = _Tables()
 
-    def __init__(self):
+    def __init__(self, tag=0.0):
         self.calls = []
+        self.tag = tag
 
-    def solve(self, params, y_init=None, max_rec=0, lu_bits=64, order=None):
+    def solve(self, params, y_init=None, max_rec=0, lu_bits=64, order=None, return_rates=False):
         self.calls.append(dict(params=params.copy(), y_init=None if y_init is None else y_init.copy(),
-                               max_rec=max_rec, order=order))
+                               max_rec=max_rec, order=order, return_rates=return_rates))
         n = params.shape[0]
         dens = params[:, INDEX["initialdens"]]
         n_rec = (2 + np.arange(n) % 3).astype(np.int32)          # 2, 3, 4, 2, ... rows written
@@ -42,7 +46,11 @@ class _FakeNet:
             y[q, :NSPEC] = rec[q, n_rec[q] - 1, 4:]
             y[q, NSPEC] = dens[q]
         status = np.where(dens > 1e6, -5, 0).astype(np.int32)
-        return dict(y=y, rec=rec, n_rec=n_rec, status=status, stats=np.zeros((n, 8), np.int32))
+        out = dict(y=y, rec=rec, n_rec=n_rec, status=status, stats=np.zeros((n, 8), np.int32))
+        if return_rates:   # rate k of row r: density + 10 r + k + the network's tag
+            out["rates"] = (dens[:, None, None] + 10.0 * np.arange(max_rec)[None, :, None]
+                            + np.arange(NREAC)[None, None, :] + self.tag)
+        return out
 
 
 @pytest.fixture
@@ -94,3 +102,32 @@ def test_starting_chemistry_and_hot_core_arguments(net):
     model.hot_core_grid(3, 250.0, [{"initialDens": 1e6}])
     p = net.calls[-1]["params"][0]
     assert (p[INDEX["model"]], p[INDEX["tempindx"]], p[INDEX["maxtemp"]]) == (1.0, 3.0, 250.0)
+
+
+def test_rates_rows_and_coefficient_groups(net, monkeypatch):
+    """return_rates gives ratesArray[rows, points, nreac] trimmed like the abundances; points
+    whose dictionaries carry the same alpha/beta/gamma changes run together on one derived
+    network, and every result lands at its point's index."""
+    derived = {}
+
+    def coeff_net(name, changes):
+        if not changes:
+            return net
+        return derived.setdefault(changes, _FakeNet(tag=1000.0 * (len(derived) + 1)))
+    monkeypatch.setattr(model, "_coefficient_network", coeff_net)
+    dicts = [{"initialDens": 1e3, "alpha": {2: 1e-9}}, {"initialDens": 1e4}, {"initialDens": 1e5, "ALPHA": {"2": 1e-9}},
+             {"initialDens": 2e3, "gamma": {1: 10.0}, "beta": {3: 0.5}}]
+    phys, chem, rates, start, flags = model.cloud_grid(dicts, return_array=True, return_rates=True, timepoints=10)
+    assert len(derived) == 2 and len(net.calls) == 1 and net.calls[0]["return_rates"]
+    a = derived[(("alpha", ((2, 1e-9),)),)]
+    assert [c["params"].shape[0] for c in a.calls] == [2]
+    assert rates.shape == (3, 4, NREAC)
+    n_rec = {0: 2, 1: 2, 2: 3, 3: 2}          # the fake writes 2 + (position in its launch) % 3 rows
+    tags = [1000.0, 0.0, 1000.0, 2000.0]
+    for q, d in enumerate(dicts):
+        dens = d["initialDens"]
+        assert np.all(phys[:n_rec[q], q, 1] == dens)
+        for r in range(n_rec[q]):
+            assert np.array_equal(rates[r, q], dens + 10.0 * r + np.arange(NREAC) + tags[q])
+        assert np.all(rates[n_rec[q]:, q] == 0)
+    assert model.cloud_grid(dicts[1:2], return_array=True)[2] is None

@@ -142,3 +142,59 @@ def test_assembly_maps_address_their_entries(plan, tables):
         got = sorted(plan["nw_gs_slot"][plan["nw_gs_ptr"][j]:plan["nw_gs_ptr"][j + 1]])
         want = sorted(int(jslot[q]) for q in range(len(jslot)) if a["jnz_col"][q] == j and int(a["jnz_row"][q]) in surf)
         assert got == want
+
+
+def test_charge_row_slots(plan, tables):
+    """The charge row lists every assembly slot of the electron row and its positive-ion
+    entries (ion_list order)."""
+    from uclchem_amd.makerates_hip import ion_list
+    rc = slot_table(plan)
+    inv = plan["nw_inv"]
+    a = tables.arrays
+    NE = tables.scalars["nelec"]
+    ions = ion_list(tables)
+    assert len(ions) > 0 and len(plan["nw_eion"]) == len(ions)
+    erow = set(int(q) for q in plan["nw_erow"])
+    assert all(rc[q, 0] == inv[NE] for q in erow)
+    assert all(tuple(rc[q]) == (inv[NE], inv[i]) for q, i in zip(plan["nw_eion"], ions))
+    assert set(int(q) for q in plan["nw_eion"]) <= erow
+    assert int(plan["nw_diag_slot"][NE]) in erow
+    assert {int(q) for q, i in zip(plan["nw_jnz_slot"], a["jnz_row"]) if i == NE} <= erow
+
+
+def test_charge_row_replacement_is_exact(plan, tables):
+    """With charge enforcement the Newton matrix's electron row is e_e - gamma sum_ion J_ion.
+    The device clears the electron row's slots, writes 1 and -1 (e_e - sum_ion e_ion) and
+    subtracts the ion entries of the right-hand side from the electron's: the plan's
+    schedule on that system gives the solution of the true matrix (numpy.linalg.solve)."""
+    from uclchem_amd.makerates_hip import ion_list
+    rc = slot_table(plan)
+    s = npl.plan_summary(plan)
+    N = s["N"]
+    perm, inv = plan["nw_perm"], plan["nw_inv"]
+    NE = tables.scalars["nelec"]
+    ions = ion_list(tables)
+    struct = set()
+    for k in ("nw_jnz_slot", "nw_diag_slot", "nw_ph_slot", "nw_tr_slot", "nw_s1c", "nw_s1r", "nw_s2c", "nw_s2r",
+              "nw_misc", "nw_nbrow", "nw_nsrow"):
+        struct |= {int(x) for x in plan[k] if x >= 0}
+    idx = np.array(sorted(struct))
+    rng = np.random.default_rng(5)
+    for trial in range(2):
+        v = np.zeros(len(rc))
+        v[idx] = rng.uniform(-1, 1, len(idx))
+        v[[q for q in idx if rc[q, 0] == rc[q, 1]]] += 30.0
+        A = np.zeros((N, N))                       # original indices
+        A[perm[rc[idx, 0]], perm[rc[idx, 1]]] = v[idx]
+        Mt = A.copy()
+        Mt[NE] = np.eye(N)[NE] + (A[ions] - np.eye(N)[ions]).sum(axis=0)
+        b = rng.standard_normal(N)
+        v[plan["nw_erow"]] = 0.0
+        v[plan["nw_eion"]] = -1.0
+        v[plan["nw_diag_slot"][NE]] = 1.0
+        bt = b.copy()
+        bt[NE] -= b[ions].sum()
+        x = np.empty(N)
+        x[perm] = execute(plan, v, bt[perm])
+        ref = np.linalg.solve(Mt, b)
+        assert np.allclose(x, ref, rtol=1e-10, atol=1e-11 * np.abs(ref).max())

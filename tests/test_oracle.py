@@ -122,7 +122,7 @@ def test_oracle_rates_rhs_bit_exact_vs_reference(reflib, oracle, tables, seed):
             y[-1] = y0[-1]
 
 
-def _nondefault(rng):
+def _nondefault(rng, enforce=0.0):
     """A seeded parameter vector away from every default the elemental start, the
     desorption constants and the switches read."""
     from oracle import make_params
@@ -139,7 +139,7 @@ def _nondefault(rng):
                        omega=rng.uniform(0.2, 0.8), desorb=float(sw[0] or sw[1]), h2desorb=float(sw[1]),
                        crdesorb=float(sw[2]), uvdesorb=float(sw[3]), thermdesorb=float(sw[4]),
                        cosmicrayattenuation=float(sw[5]), ionmodel=float(rng.integers(0, 2)),
-                       improvedh2crpdissociation=float(sw[5]))
+                       improvedh2crpdissociation=float(sw[5]), enforcechargeconservation=enforce)
 
 
 @pytest.mark.parametrize("seed", [3, 4])
@@ -149,7 +149,7 @@ def test_oracle_nondefault_parameters_bit_exact_vs_reference(reflib, oracle, tab
     reference (driven through its parameter globals)."""
     rng = np.random.default_rng(seed)
     for trial in range(8):
-        p = _nondefault(rng)
+        p = _nondefault(rng, enforce=float(trial % 2))
         rc, y0r = reflib.init(p)
         pt, y0 = oracle.init_point(p)
         assert rc == 0 and np.array_equal(y0r, y0), trial

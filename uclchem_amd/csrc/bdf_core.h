@@ -39,14 +39,16 @@ namespace bdf {
 
 constexpr int QMAX = 5;
 constexpr double UROUND = 2.220446049250313e-16;
-// step/order control constants (VODE/CVODE published defaults)
+// step/order control constants (VODE/CVODE published defaults; the error-test failure
+// limits are DVODE_F90's: order crash from the third consecutive failure (KFC = -3), give up
+// at the fifteenth (KFH = -15, raised from VODE's 7), dvode.f90:1865-1876)
 constexpr double ADDON = 1.0e-6, BIAS1 = 6.0, BIAS2 = 6.0, BIAS3 = 10.0, THRESH = 1.5;
 constexpr double ETAMX1 = 1.0e4, ETAMX2 = 10.0, ETAMX3 = 10.0, ETAMXF = 0.2, ETAMIN = 0.1;
 #ifndef BDF_DGMAX
 #define BDF_DGMAX 0.3
 #endif
 constexpr double ETACF = 0.25, CRDOWN = 0.3, DGMAX = BDF_DGMAX, RDIV = 2.0, NLSCOEF = 0.1;
-constexpr int SMALL_NST = 10, MXNEF = 7, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
+constexpr int SMALL_NST = 10, MXNEF = 15, MXNEF1 = 3, SMALL_NEF = 2, LONG_WAIT = 10;
 constexpr int MSBP = 20, MAXCOR = 3, MXNCF = 10, HIN_ITERS = 4;
 // saved-Jacobian reuse (DVODE's JSV=1 / CVODE's lsetup): re-evaluate J at most every
 // MSBJ steps, after a convergence failure, or when a stale-J failure is not explained by
@@ -317,7 +319,7 @@ BDF_HD int step(Ops &o, State &s, Stats &st)
         restore(o, s, saved_t);
         if (nef == MXNEF) return -4;
         s.etamax = 1.0;
-        if (nef <= MXNEF1) {
+        if (nef < MXNEF1) {
             s.eta = 1.0 / (pow(BIAS2 * dsm, 1.0 / s.L) + ADDON);
             s.eta = dmax(ETAMIN, s.eta);
             if (nef >= SMALL_NEF) s.eta = dmin(s.eta, ETAMXF);

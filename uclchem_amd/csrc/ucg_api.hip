@@ -313,7 +313,7 @@ struct FixedWs {
 int launch_solve(NetHandle &h, int n, const double *params, const double *y_init, double *y_out, int max_rec,
                  double *rec, int *n_rec, int *status, int *stats, double *t_start, double *t_done, int lu_bits,
                  int n_slots, hipStream_t stream, double *timers = nullptr, const IntervalArgs &iv = IntervalArgs(),
-                 const FixedWs &fixed = FixedWs())
+                 const FixedWs &fixed = FixedWs(), double *rates_rec = nullptr)
 {
     const ucg::DevNet &N = h.dev;
     if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
@@ -341,6 +341,7 @@ int launch_solve(NetHandle &h, int n, const double *params, const double *y_init
     A.y_out = y_out;
     A.max_rec = rec ? max_rec : 0;
     A.rec = rec;
+    A.rates_rec = A.max_rec > 0 ? rates_rec : nullptr;
     A.n_rec = n_rec;
     A.status = status;
     A.stats = stats;
@@ -529,6 +530,15 @@ static int check_plan(const ucg_network_desc *d)
         !in(d->nw_s2r, Q, -1, nval) || !mono(d->nw_gs_ptr, Q, 0, ngs) || !in(d->nw_gs_slot, ngs, 0, nval) ||
         !in(d->nw_misc, 5, 0, nval) || !in(d->nw_nbrow, d->nbulkl, 0, nval) || !in(d->nw_nsrow, d->nsurfl, 0, nval))
         return bad("assembly slots");
+    // charge row: present whenever the network has positive ions; its ion slots lie in it
+    if (d->nw_nerow < 0 || (d->nion > 0 && d->nw_nerow == 0) || !in(d->nw_erow, d->nw_nerow, 0, nval) ||
+        !in(d->nw_eion, d->nion, 0, nval))
+        return bad("charge row slots");
+    for (int q = 0; q < d->nion; q++) {
+        bool found = false;
+        for (int r = 0; r < d->nw_nerow && !found; r++) found = d->nw_erow[r] == d->nw_eion[q];
+        if (!found) return bad("charge row slots");
+    }
     long naff_photo = 0;
     for (int k : {d->special_reactions[6], d->special_reactions[7]}) {
         if (k < 0 || k >= d->nreac) return bad("photo reactions");
@@ -679,6 +689,10 @@ int ucg_network_load_file(const char *path, int *handle)
         GI(nw_jnz_slot, d.nnz); GI(nw_diag_slot, Q); GI(nw_ph_slot, z[8]); GI(nw_tr_slot, 8 * (int64_t)d.nsurfl);
         GI(nw_s1c, Q); GI(nw_s1r, Q); GI(nw_s2c, Q); GI(nw_s2r, Q); GI(nw_gs_ptr, Q + 1); GI(nw_gs_slot, z[9]);
         GI(nw_misc, 5); GI(nw_nbrow, d.nbulkl); GI(nw_nsrow, d.nsurfl);
+        if ((rc = find("nw_nerow", 1, 1, &p))) return rc;
+        memcpy(&d.nw_nerow, p, sizeof d.nw_nerow);
+        if (d.nw_nerow < 0) return fail(UCG_E_ARG, "network file record nw_nerow is negative");
+        GI(nw_erow, d.nw_nerow); GI(nw_eion, d.nion);
 #undef GI
     }
     return ucg_network_load(&d, handle);
@@ -836,6 +850,8 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         UP(nw_tr_slot, (size_t)8 * d->nsurfl); UP(nw_s1c, Q); UP(nw_s1r, Q); UP(nw_s2c, Q); UP(nw_s2r, Q);
         UP(nw_gs_ptr, Q + 1); UP(nw_gs_slot, (size_t)z[9]); UP(nw_misc, (size_t)5);
         UP(nw_nbrow, (size_t)d->nbulkl); UP(nw_nsrow, (size_t)d->nsurfl);
+        N.nw_nerow = d->nw_nerow;
+        UP(nw_erow, (size_t)d->nw_nerow); UP(nw_eion, (size_t)d->nion);
         // solve row records: sparse rows in level order, then the tail rows
         std::vector<int> sv(4 * NN);
         for (size_t q = 0; q < NN; q++) {
@@ -1074,8 +1090,8 @@ int ucg_solve_batch_profiled(int handle, int n, const double *params, double *y_
     return 0;
 }
 
-int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out, int max_rec,
-                    double *rec, int *n_rec, int *status, int *stats, int lu_bits)
+static int solve_batch_host(int handle, int n, const double *params, const double *y_init, double *y_out, int max_rec,
+                            double *rec, double *rates_rec, int *n_rec, int *status, int *stats, int lu_bits)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     NetHandle *h = get(handle);
@@ -1083,27 +1099,43 @@ int ucg_solve_batch(int handle, int n, const double *params, const double *y_ini
     if (n <= 0 || !params || !y_out || !status || !stats) return fail(UCG_E_ARG, "bad arguments");
     int rc = check_params(n, params);
     if (rc) return rc;
-    const int neq = h->dev.neq, ns = h->dev.nspec;
+    const int neq = h->dev.neq, ns = h->dev.nspec, nr = h->dev.nreac;
     if (!rec) max_rec = 0;
-    DevBuf dp, dy0, dy, drec, dnrec, dst, dstat;
+    DevBuf dp, dy0, dy, drec, drr, dnrec, dst, dstat;
     if ((rc = to_dev(dp, params, (size_t)n * UCG_NPARAM))) return rc;
     if (y_init && (rc = to_dev(dy0, y_init, (size_t)n * neq))) return rc;
     if ((rc = alloc_dev<double>(dy, (size_t)n * neq))) return rc;
     if (max_rec > 0 && (rc = alloc_dev<double>(drec, (size_t)n * max_rec * (4 + ns)))) return rc;
+    if (max_rec > 0 && rates_rec && (rc = alloc_dev<double>(drr, (size_t)n * max_rec * nr))) return rc;
     if ((rc = alloc_dev<int>(dnrec, (size_t)n))) return rc;
     if ((rc = alloc_dev<int>(dst, (size_t)n))) return rc;
     if ((rc = alloc_dev<int>(dstat, (size_t)n * 8))) return rc;
     rc = launch_solve(*h, n, (const double *)dp.p, (const double *)dy0.p, (double *)dy.p, max_rec,
                       (double *)drec.p, (int *)dnrec.p, (int *)dst.p, (int *)dstat.p, nullptr, nullptr, lu_bits, 0,
-                      nullptr);
+                      nullptr, nullptr, IntervalArgs(), FixedWs(), (double *)drr.p);
     if (rc) return rc;
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(y_out, dy.p, (size_t)n * neq * sizeof(double), hipMemcpyDeviceToHost));
     if (max_rec > 0) HIPCHK(hipMemcpy(rec, drec.p, (size_t)n * max_rec * (4 + ns) * sizeof(double), hipMemcpyDeviceToHost));
+    if (drr.p)
+        HIPCHK(hipMemcpy(rates_rec, drr.p, (size_t)n * max_rec * nr * sizeof(double), hipMemcpyDeviceToHost));
     if (n_rec) HIPCHK(hipMemcpy(n_rec, dnrec.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(status, dst.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(stats, dstat.p, (size_t)n * 8 * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
+}
+
+int ucg_solve_batch(int handle, int n, const double *params, const double *y_init, double *y_out, int max_rec,
+                    double *rec, int *n_rec, int *status, int *stats, int lu_bits)
+{
+    return solve_batch_host(handle, n, params, y_init, y_out, max_rec, rec, nullptr, n_rec, status, stats, lu_bits);
+}
+
+int ucg_solve_batch_rates(int handle, int n, const double *params, const double *y_init, double *y_out, int max_rec,
+                          double *rec, double *rates_rec, int *n_rec, int *status, int *stats, int lu_bits)
+{
+    if (!rec || max_rec <= 0 || !rates_rec) return fail(UCG_E_ARG, "rates output needs rec, max_rec > 0 and rates_rec");
+    return solve_batch_host(handle, n, params, y_init, y_out, max_rec, rec, rates_rec, n_rec, status, stats, lu_bits);
 }
 
 int ucg_interval(int handle, int n, const double *params, const double *phys, const double *y,

@@ -112,6 +112,8 @@ struct DevNet {
     const int32_t *nw_tg_slot, *nw_tg_ptr, *nw_tg_div, *nw_pr_l, *nw_pr_u, *nw_jnz_slot, *nw_diag_slot;
     const int32_t *nw_ph_slot, *nw_tr_slot, *nw_s1c, *nw_s1r, *nw_s2c, *nw_s2r, *nw_gs_ptr, *nw_gs_slot;
     const int32_t *nw_misc, *nw_nbrow, *nw_nsrow;
+    int32_t nw_nerow;                   // charge row (enforceChargeConservation): slots of the electron row,
+    const int32_t *nw_erow, *nw_eion;   // then of its entries in the positive-ion columns
     const int32_t *sv_rec;   // solve rows {row, first, diagonal, end}: sparse rows in level order, then tail rows
     const int32_t *nw_pr;    // Crout pairs interleaved {L slot, U slot}
     // the last elimination phase is the Schur-complement update of the dense tail; its

@@ -498,6 +498,14 @@ UCG_BIG __device__ void block_rhs(const DevNet &N, const Lds L, uint64_t *split 
     }
     if (tid() == 0) sc[S_SPRE] = S;
     __syncthreads();
+    if (prm[P_ENFORCECHARGE] != 0.0 && N.nelec < N.nspec) {
+        // enforceChargeConservation: ydot(e-) is the sum of the positive-ion rows (chemistry.f90:333-335)
+        double e = 0.0;
+        for (int q = tid(); q < N.nion; q += NT) e += f[N.ion_list[q]];
+        e = block_sum(e, L.red);
+        if (tid() == 0) f[N.nelec] = e;
+        __syncthreads();
+    }
     if (split) {   // totalSwap, photo rates + fluxes, chunked sums, species totals + transfer
         split[0] += c1 - c0;
         split[1] += c2 - c1;
@@ -722,6 +730,15 @@ UCG_BIG __device__ void block_newton_matrix(const DevNet &N, const Lds L, LUT *M
         M[(size_t)j * ld + N.nsurface] = (LUT)ss;
     }
     __syncthreads();
+    // 5b. with enforceChargeConservation the electron row is the sum of the positive-ion rows
+    if (L.prm[P_ENFORCECHARGE] != 0.0 && N.nelec < N.nspec) {
+        for (int j = tid(); j < neq; j += NT) {
+            double se = 0.0;
+            for (int q = 0; q < N.nion; q++) se += (double)M[(size_t)j * ld + N.ion_list[q]];
+            M[(size_t)j * ld + N.nelec] = (LUT)se;
+        }
+        __syncthreads();
+    }
     if (!convert) return;
     // 6. M = I - gamma W J W^-1 (W = I unless the caller equilibrates)
     if (!scaled) {
@@ -1400,7 +1417,7 @@ UCG_BIG __device__ void sparse_jacobian(const DevNet &N, const Lds L, double *V)
 // that carry gamma are scaled, the rank-one column factors and c1 are not; then the
 // identity, the border diagonals and the replaced BULK/SURFACE rows.
 template <typename LUT>
-UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT *V, double gamma)
+UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT *V, double gamma, bool enforce)
 {
     for (int q = tid(); q < N.nw_nval; q += NT) V[q] = (LUT)(N.nw_unscaled[q] ? J[q] : gamma * J[q]);
     __syncthreads();
@@ -1417,6 +1434,13 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
         V[N.nw_misc[4]] = (LUT)1.0;
     }
     __syncthreads();
+    if (enforce && N.nelec < N.nspec) {   // charge row: row(e) - sum_ion row(ion) = e_e - sum_ion e_ion
+        for (int q = tid(); q < N.nw_nerow; q += NT) V[N.nw_erow[q]] = (LUT)0.0;
+        __syncthreads();
+        for (int q = tid(); q < N.nion; q += NT) V[N.nw_eion[q]] = (LUT)(-1.0);
+        if (tid() == 0) V[N.nw_diag_slot[N.nelec]] = (LUT)1.0;
+        __syncthreads();
+    }
 }
 
 // Crout elimination of the sparse pivots in plan phases (one thread per target entry,
@@ -1661,6 +1685,13 @@ UCG_BIG __device__ void sparse_lu_solve(const DevNet &N, const Lds L, const LUT 
 {
     const uint64_t c0 = wall_clock64();
     const int NN = N.nw_N, nsp = N.nw_nsp;
+    if (L.prm[P_ENFORCECHARGE] != 0.0 && N.nelec < N.nspec) {   // right-hand side of the charge row
+        double se = 0.0;
+        for (int q = tid(); q < N.nion; q += NT) se += b[N.ion_list[q]];
+        se = block_sum(se, L.red);
+        if (tid() == 0) b[N.nelec] -= se;
+        __syncthreads();
+    }
     double sb = 0.0, ss = 0.0;
     for (int q = tid(); q < N.nbulkl; q += NT) sb += b[N.bulk_list[q]];
     for (int q = tid(); q < N.nsurfl; q += NT) ss += b[N.surface_list[q]];
@@ -1938,7 +1969,7 @@ struct GpuOps {
         uint64_t t1;
         if (sparse) {
             if (new_jacobian) sparse_jacobian(N, L, Jsave);
-            sparse_form_newton<LUT>(N, Jsave, M, gamma);
+            sparse_form_newton<LUT>(N, Jsave, M, gamma, L.prm[P_ENFORCECHARGE] != 0.0);
             t1 = wall_clock64();
             r = sparse_lu<LUT>(N, M, L.flux, L.red, L.b, t_lu_split);
         } else {
@@ -2001,6 +2032,7 @@ struct SolveArgs {
     double *y_out;            // [n][neq]
     int max_rec;              // rows per point in rec (0: none)
     double *rec;              // [n][max_rec][4 + nspec]
+    double *rates_rec;        // [n][max_rec][nreac] rate coefficients per output row, or null
     int *n_rec;               // [n]
     int *status;              // [n]
     int *stats;               // [n][8]: nst nfe nje nni ncfn netf n_intervals n_retries
@@ -2109,6 +2141,13 @@ __device__ void record_row(const DevNet &N, const Lds L, const SolveArgs &A, int
         r[3] = L.sc[S_AV];
     }
     for (int i = tid(); i < ns; i += NT) r[4 + i] = L.y[i];
+    if (A.rates_rec) {
+        // ratesarray (io.f90:90-93): the rate array as the row is written, i.e. the last
+        // attempt's coefficients with the H2/CO photo rates of the last RHS evaluation; before
+        // the first interval the point has no rates yet (zeros)
+        double *rr = A.rates_rec + ((size_t)pidx * A.max_rec + row) * N.nreac;
+        for (int k = tid(); k < N.nreac; k += NT) rr[k] = row ? L.rate[k] : 0.0;
+    }
 }
 
 // LDS layout shared by all kernels; the rate coefficients live in the workspace slot
@@ -2513,7 +2552,7 @@ __global__ __launch_bounds__(NT) void newton_solve_kernel(const DevNet *__restri
     if (A.sparse) {
         double *J = (double *)((char *)A.ws + (size_t)blockIdx.x * A.ws_stride + A.ws_mat_bytes);
         sparse_jacobian(N, L, J);
-        sparse_form_newton<LUT>(N, J, M, A.gamma);
+        sparse_form_newton<LUT>(N, J, M, A.gamma, L.prm[P_ENFORCECHARGE] != 0.0);
         bad = sparse_lu<LUT>(N, M, L.flux, L.red, L.b);
     } else {
         block_newton_matrix<LUT>(N, L, M, A.gamma);

@@ -61,6 +61,17 @@ MODULE uclchem_gpu
             INTEGER(C_INT), INTENT(OUT) :: n_rec(*), status(*), stats(*)
         END FUNCTION c_solve_batch
 
+        INTEGER(C_INT) FUNCTION c_solve_batch_rates(handle, n, params, y_init, y_out, max_rec, rec, rates, n_rec, &
+                                                    status, stats, lu_bits) BIND(C, name="ucg_solve_batch_rates")
+            IMPORT :: C_INT, C_DOUBLE, C_PTR
+            INTEGER(C_INT), VALUE :: handle, n, max_rec, lu_bits
+            REAL(C_DOUBLE), INTENT(IN) :: params(*)
+            TYPE(C_PTR), VALUE :: y_init
+            REAL(C_DOUBLE), INTENT(OUT) :: y_out(*)
+            TYPE(C_PTR), VALUE :: rec, rates
+            INTEGER(C_INT), INTENT(OUT) :: n_rec(*), status(*), stats(*)
+        END FUNCTION c_solve_batch_rates
+
         ! stop running device solves on a handle (points not finished report cancelled)
         INTEGER(C_INT) FUNCTION ucg_cancel(handle) BIND(C, name="ucg_cancel")
             IMPORT :: C_INT
@@ -132,8 +143,10 @@ CONTAINS
 
     ! Solve npoints independent grid points (the solveAbundances loop of wrap.f90:549-697
     ! per point).  abund_init absent: UCLCHEM's elemental start.  rec optional.
+    ! rates (optional, with rec): the rate coefficients of every output row, rates(nreac, max_rec, n)
+    ! (the reference's ratesarray, io.f90:90-93; row 1 is written before any interval and is zero)
     INTEGER FUNCTION ucg_solve_batch_f(handle, params, abund_out, status, stats, abund_init, rec, n_rec, &
-                                       lu_bits) RESULT(rc)
+                                       lu_bits, rates) RESULT(rc)
         INTEGER, INTENT(IN) :: handle
         REAL(C_DOUBLE), INTENT(IN) :: params(:, :)
         REAL(C_DOUBLE), INTENT(OUT) :: abund_out(:, :)
@@ -142,6 +155,7 @@ CONTAINS
         REAL(C_DOUBLE), INTENT(OUT), OPTIONAL, TARGET, CONTIGUOUS :: rec(:, :, :)
         INTEGER(C_INT), INTENT(OUT), OPTIONAL :: n_rec(:)
         INTEGER, INTENT(IN), OPTIONAL :: lu_bits
+        REAL(C_DOUBLE), INTENT(OUT), OPTIONAL, TARGET, CONTIGUOUS :: rates(:, :, :)
         INTEGER(C_INT) :: n, bits, max_rec, nspec, neq, nreac
         INTEGER(C_INT), ALLOCATABLE :: nrec_local(:)
         TYPE(C_PTR) :: p_init, p_rec
@@ -161,6 +175,10 @@ CONTAINS
         IF (PRESENT(n_rec)) THEN
             IF (SIZE(n_rec) < n) RETURN
         END IF
+        IF (PRESENT(rates)) THEN
+            IF (.NOT. PRESENT(rec)) RETURN
+            IF (SIZE(rates, 1) /= nreac .OR. SIZE(rates, 2) /= SIZE(rec, 2) .OR. SIZE(rates, 3) < n) RETURN
+        END IF
         bits = 64
         IF (PRESENT(lu_bits)) bits = lu_bits
         p_init = C_NULL_PTR
@@ -172,8 +190,13 @@ CONTAINS
             max_rec = SIZE(rec, 2)
         END IF
         ALLOCATE(nrec_local(n))
-        rc = c_solve_batch(INT(handle, C_INT), n, params, p_init, abund_out, max_rec, p_rec, nrec_local, status, &
-                           stats, bits)
+        IF (PRESENT(rates)) THEN
+            rc = c_solve_batch_rates(INT(handle, C_INT), n, params, p_init, abund_out, max_rec, p_rec, C_LOC(rates), &
+                                     nrec_local, status, stats, bits)
+        ELSE
+            rc = c_solve_batch(INT(handle, C_INT), n, params, p_init, abund_out, max_rec, p_rec, nrec_local, status, &
+                               stats, bits)
+        END IF
         IF (PRESENT(n_rec)) n_rec = nrec_local
     END FUNCTION ucg_solve_batch_f
 

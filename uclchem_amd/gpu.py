@@ -37,7 +37,8 @@ class _Desc(ctypes.Structure):
                 [("special_species", ctypes.c_int * 25), ("special_reactions", ctypes.c_int * 10)] +
                 [("nw_sizes", ctypes.c_int * 10)] + [(n, ctypes.c_void_p) for n in mh.DESC_NW] +
                 [("nion", ctypes.c_int), ("ion_list", ctypes.c_void_p), ("ngar", ctypes.c_int),
-                 ("gar_params", ctypes.c_void_p)])
+                 ("gar_params", ctypes.c_void_p), ("nw_nerow", ctypes.c_int), ("nw_erow", ctypes.c_void_p),
+                 ("nw_eion", ctypes.c_void_p)])
 
 
 class GpuError(RuntimeError):
@@ -137,19 +138,24 @@ class Network:
 
     # ------------------------------------------------------------ full solves
     def solve(self, params: np.ndarray, y_init: Optional[np.ndarray] = None, max_rec: int = 0,
-              lu_bits: int = 64, order=None):
+              lu_bits: int = 64, order=None, return_rates: bool = False):
         """Host-array solve (ucg_solve_batch).  order: None (index order) or "densest"
         (points taken highest initial density first, as solve_device); results are returned
-        in index order and are the same either way."""
+        in index order and are the same either way.  return_rates (needs max_rec > 0): also
+        the rate coefficients of every output row, "rates" [n, max_rec, nreac]
+        (ucg_solve_batch_rates)."""
         params = np.atleast_2d(params)
         n = params.shape[0]
         params = _rows(params, n, NPARAM, "params")
         y0 = None if y_init is None else _rows(y_init, n, self.neq, "y_init")
+        if return_rates and max_rec <= 0:
+            raise ValueError("return_rates needs max_rec > 0")
         if order is not None:
             if order != "densest":
                 raise ValueError(f"unknown order {order!r} (expected 'densest')")
             perm = np.argsort(-params[:, INDEX["initialdens"]], kind="stable")
-            out = self.solve(params[perm], None if y0 is None else y0[perm], max_rec=max_rec, lu_bits=lu_bits)
+            out = self.solve(params[perm], None if y0 is None else y0[perm], max_rec=max_rec, lu_bits=lu_bits,
+                             return_rates=return_rates)
             back = np.empty_like(perm)
             back[perm] = np.arange(n)
             return {k: (None if v is None else v[back]) for k, v in out.items()}
@@ -158,11 +164,20 @@ class Network:
         n_rec = np.zeros(n, dtype=np.int32)
         status = np.zeros(n, dtype=np.int32)
         stats = np.zeros((n, 8), dtype=np.int32)
+        out = dict(y=y_out, rec=rec, n_rec=n_rec, status=status, stats=stats)
+        if return_rates:
+            rates = np.zeros((n, max_rec, self.nreac))
+            rc = lib().ucg_solve_batch_rates(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(y0),
+                                             _p(y_out), ctypes.c_int(max_rec), _p(rec), _p(rates), _p(n_rec),
+                                             _p(status), _p(stats), ctypes.c_int(lu_bits))
+            _check(rc, "ucg_solve_batch_rates")
+            out["rates"] = rates
+            return out
         rc = lib().ucg_solve_batch(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(y0), _p(y_out),
                                    ctypes.c_int(max_rec), _p(rec), _p(n_rec), _p(status), _p(stats),
                                    ctypes.c_int(lu_bits))
         _check(rc, "ucg_solve_batch")
-        return dict(y=y_out, rec=rec, n_rec=n_rec, status=status, stats=stats)
+        return out
 
     def interval(self, params, phys, y, t_target, lu_bits: int = 64):
         """One updateChemistry interval per point (ucg_interval): from the state (phys, y) —

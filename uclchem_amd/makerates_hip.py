@@ -689,6 +689,30 @@ def _static_rate_constants(names, species, reactions, re, pr, rtype, alpha, gama
 UNSUPPORTED_TYPES: tuple = ()   # every reaction type of rates.f90 has a device rate law
 
 
+def with_coefficients(t: NetworkTables, alpha=None, beta=None, gamma=None) -> NetworkTables:
+    """A copy of the tables with reaction coefficients replaced: the reference's ``alpha``,
+    ``beta`` and ``gamma`` parameter-dictionary keys, each a dictionary from the 1-based
+    reaction index to its new value (wrap.f90:744-761, coefficientParser :985-1019).  The
+    network constants derived from them (LH tunnelling widths and LHDES/ERDES desorption
+    fractions, which the reference computes from gama at every rate evaluation) are
+    recomputed, so the copy behaves as the reference does after the same assignment."""
+    a = {k: v.copy() for k, v in t.arrays.items()}
+    for key, changes in (("alpha", alpha), ("beta", beta), ("gama", gamma)):
+        for i, v in (changes or {}).items():
+            k = int(i)
+            if not 1 <= k <= t.nreac:
+                raise ValueError(f"{key if key != 'gama' else 'gamma'} index {i} outside 1..{t.nreac}")
+            a[key][k - 1] = float(v)
+    names = list(t.species_names)
+    idx = {n: i for i, n in enumerate(names)}
+    a["rstat"], a["ice_kind"] = _static_rate_constants(
+        names, None, range(t.nreac), a["re"], a["pr"], a["rtype"], a["alpha"], a["gama"], a["reduced_mass"],
+        a["mass"], [int(i) for i in a["ice_list"]], [int(i) for i in a["gas_ice_list"]], a["binding_energy"],
+        a["formation_enthalpy"], a["n_atoms"], [int(i) for i in a["bulk_list"]], idx, t.nspec)
+    return NetworkTables(name=t.name, species_names=list(t.species_names), nspec=t.nspec, neq=t.neq, nreac=t.nreac,
+                         arrays=a, scalars=dict(t.scalars))
+
+
 # ---------------------------------------------------------------- persistence
 
 def save_network(t: NetworkTables, path: str) -> None:
@@ -772,6 +796,9 @@ def device_descriptor(t: NetworkTables):
     sizes["nw_sizes"] = np.ascontiguousarray(a["nw_sizes"], dtype=np.int32)
     arrays["ion_list"] = ion_list(t)
     sizes["nion"] = len(arrays["ion_list"])
+    for n in ("nw_erow", "nw_eion"):
+        arrays[n] = np.ascontiguousarray(a[n], dtype=np.int32)
+    sizes["nw_nerow"] = len(arrays["nw_erow"])
     gar = np.ascontiguousarray(np.asarray(a.get("gar_params", np.zeros((0, 7))), dtype=np.float64).reshape(-1, 7))
     arrays["gar_params"] = gar.reshape(-1)
     sizes["ngar"] = len(gar)
@@ -798,8 +825,10 @@ def save_network_bin(t: NetworkTables, path: str) -> None:
     sizes, arrays, ssp, srx = device_descriptor(t)
     recs = [("sizes", np.array([sizes[k] for k in DESC_SIZES], dtype=np.int32)),
             ("special_species", ssp), ("special_reactions", srx), ("nw_sizes", sizes["nw_sizes"]),
-            ("nion", np.array([sizes["nion"]], dtype=np.int32)), ("ngar", np.array([sizes["ngar"]], dtype=np.int32))]
-    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8 + DESC_NW + ["ion_list", "gar_params"]]
+            ("nion", np.array([sizes["nion"]], dtype=np.int32)), ("ngar", np.array([sizes["ngar"]], dtype=np.int32)),
+            ("nw_nerow", np.array([sizes["nw_nerow"]], dtype=np.int32))]
+    recs += [(n, arrays[n]) for n in DESC_F64 + DESC_I32 + DESC_I8 + DESC_NW +
+             ["ion_list", "gar_params", "nw_erow", "nw_eion"]]
     import struct
     with open(path, "wb") as f:
         f.write(BIN_MAGIC)

@@ -9,12 +9,19 @@ reference's own layouts:
 
 * ``return_array=False``: per point, ``[success_flag] + abundances of out_species``
   (``_format_output``, ``model.py:71-76``);
-* ``return_array=True``: ``(physicsArray, chemicalAbunArray, None, abundanceStart,
+* ``return_array=True``: ``(physicsArray, chemicalAbunArray, ratesArray, abundanceStart,
   success_flags)`` with ``physicsArray[timepoints+1, points, 8]`` (Time, Density, gasTemp,
   dustTemp, Av, radfield, zeta, dstep; ``io.f90:74-82``) and
   ``chemicalAbunArray[timepoints+1, points, nspec]``, trimmed like ``_array_clean``
-  (``model.py:158-190``) to the last written row; rates arrays are not recorded (None, as
-  the reference returns without ``return_rates``).
+  (``model.py:158-190``) to the last written row; ``ratesArray[rows, points, nreac]`` (the
+  rate coefficients as each row is written, ``io.f90:90-93``) with ``return_rates=True``,
+  else None, as the reference returns it.
+
+A parameter dictionary may carry the reference's ``alpha``, ``beta`` and ``gamma`` keys
+(1-based reaction index -> coefficient, ``wrap.f90:744-761``): points with the same changes
+run together on a copy of the network with those coefficients
+(``makerates_hip.with_coefficients``).  The reference keeps such a change in its module
+arrays for later runs in the process; here it applies to the dictionary that carries it.
 
 ``get_rates`` / ``get_odes`` restate ``wrap.f90:446-547``: parse the dictionary, start the
 physics, take the given abundances, run the first update (rates at the given state, one
@@ -43,6 +50,35 @@ def network(name: str = "default") -> gpu.Network:
     return _nets[name]
 
 
+_COEFF_KEYS = ("alpha", "beta", "gamma")
+_derived: Dict[tuple, gpu.Network] = {}
+
+
+def _coefficient_network(name: str, changes: tuple) -> gpu.Network:
+    """The named network with reaction coefficients replaced (cached per change set)."""
+    if not changes:
+        return network(name)
+    key = (name, changes)
+    if key not in _derived:
+        ch = {k: dict(v) for k, v in changes}
+        t = mh.with_coefficients(network(name).t, **ch)
+        _derived[key] = gpu.Network(t)
+    return _derived[key]
+
+
+def _split_dict(d: Optional[Dict]):
+    """(parameter dict without coefficient keys, hashable coefficient change set)."""
+    rest, changes = {}, []
+    for k, v in dict(d or {}).items():
+        if str(k).lower() in _COEFF_KEYS:
+            items = tuple(sorted((int(i), float(x)) for i, x in dict(v).items()))
+            if items:
+                changes.append((str(k).lower(), items))
+        else:
+            rest[k] = v
+    return rest, tuple(sorted(changes))
+
+
 def _arrays(net: gpu.Network, params: np.ndarray, out, timepoints: int):
     n = params.shape[0]
     rows = int(out["n_rec"].max()) if n else 0
@@ -62,13 +98,20 @@ def _arrays(net: gpu.Network, params: np.ndarray, out, timepoints: int):
         chem[:k, q, :] = rec[:, 4:]
     last = np.maximum(out["n_rec"] - 1, 0)
     start = np.stack([chem[last[q], q, :] for q in range(n)]) if n else np.zeros((0, net.nspec))
-    return phys[:rows], chem[:rows], start
+    rates = None
+    if "rates" in out:
+        rates = np.zeros((rows, n, net.nreac))
+        for q in range(n):
+            k = int(out["n_rec"][q])
+            rates[:k, q, :] = out["rates"][q, :k]
+    return phys[:rows], chem[:rows], start, rates
 
 
 def _run(param_dicts: Iterable[Dict], out_species, return_array, starting_chemistry, timepoints, network_name,
-         lu_bits, extra: Dict):
+         lu_bits, extra: Dict, return_rates: bool = False):
     net = network(network_name)
-    params = np.stack([make_params(dict(d or {}), **extra) for d in param_dicts])
+    split = [_split_dict(d) for d in param_dicts]
+    params = np.stack([make_params(d, **extra) for d, _ in split])
     y0 = None
     if starting_chemistry is not None:
         sc = np.atleast_2d(np.asarray(starting_chemistry, dtype=np.float64))
@@ -77,11 +120,25 @@ def _run(param_dicts: Iterable[Dict], out_species, return_array, starting_chemis
         if sc.shape[0] == 1 and params.shape[0] > 1:
             sc = np.repeat(sc, params.shape[0], axis=0)
         y0 = np.concatenate([sc, params[:, INDEX["initialdens"]][:, None]], axis=1)
-    out = net.solve(params, y_init=y0, max_rec=timepoints + 1, lu_bits=lu_bits, order="densest")
+    want_rates = bool(return_rates and return_array)
+    groups: Dict[tuple, List[int]] = {}
+    for q, (_, ch) in enumerate(split):
+        groups.setdefault(ch, []).append(q)
+    out = None
+    for ch, members in groups.items():   # one launch per coefficient change set
+        m = np.array(members)
+        part = _coefficient_network(network_name, ch).solve(
+            params[m], y_init=None if y0 is None else y0[m], max_rec=timepoints + 1, lu_bits=lu_bits,
+            order="densest", return_rates=want_rates)
+        if out is None:
+            out = {k: (None if v is None else np.zeros((len(params),) + v.shape[1:], v.dtype)) for k, v in part.items()}
+        for k, v in part.items():
+            if v is not None:
+                out[k][m] = v
     flags = out["status"].astype(int)
     if return_array:
-        phys, chem, start = _arrays(net, params, out, timepoints)
-        return phys, chem, None, start, flags
+        phys, chem, start, rates = _arrays(net, params, out, timepoints)
+        return phys, chem, rates, start, flags
     names = net.t.species_names
     idx = [names.index(s) for s in (out_species or [])]
     res = []
@@ -95,19 +152,21 @@ def _run(param_dicts: Iterable[Dict], out_species, return_array, starting_chemis
 
 def cloud_grid(param_dicts: Sequence[Dict], out_species: Optional[List[str]] = None, return_array: bool = False,
                starting_chemistry=None, timepoints: int = TIMEPOINTS, network_name: str = "default",
-               lu_bits: int = 64):
+               lu_bits: int = 64, return_rates: bool = False):
     """``uclchem.model.cloud`` for every dict of `param_dicts` in one GPU launch."""
-    return _run(param_dicts, out_species, return_array, starting_chemistry, timepoints, network_name, lu_bits, {})
+    return _run(param_dicts, out_species, return_array, starting_chemistry, timepoints, network_name, lu_bits, {},
+                return_rates)
 
 
 def hot_core_grid(temp_indx: int, max_temperature: float, param_dicts: Sequence[Dict],
                   out_species: Optional[List[str]] = None, return_array: bool = False, starting_chemistry=None,
-                  timepoints: int = TIMEPOINTS, network_name: str = "default", lu_bits: int = 64):
+                  timepoints: int = TIMEPOINTS, network_name: str = "default", lu_bits: int = 64,
+                  return_rates: bool = False):
     """``uclchem.model.hot_core(temp_indx, max_temperature, ...)`` for every dict in one launch."""
     if not 1 <= int(temp_indx) <= 6:
         raise ValueError("temp_indx must be 1..6 (hotcore.f90:21-22)")
     return _run(param_dicts, out_species, return_array, starting_chemistry, timepoints, network_name, lu_bits,
-                dict(model=1, tempindx=int(temp_indx), maxtemp=float(max_temperature)))
+                dict(model=1, tempindx=int(temp_indx), maxtemp=float(max_temperature)), return_rates)
 
 
 def _initial_phys(params: np.ndarray) -> np.ndarray:

@@ -20,6 +20,10 @@ fixed once per network and the device only does arithmetic:
 * Sum rows.  ydot(BULK) and ydot(SURFACE) are the sums of the bulk and surface rows
   (odes.f90:4801,4814), so their rows of M are replaced exactly by
   row(BULK) - sum_b row(b) = e_BULK - sum_b e_b (right-hand side transformed alike).
+* Charge row.  With enforceChargeConservation (a per-point switch) ydot(e-) is the sum of
+  the positive-ion rows (chemistry.f90:333-335), so the electron row of M is replaced the
+  same way, by e_e - sum_ion e_ion; the pattern holds those entries beside the electron's
+  own chemistry row, and the point's switch picks one of the two when M is formed.
 * Ordering: minimum degree on the symmetrised pattern (ties by index), no pivoting.
 * Split: pivots [0, ns) are eliminated sparsely (Crout dot products, one thread per
   entry, grouped in dependency levels); the trailing T = N - ns block, nearly full after
@@ -81,6 +85,14 @@ def _augmented_pattern(t) -> tuple:
     for j in gs_cols:
         pat.add((S2, j))
     pat.add((S2, S1))
+    # the electron row's replacement under charge enforcement
+    NE = t.scalars["nelec"]
+    from .makerates_hip import ion_list
+    ions = [int(i) for i in ion_list(t)]
+    if ions and NE in (NB, NS):
+        raise ValueError("the electron cannot be a BULK/SURFACE sum species")
+    for i in ions:
+        pat.add((NE, i))
     # replaced sum rows
     pat = {(i, j) for (i, j) in pat if i not in (NB, NS)}
     pat.add((NB, NB))
@@ -90,7 +102,7 @@ def _augmented_pattern(t) -> tuple:
     for s in surf:
         pat.add((NS, s))
     return N, pat, dict(aff=aff, sw_rows=sw_rows, g1_cols=g1_cols, gs_cols=gs_cols, surf=surf, bulk=bulk,
-                        bps=bps, NB=NB, NS=NS, S1=S1, S2=S2)
+                        bps=bps, NB=NB, NS=NS, S1=S1, S2=S2, NE=NE, ions=ions)
 
 
 def minimum_degree(N: int, pat) -> np.ndarray:
@@ -263,6 +275,10 @@ def build_plan(t, split: int | None = None) -> Dict[str, np.ndarray]:
     misc = np.array([oslot(S1, S1), oslot(S2, S2), oslot(S2, S1), oslot(NB, NB), oslot(NS, NS)], dtype=np.int64)
     nbrow = np.array([oslot(NB, b) for b in info["bulk"]], dtype=np.int64)
     nsrow = np.array([oslot(NS, s) for s in info["surf"]], dtype=np.int64)
+    # charge row: every pattern slot of the electron row (cleared), then its ion entries
+    NE = info["NE"]
+    erow = np.array(sorted(oslot(NE, j) for (i, j) in pat if i == NE) if info["ions"] else [], dtype=np.int64)
+    eion = np.array([oslot(NE, i) for i in info["ions"]], dtype=np.int64)
     i32 = lambda x: np.ascontiguousarray(x, dtype=np.int32)
     lev_ptr = np.zeros(nlev + 1, dtype=np.int64)
     for l in range(nlev):
@@ -276,7 +292,8 @@ def build_plan(t, split: int | None = None) -> Dict[str, np.ndarray]:
         nw_pr_l=i32(pr_l), nw_pr_u=i32(pr_u),
         nw_jnz_slot=i32(jnz_slot), nw_diag_slot=i32(diag_slot), nw_ph_slot=i32(ph_slot), nw_tr_slot=i32(tr_slot),
         nw_s1c=i32(s1c), nw_s1r=i32(s1r), nw_s2c=i32(s2c), nw_s2r=i32(s2r), nw_gs_ptr=i32(gs_ptr),
-        nw_gs_slot=i32(gs_slot), nw_misc=i32(misc), nw_nbrow=i32(nbrow), nw_nsrow=i32(nsrow))
+        nw_gs_slot=i32(gs_slot), nw_misc=i32(misc), nw_nbrow=i32(nbrow), nw_nsrow=i32(nsrow),
+        nw_erow=i32(erow), nw_eion=i32(eion))
     return plan
 
 

@@ -13,7 +13,11 @@ odes.f90 written by uclchem_amd/reference_format.py):
   (Makerates/data/databases/weingartner01_grain_assisted_recombination.yaml values);
 * DESORB: a user desorption route, whose rate the reference leaves at zero.
 
-Every added reaction conserves atoms and charge.  Reactions are ordered by type as MakeRates
+Every added reaction conserves atoms and charge.  The network has no excited species, so the
+excited-state laws act on ground-state ices: EXRELAX (rate = the reactant's diffusion
+frequency, ~1e12 s^-1, rates.f90:164-175) breaks two trace radicals up, and EXSOLID carries a
+small branching ratio (alpha) so that its diffusion-limited rate does not consume the major
+ices at once; the rate laws themselves are checked for any values.  Reactions are ordered by type as MakeRates
 orders them (alphabetically, TWOBODY last), so the reference's range loops see each type as
 one contiguous range.
 
@@ -49,10 +53,10 @@ EXTRA = [
     (("C+", "NH3", N), ("HCN+", "H2", N, N), 0.5, 2.3e-9, 1.47),
     (("#H2O", "CRS", N), ("#OH", "#H", N, N), 1.0, 1.747, 1.287e-15),    # CRS
     (("#CO", "CRS", N), ("#C", "#O", N, N), 1.0, 1.269, 1.287e-15),
-    (("#OH", "EXRELAX", N), ("#OH", N, N, N), 1.0, 0.0, 0.0),            # EXRELAX
-    (("#CO", "EXRELAX", N), ("#CO", N, N, N), 1.0, 0.0, 0.0),
-    (("#O", "#CO", "EXSOLID"), ("#CO2", N, N, N), 1.0, 0.0, 0.0),        # EXSOLID
-    (("#H", "#OH", "EXSOLID"), ("#H2O", N, N, N), 0.5, 0.0, 0.0),
+    (("#HCO", "EXRELAX", N), ("#H", "#CO", N, N), 1.0, 0.0, 0.0),        # EXRELAX
+    (("#HNO", "EXRELAX", N), ("#H", "#NO", N, N), 1.0, 0.0, 0.0),
+    (("#O", "#CO", "EXSOLID"), ("#CO2", N, N, N), 1e-15, 0.0, 0.0),      # EXSOLID
+    (("#H", "#OH", "EXSOLID"), ("#H2O", N, N, N), 5e-16, 0.0, 0.0),
     (("C+", "GAR", N), ("C", N, N, N), 1.0, 0.0, 0.0),                   # GAR
     (("MG+", "GAR", N), ("MG", N, N, N), 1.0, 0.0, 0.0),
     (("SI+", "GAR", N), ("SI", N, N, N), 1.0, 0.0, 0.0),

@@ -23,11 +23,14 @@ walked state with its temperature raised to 20, 50, 100 and 300 K ("jump" rows: 
 desorption conditions of the warm-up, one interval each, from an icy state the walk reached).
 
 On the first-attempt stretches the reference's DVODE spends its whole step budget, and its
-answer for the smallest abundances depends on its tolerance.  For those rows y_tight holds
-the reference run over the same stretch at 1000 times tighter relative and absolute
-tolerances (reltol and abstol_factor / 1000, no step bound), so that a test can tell the
-reference's own integration error from a difference of method (other rows: NaN).
-    python tests/golden/make_interval_fixtures.py --tight-only   # add y_tight to the fixture
+answer for the smallest abundances depends on its tolerance.  For those rows the fixture also
+holds y_tight, the stretch's solution at tighter tolerances, so that a test can tell the
+reference's own integration error from a difference of method (other rows: NaN): the
+reference itself at 100 times tighter reltol and abstol_factor where it finishes within the
+per-row cap (tight_src 1), otherwise the BDF core of tests/bdf_host.cpp on the reference's own
+F (ucref_rhs, the stretch's rates) with a finite-difference Jacobian and pivoted dense LU at
+1000 times tighter tolerances (tight_src 2).
+    python tests/golden/make_interval_fixtures.py --tight-only [--cap 1800]   # add y_tight
 """
 from __future__ import annotations
 
@@ -119,46 +122,101 @@ def select(rows, n_hard=3, n_first=2):
     return out
 
 
-def tight(job):
-    """The reference over one first-attempt stretch at 1000x tighter tolerances."""
-    p, y, phys, target = job
-    from oracle import RefLib
+def _host_core():
+    """The host build of the BDF core (tests/bdf_host.cpp), compiled to a temporary file."""
+    import ctypes
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(), "libbdfhost.so")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", out,
+                           os.path.join(os.path.dirname(HERE), "bdf_host.cpp")])
+    lib = ctypes.CDLL(out)
+    lib.bdf_host_solve_fn.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    return lib
+
+
+def tight_ref(job):
+    """The reference over one first-attempt stretch at 100x tighter tolerances, in a child
+    process with a wall-time cap; None when it is exceeded or the stretch is not reached."""
+    import subprocess
+    import tempfile
+    p, y, phys, target, cap = job
     p = p.copy()
     ph = phys.copy()
-    p[INDEX["reltol"]] *= 1e-3
+    p[INDEX["reltol"]] *= 1e-2
     p[INDEX["mxstep"]] = 1e7
-    ph[6] *= 1e-3
-    rc, y1, ph1, st = RefLib().interval(p, y, ph, target)
-    ok = rc == 0 and ph1[7] >= target * (1 - 1e-12)
-    print(f"tight: {st[0]} steps, rc {rc}, reached {ok}", flush=True)
-    return y1 if ok else np.full_like(y1, np.nan)
+    ph[6] *= 1e-2
+    with tempfile.NamedTemporaryFile(suffix=".npz") as tmp:
+        np.savez(tmp.name + ".in.npz", p=p, y=y, ph=ph, t=target)
+        code = ("import sys,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
+                "z=np.load(sys.argv[1]); rc,y1,ph1,st=RefLib().interval(z['p'],z['y'],z['ph'],float(z['t'])); "
+                "np.savez(sys.argv[2],rc=rc,y1=y1,ph1=ph1,st=st)") % os.path.join(ROOT, "oracle")
+        try:
+            subprocess.run([sys.executable, "-c", code, tmp.name + ".in.npz", tmp.name], timeout=cap, check=True)
+        except subprocess.TimeoutExpired:
+            return None
+        finally:
+            os.remove(tmp.name + ".in.npz")
+        z = np.load(tmp.name)
+        ok = int(z["rc"]) == 0 and z["ph1"][7] >= target * (1 - 1e-12)
+        print(f"tight reference: {int(z['st'][0])} steps, rc {int(z['rc'])}, reached {ok}", flush=True)
+        return z["y1"].copy() if ok else None
 
 
-def add_tight(procs):
+def tight_host(p, y, phys, target):
+    """The BDF core on the reference's F over the stretch at 1000x tighter tolerances."""
+    import ctypes
+    from oracle import RefLib
+    lib = _host_core()
+    ref = RefLib()
+    ref.interval(p, y, phys, target, maxsteps=1.0)     # leaves the stretch's rates in the reference
+    rhs = ctypes.cast(ref.lib.ucref_rhs, ctypes.c_void_p)
+    atol = np.maximum(1e-3 * phys[6] * y, 1e-3 * p[INDEX["abstol_min"]])
+    yo = y.copy()
+    st = np.zeros(7, dtype=np.int32)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    r = lib.bdf_host_solve_fn(len(yo), rhs, phys[7], target, 1e-3 * p[INDEX["reltol"]], ptr(atol), ptr(yo), 2000000,
+                              ptr(st))
+    print(f"tight host core: {st[0]} steps, return {r}", flush=True)
+    return yo if r == 2 else None
+
+
+def add_tight(procs, cap):
     from uclchem_amd.params import widen
     path = os.path.join(HERE, "ref_intervals.npz")
     f = dict(np.load(path))
     rows = [q for q in range(len(f["target"])) if f["how"][q] == "first-attempt"]
     params = widen(f["params"])
-    jobs = [(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q])) for q in rows]
+    jobs = [(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q]), cap) for q in rows]
     with mp.get_context("fork").Pool(procs) as pool:
-        res = pool.map(tight, jobs, chunksize=1)
+        res = pool.map(tight_ref, jobs, chunksize=1)
     yt = np.full_like(f["y_ref"], np.nan)
+    src = np.zeros(len(f["target"]), dtype=np.int32)
     for q, y1 in zip(rows, res):
+        if y1 is None:
+            y1 = tight_host(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q]))
+            if y1 is None:
+                continue
+            src[q] = 2
+        else:
+            src[q] = 1
         yt[q] = y1
     f["y_tight"] = yt
+    f["tight_src"] = src
     np.savez_compressed(path, **f)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--tight-only", action="store_true", help="add y_tight to the existing fixture")
+    ap.add_argument("--cap", type=float, default=1800.0, help="wall seconds per tight reference stretch")
     ap.add_argument("--procs", type=int, default=6)
     ap.add_argument("--budget", type=float, default=2400.0, help="wall seconds per hot-core trajectory walk")
     ap.add_argument("--tail-budget", type=float, default=900.0, help="wall seconds per tail-point walk")
     a = ap.parse_args()
     if a.tight_only:
-        add_tight(a.procs)
+        add_tight(a.procs, a.cap)
         return
     jobs = [("hot", i, a.budget) for i in hot_points()] + [("tail", i, a.tail_budget) for i in TAIL_POINTS]
     with mp.get_context("fork").Pool(a.procs) as pool:

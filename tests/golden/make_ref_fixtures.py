@@ -95,19 +95,22 @@ def example_models():
 
 
 def _model_capped(args):
-    """One reference model in a child process with a wall-time cap; None when it is exceeded."""
+    """One reference model in a child process with a wall-time cap; None when it is exceeded.
+    args = (params, cap) or (params, cap, network name of an oracle/_ref/net_<name> build)."""
     import subprocess
     import tempfile
-    p, cap = args
+    p, cap = args[:2]
+    lib = _lib_path(args[2]) if len(args) > 2 else None
     with tempfile.NamedTemporaryFile(suffix=".npz") as tmp:
         code = ("import sys,time,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
                 "p=np.array([float(x) for x in sys.argv[1].split(',')]); t=time.time(); "
-                "rc,rows,st=RefLib().model(p,maxrec=2000); "
+                "ref=RefLib(sys.argv[3]) if len(sys.argv)>3 else RefLib(); "
+                "rc,rows,st=ref.model(p,maxrec=2000); "
                 "np.savez(sys.argv[2],rc=rc,final=rows[-1],nrows=len(rows),stats=st,wall=time.time()-t)"
                 ) % os.path.join(ROOT, "oracle")
         try:
-            subprocess.run([sys.executable, "-c", code, ",".join(repr(float(x)) for x in p), tmp.name],
-                           timeout=cap, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            subprocess.run([sys.executable, "-c", code, ",".join(repr(float(x)) for x in p), tmp.name] +
+                           ([lib] if lib else []), timeout=cap, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
         except subprocess.TimeoutExpired:
             return None
         z = np.load(tmp.name)
@@ -141,15 +144,15 @@ def grid4096_sample(n, procs, cap):
         np.stack([r[3] for r in res]), np.array([r[4] for r in res]), np.array([r[2] for r in res])
 
 
-def grid_tight(procs, cap):
-    """Add to ref_cloud_grid.npz the reference's answer at tight tolerances (reltol 1e-11,
-    abstol_factor 1e-17, abstol_min 1e-30) for each of its points: final_tight (NaN where the
-    run exceeds `cap` seconds).  |final - final_tight| is the reference's own integration
-    noise on that point: on the grid's stiff tail it includes the different places where
-    its integrator runs out of steps and the rates are recomputed mid-interval
-    (chemistry.f90:164-221)."""
+def grid_tight(procs, cap, name="ref_cloud_grid.npz", net=None):
+    """Add to a whole-model fixture (ref_cloud_grid.npz, or ref_alltypes.npz with net
+    "alltypes") the reference's answer at tight tolerances (reltol 1e-11, abstol_factor 1e-17,
+    abstol_min 1e-30) for each of its points: final_tight (NaN where the run exceeds `cap`
+    seconds).  |final - final_tight| is the reference's own integration noise on that point:
+    on the grid's stiff tail it includes the different places where its integrator runs out
+    of steps and the rates are recomputed mid-interval (chemistry.f90:164-221)."""
     from uclchem_amd.params import INDEX, widen
-    path = os.path.join(HERE, "ref_cloud_grid.npz")
+    path = os.path.join(HERE, name)
     f = dict(np.load(path))
     params = widen(f["params"])
     tight = params.copy()
@@ -157,15 +160,15 @@ def grid_tight(procs, cap):
     tight[:, INDEX["abstol_factor"]] = 1e-17
     tight[:, INDEX["abstol_min"]] = 1e-30
     with mp.get_context("fork").Pool(procs) as pool:
-        res = pool.map(_model_capped, [(p, cap) for p in tight], chunksize=1)
+        res = pool.map(_model_capped, [(p, cap) if net is None else (p, cap, net) for p in tight], chunksize=1)
     ft = np.full_like(f["final"], np.nan)
     wall = np.full(len(params), np.nan)
     for q, r in enumerate(res):
         if r is not None and r[0] == 0 and r[2] == f["nrows"][q]:
             ft[q] = r[1]
             wall[q] = r[4]
-        print(f"grid point {f['index'][q]}: tight {'capped' if r is None else (r[0], r[2], round(r[4], 1))}",
-              flush=True)
+        print(f"point {f['index'][q] if 'index' in f else q}: tight "
+              f"{'capped' if r is None else (r[0], r[2], round(r[4], 1))}", flush=True)
     f["final_tight"] = ft
     f["tight_wall"] = wall
     np.savez_compressed(path, **f)
@@ -291,10 +294,14 @@ def main():
     ap.add_argument("--alltypes-only", action="store_true", help="only the all-reaction-types network points")
     ap.add_argument("--grid-tight", action="store_true", help="add tight-tolerance answers to ref_cloud_grid.npz")
     ap.add_argument("--coeff-only", action="store_true", help="only the coefficient-change / rates-output model")
+    ap.add_argument("--alltypes-tight", action="store_true", help="add tight-tolerance answers to ref_alltypes.npz")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     a = ap.parse_args()
     if a.grid_tight:
         grid_tight(a.procs, a.cap)
+        return
+    if a.alltypes_tight:
+        grid_tight(a.procs, a.cap, "ref_alltypes.npz", "alltypes")
         return
     if a.coeff_only:
         coeff_fixture()

@@ -89,3 +89,15 @@ def conserved_elements(tables):
         ok &= bal == 0
     ok[ELEMENTS.index("E-")] = False
     return np.where(ok)[0]
+
+
+def excess_over_reference(y, y_ref, y_tight=None, floor=1e-14):
+    """Largest |y - y_ref| / allowance over species with y_ref > floor, where the allowance is
+    1% of y_ref plus, when the reference's tight-tolerance answer y_tight is given (finite),
+    twice the reference's own integration noise |y_ref - y_tight|.  <= 1 passes."""
+    import numpy as np
+    m = y_ref > floor
+    allow = 0.01 * y_ref[m]
+    if y_tight is not None and np.all(np.isfinite(y_tight)):
+        allow = allow + 2.0 * np.abs(y_ref[m] - y_tight[m])
+    return float((np.abs(y[m] - y_ref[m]) / allow).max())

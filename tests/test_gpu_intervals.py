@@ -9,9 +9,11 @@ reference needs up to ~1e5 steps.  Gates: the interval's end is reached, every a
 1e-14 within 1% of the reference, element totals conserved to 1e-10 over the interval.
 
 On the "first-attempt" rows (stretches where the reference's DVODE spends its whole step budget)
-the reference's smallest abundances move by tens of percent when its own tolerances are
-tightened 1000-fold (y_tight, the reference rerun that way); there an abundance passes within
-1% of the reference plus twice the reference's own change, |y_ref - y_tight|.
+the reference's smallest abundances move by tens of percent when the stretch is integrated at
+tighter tolerances (y_tight: the reference itself at 100x tighter tolerances, or, where that
+does not finish, the host BDF core on the reference's own F at 1000x tighter; tight_src 1 or
+2); there an abundance passes within 1% of the reference plus twice the reference's own
+integration error, |y_ref - y_tight|.
 """
 import os
 

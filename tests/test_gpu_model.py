@@ -88,7 +88,9 @@ def test_coefficient_changes_and_rates_output_match_reference():
     same way (tests/golden/ref_coeff.npz): the static cloud to 1e5 yr, every abundance above
     1e-14 within 1%, and the rate array at five output rows within 1e-3 relative (the
     coefficients follow the state each interval starts from, which the two integrators reach
-    within their tolerances)."""
+    within their tolerances).  The H2 and CO photodissociation rates of a row are those of the
+    integrator's last RHS evaluation (chemistry.f90:311-319), at a state inside its last step,
+    which differs between the two integrators: within 10%."""
     import os
     from uclchem_amd import model
     f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_coeff.npz"))
@@ -106,8 +108,12 @@ def test_coefficient_changes_and_rates_output_match_reference():
     assert plain[2] is None
     got = rates[f["rate_rows"], 0, :]
     want = f["rates"]
+    photo = [model.network().t.scalars[k] for k in ("nR_H2_hv", "nR_CO_hv")]
     nz = np.abs(want) > 1e-30
-    assert np.all(got[~nz] == 0.0) or np.abs(got[~nz]).max() < 1e-30
-    rel = np.abs(got[nz] - want[nz]) / np.abs(want[nz])
-    assert rel.max() < 1e-3, rel.max()
+    assert not np.any(np.abs(got[~nz]) >= 1e-30)
+    rel = np.where(nz, np.abs(got - want) / np.where(nz, np.abs(want), 1.0), 0.0)
+    other = np.ones(rel.shape[1], dtype=bool)
+    other[photo] = False
+    assert rel[:, other].max() < 1e-3, (rel[:, other].max(), np.unravel_index(np.argmax(rel[:, other]), rel[:, other].shape))
+    assert rel[:, photo].max() < 0.1, rel[:, photo]
     assert np.all(rates[0, 0] == 0.0)

@@ -59,21 +59,31 @@ def _ref_grid(name):
 def test_grid_matches_reference(gpu_net, tables, lu_bits):
     """Points of the benchmarked 4096-point static-cloud grid (one per initial-density
     stratum, tests/golden/ref_cloud_grid.npz) against the reference Fortran DVODE: the same
-    47 output rows to 1 Myr and every abundance above 1e-14 within 1%.  The grid's
-    pathologically stiff tail points are checked interval by interval (test_gpu_intervals)."""
+    47 output rows to 1 Myr and every abundance above 1e-14 within 1% plus twice the
+    reference's own integration noise, |final - final_tight| (final_tight: the reference at
+    reltol 1e-11, abstol_factor 1e-17, abstol_min 1e-30).  The noise term matters on the
+    grid's stiff tail, where the reference's integrator exhausts MXSTEP and recomputes the
+    rates mid-interval at places that depend on its step count (chemistry.f90:164-221); those
+    stretches are pinned call by call in test_gpu_intervals."""
     ref = _ref_grid("ref_cloud_grid.npz")
+    if "final_tight" not in ref.files:
+        pytest.skip("ref_cloud_grid.npz has no tight-tolerance answers (make_ref_fixtures.py --grid-tight)")
     from uclchem_amd import workloads
     from uclchem_amd.params import widen
     params = widen(ref["params"])
     assert np.array_equal(params, workloads.static_cloud_grid(4096)[ref["index"]])
     out = gpu_net.solve(params, lu_bits=lu_bits, max_rec=80, order="densest")
-    bad = []
+    from helpers import excess_over_reference
+    bad, noisy = [], 0
     for q in range(len(params)):
         assert ref["rc"][q] == 0 and ref["nrows"][q] == 47, (q, ref["rc"][q], ref["nrows"][q])
         assert out["status"][q] == 0 and out["n_rec"][q] == 47, (q, out["status"][q], out["n_rec"][q])
-        rel = _rel(out["y"][q, :tables.nspec], ref["final"][q, 4:])
-        if rel.max() > 0.01:
-            bad.append((int(ref["index"][q]), float(rel.max())))
+        y_ref, tight = ref["final"][q, 4:], ref["final_tight"][q, 4:]
+        noisy += int(excess_over_reference(tight, y_ref) > 1.0) if np.all(np.isfinite(tight)) else 0
+        e = excess_over_reference(out["y"][q, :tables.nspec], y_ref, tight)
+        if e > 1.0:
+            bad.append((int(ref["index"][q]), e))
+    print(f"[grid] {noisy} of {len(params)} points where the reference's own noise exceeds 1%", flush=True)
     assert not bad, bad
 
 

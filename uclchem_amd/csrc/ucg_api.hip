@@ -499,6 +499,8 @@ static int check_plan(const ucg_network_desc *d)
         nph < 1 || nphs < 0 || ngs < 0)
         return bad("inconsistent sizes");
     if (T > 0 && NN - ns > ucg::NT * ucg::EPT) return bad("tail too large");
+    // the tail's Gauss-Jordan keeps its 8x8 pivot-block inverse in the right-hand-side vector
+    if (T > 0 && ((d->neq + 7) & ~7) < ucg::GJ_B * ucg::GJ_B) return bad("network too small for the dense tail");
     auto in = [](const int *a, long n, long lo, long hi) {   // every a[i] in [lo, hi)
         if (n > 0 && !a) return false;
         for (long i = 0; i < n; i++)
@@ -734,6 +736,28 @@ static int load_tables(const ucg_network_desc *d, NetHandle &h)
         if ((rc = upload(h, facp.data(), facp.size(), N.facp))) return rc;
         if ((rc = upload(h, foff.data(), foff.size(), N.fac_off))) return rc;
         if ((rc = upload(h, jpk.data(), jpk.size(), N.jc_pk))) return rc;
+    }
+    {   // Jacobian entries in two balanced lists: short contribution lists one per lane, longest
+        // first and dealt to the lanes in alternating directions (so each lane's total is near
+        // the mean), long ones (the density column of the hydrogen-bearing species: hundreds of
+        // terms) by groups of JZ_GROUP lanes
+        std::vector<int> sh, lo;
+        for (int n = 0; n < d->nnz; n++)
+            (d->jnz_ptr[n + 1] - d->jnz_ptr[n] > ucg::JZ_LONG ? lo : sh).push_back(n);
+        std::stable_sort(sh.begin(), sh.end(), [&](int a, int b) {
+            return d->jnz_ptr[a + 1] - d->jnz_ptr[a] > d->jnz_ptr[b + 1] - d->jnz_ptr[b];
+        });
+        std::vector<int> dealt(sh.size());
+        const size_t nt = ucg::NT, full = sh.size() / nt * nt;
+        for (size_t p = 0; p < sh.size(); p++) {
+            const size_t blk = p / nt, r = p % nt;
+            const size_t lane = (p < full && (blk & 1)) ? nt - 1 - r : r;
+            dealt[blk * nt + lane] = sh[p];
+        }
+        N.nj_short = (int)dealt.size();
+        N.nj_long = (int)lo.size();
+        if ((rc = upload(h, dealt.data(), dealt.size(), N.jz_short))) return rc;
+        if ((rc = upload(h, lo.data(), lo.size(), N.jz_long))) return rc;
     }
     {   // production/loss lists index fluxes: checked here, the device trusts them
         for (int kind = 0; kind < 2; kind++) {

@@ -39,6 +39,8 @@ __device__ __forceinline__ int4 ld_int4(const GLB int *p, int i)
 
 constexpr int NT = 256;          // threads per workgroup (one workgroup solves one grid point)
 constexpr int EPT = 2;           // state elements owned per thread => NEQ <= 512
+constexpr int JZ_LONG = 16;      // Jacobian entries with more contributions than this...
+constexpr int JZ_GROUP = 16;     // ...are summed by a group of this many lanes
 constexpr int MAXOPS = 6;
 constexpr int MAXFAC = 10;
 constexpr int NTYPES = 23;
@@ -95,6 +97,11 @@ struct DevNet {
     const int32_t *surface_list, *bulk_list, *bulk_partner_of_surface, *surface_partner_of_bulk;
     const int32_t *freeze_partners, *is_surface, *is_bulk;          // is_*: [neq]
     const int32_t *jnz_row, *jnz_col, *jnz_ptr, *jc_k, *jc_m, *jc_c;
+    // Jacobian entries in evaluation order: nj_short with at most JZ_LONG contributions (one
+    // lane each, dealt so that every lane gets a near-equal share), nj_long with more (one
+    // group of JZ_GROUP lanes each)
+    int32_t nj_short, nj_long;
+    const int32_t *jz_short, *jz_long;
     const int32_t *aff_ptr, *aff_sp, *aff_cf, *sw_ptr, *sw_k, *sw_c;
     // special species (0-based; missing -> nspec, the density slot)
     int32_t nh, nh2, nhe, nc, ncx, nn, no, nf, np, nmg, nmgx, nna, nli, npah, nsx, nsix, nclx, nd,

@@ -578,6 +578,34 @@ __device__ __forceinline__ double jac_entry(const DevNet &N, const LDS double *P
     return acc;
 }
 
+// Every generated entry of dJ, handed to store(n, value): the short contribution lists one per
+// lane in the dealt order, the long ones split over JZ_GROUP lanes (strided partial sums, then a
+// shuffle tree inside the group; all lanes of a group take the same trip count, so the shuffles
+// read active lanes only).  The longest list (hundreds of terms) no longer serialises one lane.
+template <typename Store>
+__device__ __forceinline__ void jac_entries(const DevNet &N, const LDS double *P, Store store)
+{
+    const GLB int *js = glb(N.jz_short);
+    for (int q = tid(); q < N.nj_short; q += NT) {
+        const int n = js[q];
+        store(n, jac_entry(N, P, n));
+    }
+    const GLB int *jl = glb(N.jz_long), *jp = glb(N.jnz_ptr), *pk_ = glb(N.jc_pk);
+    const int grp = tid() / JZ_GROUP, ln = tid() % JZ_GROUP;
+    for (int q = grp; q < N.nj_long; q += NT / JZ_GROUP) {
+        const int n = jl[q];
+        double acc = 0.0;
+        const int q1 = jp[n + 1];
+        for (int c = jp[n] + ln; c < q1; c += JZ_GROUP) {
+            const int pk = pk_[c];
+            acc += (double)((pk & 15) - 8) * P[pk >> 4];
+        }
+#pragma unroll
+        for (int off = JZ_GROUP / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (ln == 0) store(n, acc);
+    }
+}
+
 // Newton matrix M = I - gamma J(y) in column-major storage, J from the generated
 // structure plus the structured couplings (photo rates, totalSwap, ice transfer and the
 // BULK/SURFACE totals).  With scale != 0 the matrix is equilibrated by the error weights:
@@ -609,7 +637,7 @@ UCG_BIG __device__ void block_newton_matrix(const DevNet &N, const Lds L, LUT *M
     block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 2. generated sparse part
-    for (int n = tid(); n < N.nnz; n += NT) M[(size_t)N.jnz_col[n] * ld + N.jnz_row[n]] = (LUT)jac_entry(N, P, n);
+    jac_entries(N, P, [&](int n, double v) { M[(size_t)N.jnz_col[n] * ld + N.jnz_row[n]] = (LUT)v; });
     __syncthreads();
     // 3a. totalSwap dependence of SURFSWAP terms: J[i][b] += u_i * dTS/dy_b
     for (int i = tid(); i < neq; i += NT) {
@@ -1027,57 +1055,42 @@ UCG_BIG __device__ int block_lu_tiled(MP M, int n, int ld, LDS double *red, LDS 
 }
 
 // In-place inverse of the dense n x n block M (column-major, leading dimension ld) by
-// blocked Gauss-Jordan elimination without pivoting, four pivots per sweep: the 4x4 pivot
-// block is inverted in thread 0's registers, its row panel is multiplied by that inverse
-// (one column per thread), every entry outside the pivot rows and columns takes the
-// rank-4 update in 4x4 register tiles, then the pivot column panel becomes -A_IK inv(A_KK).  The
-// pivots are those of the unpivoted LU and are checked the same way; it is used for the
-// trailing block of the sparse plan so that each Newton solve applies the block as one
-// matrix-vector product instead of two dependent triangular sweeps.  stage: >= 16
+// blocked Gauss-Jordan elimination without pivoting, GJ_B = 8 pivots per sweep: the 8x8 pivot
+// block is inverted by wave 0, one element per lane (eight elimination steps whose pivot row
+// and column travel by shuffles), its row panel is multiplied by that inverse (one column per
+// thread), every entry outside the pivot rows and columns takes the rank-8 update in 4x4
+// register tiles, then the pivot column panel becomes -A_IK inv(A_KK).  Eight pivots per sweep
+// halve the sweeps and barriers of a four-pivot sweep and load each 4x4 tile once per eight
+// rank-one terms.  The pivots are those of the unpivoted LU and are checked the same way; it
+// is used for the trailing block of the sparse plan so that each Newton solve applies the
+// block as one matrix-vector product instead of two dependent triangular sweeps.  stage: >= 64
 // doubles of LDS.  Returns nonzero (uniformly) on a zero or non-finite pivot.
+constexpr int GJ_B = 8;
 template <typename LUT, typename MP>
 __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int ld, LDS double *red, LDS double *stage)
 {
-    constexpr int B = 4;
-    LDS double *P = stage;   // P[l*4+m] = inv(A_KK)(l, m)
+    constexpr int B = GJ_B;
+    LDS double *P = stage;   // P[l*B+m] = inv(A_KK)(l, m)
     for (int kb = 0; kb < n; kb += B) {
         const int nb = min(B, n - kb);
         __syncthreads();
-        if (tid() == 0) {
-            double a[B][B];
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++)
-                    a[r][c] = (r < nb && c < nb) ? (double)M[(size_t)(kb + c) * ld + kb + r] : (r == c ? 1.0 : 0.0);
-            double bad = 0.0;
+        if (tid() < 64) {   // wave 0: lane r*B + c holds element (r, c) of the pivot block
+            const int r = tid() / B, c = tid() % B;
+            double a = (r < nb && c < nb) ? (double)M[(size_t)(kb + c) * ld + kb + r] : (r == c ? 1.0 : 0.0);
+            bool bad = false;
 #pragma unroll
             for (int k = 0; k < B; k++) {
-                const double piv = a[k][k];
-                if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = 1.0;
+                const double piv = __shfl(a, k * B + k);
+                const double rowk = __shfl(a, k * B + c);   // a(k, c)
+                const double colk = __shfl(a, r * B + k);   // a(r, k)
+                if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = true;
                 const double q = 1.0 / piv;
-#pragma unroll
-                for (int c = 0; c < B; c++)
-                    if (c != k) a[k][c] *= q;
-                a[k][k] = q;
-#pragma unroll
-                for (int r = 0; r < B; r++) {
-                    if (r == k) continue;
-                    const double f = a[r][k];
-#pragma unroll
-                    for (int c = 0; c < B; c++)
-                        if (c != k) a[r][c] -= f * a[k][c];
-                    a[r][k] = -f * q;
-                }
+                if (r == k) a = (c == k) ? q : a * q;
+                else a = (c == k) ? -colk * q : a - colk * (rowk * q);
             }
-#pragma unroll
-            for (int r = 0; r < B; r++)
-#pragma unroll
-                for (int c = 0; c < B; c++) {
-                    P[r * B + c] = a[r][c];
-                    if (r < nb && c < nb) M[(size_t)(kb + c) * ld + kb + r] = (LUT)a[r][c];
-                }
-            red[8] = bad;
+            P[r * B + c] = a;
+            if (r < nb && c < nb) M[(size_t)(kb + c) * ld + kb + r] = (LUT)a;
+            if (tid() == 0) red[8] = bad ? 1.0 : 0.0;
         }
         __syncthreads();
         if (red[8] != 0.0) return 1;
@@ -1095,11 +1108,11 @@ __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int 
             }
         }
         __syncthreads();
-        const int nt = (n + 3) >> 2, kt = kb >> 2;
+        const int nt = (n + 3) >> 2;
         for (int tt = tid(); tt < nt * nt; tt += NT) {   // A_IJ -= A_IK A_KJ outside the pivot block
             const int ti = tt % nt, tj = tt / nt;
-            if (ti == kt || tj == kt) continue;
             const int i0 = 4 * ti, j0 = 4 * tj;
+            if ((i0 < kb + nb && i0 + 4 > kb) || (j0 < kb + nb && j0 + 4 > kb)) continue;
             double acc[4][4];
 #pragma unroll
             for (int c = 0; c < 4; c++)
@@ -1290,7 +1303,11 @@ UCG_BIG __device__ void sparse_jacobian(const DevNet &N, const Lds L, double *V)
     block_term_partials(N, rate, y, P, D, blr, ism, tsm, dblr, dism, dtsm);
     __syncthreads();
     // 1. generated entries of J0, stored as -dJ; u1 (totalSwap column factor)
-    for (int n = tid(); n < N.nnz; n += NT) glbw(V)[glb(N.nw_jnz_slot)[n]] = (-jac_entry(N, P, n));
+    {
+        GLB double *Vw = glbw(V);
+        const GLB int *slot = glb(N.nw_jnz_slot);
+        jac_entries(N, P, [&](int n, double v) { Vw[slot[n]] = -v; });
+    }
     for (int i = tid(); i < neq; i += NT) {
         double u = 0.0;
         for (int q = N.sw_ptr[i]; q < N.sw_ptr[i + 1]; q++) {

@@ -29,8 +29,8 @@ reference's own integration error from a difference of method (other rows: NaN):
 reference itself at 100 times tighter reltol and abstol_factor where it finishes within the
 per-row cap (tight_src 1), otherwise the BDF core of tests/bdf_host.cpp on the reference's own
 F (ucref_rhs, the stretch's rates) with a finite-difference Jacobian and pivoted dense LU at
-1000 times tighter tolerances (tight_src 2).
-    python tests/golden/make_interval_fixtures.py --tight-only [--cap 1800]   # add y_tight
+the same tighter tolerances (tight_src 2).
+    python tests/golden/make_interval_fixtures.py --tight-only [--cap 1800] [--only 3,4]
 """
 from __future__ import annotations
 
@@ -165,58 +165,63 @@ def tight_ref(job):
 
 
 def tight_host(p, y, phys, target):
-    """The BDF core on the reference's F over the stretch at 1000x tighter tolerances."""
+    """The BDF core on the reference's F over the stretch at 100x tighter tolerances."""
     import ctypes
     from oracle import RefLib
     lib = _host_core()
     ref = RefLib()
     ref.interval(p, y, phys, target, maxsteps=1.0)     # leaves the stretch's rates in the reference
     rhs = ctypes.cast(ref.lib.ucref_rhs, ctypes.c_void_p)
-    atol = np.maximum(1e-3 * phys[6] * y, 1e-3 * p[INDEX["abstol_min"]])
+    atol = np.maximum(1e-2 * phys[6] * y, 1e-2 * p[INDEX["abstol_min"]])
     yo = y.copy()
     st = np.zeros(7, dtype=np.int32)
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    r = lib.bdf_host_solve_fn(len(yo), rhs, phys[7], target, 1e-3 * p[INDEX["reltol"]], ptr(atol), ptr(yo), 2000000,
+    r = lib.bdf_host_solve_fn(len(yo), rhs, phys[7], target, 1e-2 * p[INDEX["reltol"]], ptr(atol), ptr(yo), 500000,
                               ptr(st))
     print(f"tight host core: {st[0]} steps, return {r}", flush=True)
     return yo if r == 2 else None
 
 
-def add_tight(procs, cap):
+def _tight_row(job):
+    q, p, y, phys, target, cap = job
+    y1 = tight_ref((p, y, phys, target, cap))
+    if y1 is not None:
+        return q, y1, 1
+    y1 = tight_host(p, y, phys, target)
+    return q, y1, (2 if y1 is not None else 0)
+
+
+def add_tight(procs, cap, only=None):
+    """Adds y_tight / tight_src row by row, writing the fixture after every finished row (a
+    rerun with `only` fills the rows still missing)."""
     from uclchem_amd.params import widen
     path = os.path.join(HERE, "ref_intervals.npz")
     f = dict(np.load(path))
-    rows = [q for q in range(len(f["target"])) if f["how"][q] == "first-attempt"]
+    if "y_tight" not in f:
+        f["y_tight"] = np.full_like(f["y_ref"], np.nan)
+        f["tight_src"] = np.zeros(len(f["target"]), dtype=np.int32)
+    rows = [q for q in range(len(f["target"])) if f["how"][q] == "first-attempt" and (only is None or q in only)]
     params = widen(f["params"])
-    jobs = [(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q]), cap) for q in rows]
+    jobs = [(q, params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q]), cap) for q in rows]
     with mp.get_context("fork").Pool(procs) as pool:
-        res = pool.map(tight_ref, jobs, chunksize=1)
-    yt = np.full_like(f["y_ref"], np.nan)
-    src = np.zeros(len(f["target"]), dtype=np.int32)
-    for q, y1 in zip(rows, res):
-        if y1 is None:
-            y1 = tight_host(params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q]))
-            if y1 is None:
-                continue
-            src[q] = 2
-        else:
-            src[q] = 1
-        yt[q] = y1
-    f["y_tight"] = yt
-    f["tight_src"] = src
-    np.savez_compressed(path, **f)
+        for q, y1, src in pool.imap_unordered(_tight_row, jobs):
+            f["y_tight"][q] = np.nan if y1 is None else y1
+            f["tight_src"][q] = src
+            np.savez_compressed(path, **f)
+            print(f"row {q}: tight source {src}", flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--tight-only", action="store_true", help="add y_tight to the existing fixture")
     ap.add_argument("--cap", type=float, default=1800.0, help="wall seconds per tight reference stretch")
+    ap.add_argument("--only", default=None, help="comma-separated rows for --tight-only")
     ap.add_argument("--procs", type=int, default=6)
     ap.add_argument("--budget", type=float, default=2400.0, help="wall seconds per hot-core trajectory walk")
     ap.add_argument("--tail-budget", type=float, default=900.0, help="wall seconds per tail-point walk")
     a = ap.parse_args()
     if a.tight_only:
-        add_tight(a.procs, a.cap)
+        add_tight(a.procs, a.cap, None if a.only is None else [int(x) for x in a.only.split(",")])
         return
     jobs = [("hot", i, a.budget) for i in hot_points()] + [("tail", i, a.tail_budget) for i in TAIL_POINTS]
     with mp.get_context("fork").Pool(a.procs) as pool:

@@ -144,7 +144,7 @@ def grid4096_sample(n, procs, cap):
         np.stack([r[3] for r in res]), np.array([r[4] for r in res]), np.array([r[2] for r in res])
 
 
-def grid_tight(procs, cap, name="ref_cloud_grid.npz", net=None):
+def grid_tight(procs, cap, name="ref_cloud_grid.npz", net=None, only=None):
     """Add to a whole-model fixture (ref_cloud_grid.npz, or ref_alltypes.npz with net
     "alltypes") the reference's answer at tight tolerances (reltol 1e-11, abstol_factor 1e-17,
     abstol_min 1e-30) for each of its points: final_tight (NaN where the run exceeds `cap`
@@ -159,16 +159,22 @@ def grid_tight(procs, cap, name="ref_cloud_grid.npz", net=None):
     tight[:, INDEX["reltol"]] = 1e-11
     tight[:, INDEX["abstol_factor"]] = 1e-17
     tight[:, INDEX["abstol_min"]] = 1e-30
+    todo = [q for q in range(len(params)) if only is None or q in only]
     with mp.get_context("fork").Pool(procs) as pool:
-        res = pool.map(_model_capped, [(p, cap) if net is None else (p, cap, net) for p in tight], chunksize=1)
-    ft = np.full_like(f["final"], np.nan)
-    wall = np.full(len(params), np.nan)
-    for q, r in enumerate(res):
-        if r is not None and r[0] == 0 and r[2] == f["nrows"][q]:
+        res = pool.map(_model_capped, [(tight[q], cap) if net is None else (tight[q], cap, net) for q in todo],
+                       chunksize=1)
+    ft = f["final_tight"].copy() if "final_tight" in f else np.full_like(f["final"], np.nan)
+    wall = f["tight_wall"].copy() if "tight_wall" in f else np.full(len(params), np.nan)
+    for q, r in zip(todo, res):
+        ft[q] = np.nan
+        wall[q] = np.nan
+        # the same end time (at tight tolerances the reference may split more intervals and
+        # write more rows before it)
+        if r is not None and r[0] == 0 and r[1][0] == f["final"][q, 0]:
             ft[q] = r[1]
             wall[q] = r[4]
         print(f"point {f['index'][q] if 'index' in f else q}: tight "
-              f"{'capped' if r is None else (r[0], r[2], round(r[4], 1))}", flush=True)
+              f"{'capped' if r is None else (r[0], r[2], r[1][0], round(r[4], 1))}", flush=True)
     f["final_tight"] = ft
     f["tight_wall"] = wall
     np.savez_compressed(path, **f)
@@ -295,13 +301,15 @@ def main():
     ap.add_argument("--grid-tight", action="store_true", help="add tight-tolerance answers to ref_cloud_grid.npz")
     ap.add_argument("--coeff-only", action="store_true", help="only the coefficient-change / rates-output model")
     ap.add_argument("--alltypes-tight", action="store_true", help="add tight-tolerance answers to ref_alltypes.npz")
+    ap.add_argument("--only", default=None, help="comma-separated fixture rows to (re)compute with --*-tight")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     a = ap.parse_args()
+    only = None if a.only is None else [int(x) for x in a.only.split(",")]
     if a.grid_tight:
-        grid_tight(a.procs, a.cap)
+        grid_tight(a.procs, a.cap, only=only)
         return
     if a.alltypes_tight:
-        grid_tight(a.procs, a.cap, "ref_alltypes.npz", "alltypes")
+        grid_tight(a.procs, a.cap, "ref_alltypes.npz", "alltypes", only=only)
         return
     if a.coeff_only:
         coeff_fixture()

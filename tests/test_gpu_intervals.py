@@ -11,7 +11,7 @@ reference needs up to ~1e5 steps.  Gates: the interval's end is reached, every a
 On the "first-attempt" rows (stretches where the reference's DVODE spends its whole step budget)
 the reference's smallest abundances move by tens of percent when the stretch is integrated at
 tighter tolerances (y_tight: the reference itself at 100x tighter tolerances, or, where that
-does not finish, the host BDF core on the reference's own F at 1000x tighter; tight_src 1 or
+does not finish, the host BDF core on the reference's own F at the same; tight_src 1 or
 2); there an abundance passes within 1% of the reference plus twice the reference's own
 integration error, |y_ref - y_tight|.
 """

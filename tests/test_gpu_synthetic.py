@@ -8,9 +8,10 @@ the 150-species network runs the staged plan.  Gates:
 * full free-fall solves: status 0, elements conserved to 1e-10 relative;
 * the sparse plan (staged or not) against the dense LU, and the FP32 factorisation
   against FP64: every abundance above 1e-14 within 1%.
-No reference DVODE run exists for these networks (the reference is compiled for its
-default network only), so the trajectories themselves are "parity unpinned" against the
-reference and pinned only by these invariants and cross-checks.
+The trajectories on these networks are pinned against the reference itself elsewhere: the
+reference compiled on the same networks (their network.f90/odes.f90 written by
+uclchem_amd/reference_format.py) gives the fixtures of test_gpu_config3 (500/6000 free-fall
+grid) and test_gpu_serving (requests over both networks).
 """
 import numpy as np
 import pytest

@@ -1069,22 +1069,23 @@ constexpr int GJ_B = 8;
 template <typename LUT, typename MP>
 __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int ld, LDS double *red, LDS double *stage)
 {
+    using AT = LUT;          // arithmetic in the storage precision
     constexpr int B = GJ_B;
-    LDS double *P = stage;   // P[l*B+m] = inv(A_KK)(l, m)
+    LDS AT *P = (LDS AT *)stage;   // P[l*B+m] = inv(A_KK)(l, m)
     for (int kb = 0; kb < n; kb += B) {
         const int nb = min(B, n - kb);
         __syncthreads();
         if (tid() < 64) {   // wave 0: lane r*B + c holds element (r, c) of the pivot block
             const int r = tid() / B, c = tid() % B;
-            double a = (r < nb && c < nb) ? (double)M[(size_t)(kb + c) * ld + kb + r] : (r == c ? 1.0 : 0.0);
+            AT a = (r < nb && c < nb) ? (AT)M[(size_t)(kb + c) * ld + kb + r] : (r == c ? (AT)1 : (AT)0);
             bool bad = false;
 #pragma unroll
             for (int k = 0; k < B; k++) {
-                const double piv = __shfl(a, k * B + k);
-                const double rowk = __shfl(a, k * B + c);   // a(k, c)
-                const double colk = __shfl(a, r * B + k);   // a(r, k)
-                if (!(piv == piv) || piv == 0.0 || isinf(piv)) bad = true;
-                const double q = 1.0 / piv;
+                const AT piv = __shfl(a, k * B + k);
+                const AT rowk = __shfl(a, k * B + c);   // a(k, c)
+                const AT colk = __shfl(a, r * B + k);   // a(r, k)
+                if (!(piv == piv) || piv == (AT)0 || isinf(piv)) bad = true;
+                const AT q = (AT)1 / piv;
                 if (r == k) a = (c == k) ? q : a * q;
                 else a = (c == k) ? -colk * q : a - colk * (rowk * q);
             }
@@ -1096,12 +1097,12 @@ __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int 
         if (red[8] != 0.0) return 1;
         for (int j = tid(); j < n; j += NT) {   // row panel: A_KJ <- inv(A_KK) A_KJ
             if (j >= kb && j < kb + nb) continue;
-            double v[B];
+            AT v[B];
 #pragma unroll
-            for (int m = 0; m < B; m++) v[m] = m < nb ? (double)M[(size_t)j * ld + kb + m] : 0.0;
+            for (int m = 0; m < B; m++) v[m] = m < nb ? (AT)M[(size_t)j * ld + kb + m] : (AT)0;
 #pragma unroll
             for (int l = 0; l < B; l++) {
-                double acc = 0.0;
+                AT acc = 0;
 #pragma unroll
                 for (int m = 0; m < B; m++) acc += P[l * B + m] * v[m];
                 if (l < nb) M[(size_t)j * ld + kb + l] = (LUT)acc;
@@ -1113,20 +1114,20 @@ __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int 
             const int ti = tt % nt, tj = tt / nt;
             const int i0 = 4 * ti, j0 = 4 * tj;
             if ((i0 < kb + nb && i0 + 4 > kb) || (j0 < kb + nb && j0 + 4 > kb)) continue;
-            double acc[4][4];
+            AT acc[4][4];
 #pragma unroll
             for (int c = 0; c < 4; c++)
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    acc[r][c] = (i0 + r < n && j0 + c < n) ? (double)M[(size_t)(j0 + c) * ld + i0 + r] : 0.0;
+                    acc[r][c] = (i0 + r < n && j0 + c < n) ? (AT)M[(size_t)(j0 + c) * ld + i0 + r] : (AT)0;
 #pragma unroll
             for (int m = 0; m < B; m++) {
                 if (m < nb) {
-                    double a[4], b[4];
+                    AT a[4], b[4];
 #pragma unroll
-                    for (int r = 0; r < 4; r++) a[r] = (i0 + r < n) ? (double)M[(size_t)(kb + m) * ld + i0 + r] : 0.0;
+                    for (int r = 0; r < 4; r++) a[r] = (i0 + r < n) ? (AT)M[(size_t)(kb + m) * ld + i0 + r] : (AT)0;
 #pragma unroll
-                    for (int c = 0; c < 4; c++) b[c] = (j0 + c < n) ? (double)M[(size_t)(j0 + c) * ld + kb + m] : 0.0;
+                    for (int c = 0; c < 4; c++) b[c] = (j0 + c < n) ? (AT)M[(size_t)(j0 + c) * ld + kb + m] : (AT)0;
 #pragma unroll
                     for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -1142,12 +1143,12 @@ __device__ __attribute__((always_inline)) int block_gj_inverse(MP M, int n, int 
         __syncthreads();
         for (int i = tid(); i < n; i += NT) {   // column panel: A_IK <- -A_IK inv(A_KK)
             if (i >= kb && i < kb + nb) continue;
-            double c[B];
+            AT c[B];
 #pragma unroll
-            for (int l = 0; l < B; l++) c[l] = l < nb ? (double)M[(size_t)(kb + l) * ld + i] : 0.0;
+            for (int l = 0; l < B; l++) c[l] = l < nb ? (AT)M[(size_t)(kb + l) * ld + i] : (AT)0;
 #pragma unroll
             for (int m = 0; m < B; m++) {
-                double acc = 0.0;
+                AT acc = 0;
 #pragma unroll
                 for (int l = 0; l < B; l++) acc += c[l] * P[l * B + m];
                 if (m < nb) M[(size_t)(kb + m) * ld + i] = (LUT)(-acc);
@@ -1464,8 +1465,9 @@ UCG_BIG __device__ void sparse_form_newton(const DevNet &N, const double *J, LUT
 // dependencies only on earlier phases), then the trailing block (its Schur complement):
 // inverted in place by blocked Gauss-Jordan when the plan is staged, LU-factored when it
 // is not.  The sparse factors stay in the workspace in their precision (LUT); a staged
-// plan's tail, with the Schur-update sources, is worked on in FP64 in LDS (region) and
-// stored back in LUT.  stage: 512 doubles of LDS.
+// plan's tail, with the Schur-update sources, is worked on in LDS (region) in the same
+// precision (the FP32 form does its whole factorisation in FP32: the Newton iteration's FP64
+// residuals absorb the difference) and stored back.  stage: 512 doubles of LDS.
 template <typename LUT, typename WP>
 __device__ __forceinline__ void crout_phases(const DevNet &N, WP W, LDS double *red, int nph)
 {
@@ -1525,21 +1527,35 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
     } else {
         __syncthreads();
         const bool lp = UCG_LDS_SCHUR && N.nlp_src > 0;   // the last phase (Schur update of the tail) in LDS
-        crout_phases<LUT>(N, glbw(V), red, lp ? N.nw_nph - 1 : N.nw_nph);
-        if (red[9] != 0.0) return 1;
-        LDS double *tl = region;   // the tail in FP64 whatever the factor precision
-        for (int q = tid(); q < T * T; q += NT) tl[q] = (double)V[nsp + q];
         if (lp) {
-            LDS double *src = region + T * T;
+            // the phases before it touch only the sparse part's nsp values (a staged region
+            // holds them): eliminate on an LDS copy, so that every dependent access of the
+            // phase chain is an LDS access, then store the factors back for the solve
+            LDS LUT *Ws = (LDS LUT *)region;
+            const GLB LUT *Vg = glb(V);
+            for (int q = tid(); q < nsp; q += NT) Ws[q] = Vg[q];
+            __syncthreads();
+            crout_phases<LUT>(N, Ws, red, N.nw_nph - 1);
+            if (red[9] != 0.0) return 1;
+            for (int q = tid(); q < nsp; q += NT) V[q] = Ws[q];
+            __syncthreads();
+        } else {
+            crout_phases<LUT>(N, glbw(V), red, N.nw_nph);
+            if (red[9] != 0.0) return 1;
+        }
+        LDS LUT *tl = (LDS LUT *)region;   // the tail, worked on in the factor precision
+        for (int q = tid(); q < T * T; q += NT) tl[q] = V[nsp + q];
+        if (lp) {
+            LDS LUT *src = tl + T * T;
             const GLB LUT *Vg = glb(V);
             const GLB int *lsrc = glb(N.lp_src), *tgs = glb(N.nw_tg_slot), *tgp = glb(N.nw_tg_ptr);
-            for (int k = tid(); k < N.nlp_src; k += NT) src[k] = (double)Vg[lsrc[k]];
+            for (int k = tid(); k < N.nlp_src; k += NT) src[k] = Vg[lsrc[k]];
             __syncthreads();
             const int p = N.nw_nph - 1, t0 = N.nw_ph_ptr[p], t1 = N.nw_ph_ptr[p + 1], qb = N.nw_tg_ptr[t0];
             const GLB int *pr = glb(N.lp_pr);
             for (int t = t0 + tid(); t < t1; t += NT) {
                 const int e = tgs[t] - nsp;
-                double a0 = (double)tl[e], a1 = 0.0, a2 = 0.0, a3 = 0.0;
+                LUT a0 = tl[e], a1 = 0, a2 = 0, a3 = 0;
                 const int q1 = tgp[t + 1] - qb;
                 int q = tgp[t] - qb;
                 for (; q + 4 <= q1; q += 4) {
@@ -1558,8 +1574,8 @@ UCG_BIG __device__ int sparse_lu(const DevNet &N, LUT *V, LDS double *region, LD
         }
         c1 = wall_clock64();
         __syncthreads();
-        if (T > 0) r = block_gj_inverse<double>(tl, T, T, red, stage);
-        for (int q = tid(); q < T * T; q += NT) V[nsp + q] = (LUT)tl[q];
+        if (T > 0) r = block_gj_inverse<LUT>(tl, T, T, red, stage);
+        for (int q = tid(); q < T * T; q += NT) V[nsp + q] = tl[q];
     }
     __syncthreads();
     if (split) {

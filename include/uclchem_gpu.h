@@ -45,7 +45,7 @@ extern "C" {
 #define UCG_P_RELTOL 10       /* reltol */
 #define UCG_P_ABSFAC 11       /* abstol_factor */
 #define UCG_P_ABSMIN 12       /* abstol_min */
-#define UCG_P_MODEL 13        /* 0 cloud, 1 hot core */
+#define UCG_P_MODEL 13        /* 0 cloud, 1 hot core, 2 collapse (collapse.f90) */
 #define UCG_P_TEMPINDX 14     /* hot-core mass index 1..6 */
 #define UCG_P_MAXTEMP 15      /* hot-core maxTemp [K] */
 #define UCG_P_FREEZEFAC 16    /* freezeFactor */
@@ -98,7 +98,9 @@ extern "C" {
 #define UCG_P_IONMODEL 56         /* 0: 'L', 1: 'H' */
 #define UCG_P_IMPROVEDH2CRP 57    /* improvedH2CRPDissociation (needs UCG_P_CRATTENUATION) */
 #define UCG_P_ENFORCECHARGE 58    /* enforceChargeConservation */
-/* slots 59..63 are reserved (0) */
+#define UCG_P_COLLAPSE 59         /* collapse mode for UCG_P_MODEL 2: 1 BE1.1, 2 BE4, 3 filament,
+                                     4 ambipolar (collapse_mode, collapse.f90:1-7; model.py collapse) */
+/* slots 60..63 are reserved (0) */
 
 /* status codes (UCLCHEM's own, src/fortran_src/constants.f90:22-27) */
 #define UCG_OK 0
@@ -296,6 +298,26 @@ int ucg_queue_submit(int queue, const double *params, int *request);
 int ucg_queue_test(int queue, int request, int *done);
 int ucg_queue_result(int queue, int request, double *y, int *status, int *stats);
 int ucg_queue_close(int queue);
+
+/* Shared request queue over several resident networks (no reference counterpart: the
+ * reference serves one model per call, wrap.f90:15-80).  Its n_slots resident workgroups take
+ * the next posted request whatever its network, so the capacity follows the load instead of
+ * being split per network: a burst on one network runs on every slot.  Each slot's workspace
+ * and the launch's LDS are sized for the largest network of the queue; the networks must use
+ * the same linear solver; ucg_cancel on handles[0] stops the queue.
+ *   ucg_queue_open_shared: handles[n_networks] (at most 64), then as ucg_queue_open;
+ *   ucg_queue_submit_to: posts a request for network handles[network] (ucg_queue_submit
+ *   takes requests only on a one-network queue);
+ *   ucg_queue_test / _result / _close as for a one-network queue (y [neq of the request's
+ *   network]). */
+int ucg_queue_open_shared(const int *handles, int n_networks, int capacity, int n_slots, int lu_bits,
+                          double idle_timeout_s, int *queue);
+int ucg_queue_submit_to(int queue, int network, const double *params, int *request);
+
+/* When a finished request was taken by a workgroup and when it finished, in seconds of the
+ * device's constant-rate wall clock (differences between requests of one queue are
+ * meaningful; no reference counterpart: serving latency and occupancy measurements). */
+int ucg_queue_times(int queue, int request, double *t_taken, double *t_done);
 
 #ifdef __cplusplus
 }

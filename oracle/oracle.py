@@ -27,7 +27,7 @@ P_NAMES = ["dens", "temp", "zeta", "radfield", "baseav", "rout", "rin", "finalti
            "fd", "fli", "fna", "fpah", "f15n", "f13c", "f18o",
            "ebmaxh2", "ebmaxcr", "ebmaxuvcr", "epsilon", "uv_yield", "phi", "uvcreff", "omega",
            "desorb", "h2desorb", "crdesorb", "uvdesorb", "thermdesorb", "cosmicrayattenuation", "ionmodel",
-           "improvedh2crpdissociation", "enforcechargeconservation"]
+           "improvedh2crpdissociation", "enforcechargeconservation", "collapse"]
 P_NPARAM = 64
 _F32 = lambda x: float(np.float32(x))   # noqa: E731  default-real literals
 # Defaults of defaultparameters.f90:18-122 as the compiled reference holds them.
@@ -42,7 +42,7 @@ DEFAULTS = dict(dens=1.0e2, temp=10.0, zeta=1.0, radfield=1.0, baseav=2.0,
                 ebmaxh2=1.21e3, ebmaxcr=1.21e3, ebmaxuvcr=1.0e4, epsilon=_F32(0.01), uv_yield=_F32(0.03),
                 phi=1.0e5, uvcreff=1.0e-3, omega=0.5,
                 desorb=1.0, h2desorb=1.0, crdesorb=1.0, uvdesorb=1.0, thermdesorb=1.0, cosmicrayattenuation=0.0,
-                ionmodel=0.0, improvedh2crpdissociation=0.0, enforcechargeconservation=0.0)
+                ionmodel=0.0, improvedh2crpdissociation=0.0, enforcechargeconservation=0.0, collapse=0.0)
 
 
 def make_params(**kw) -> np.ndarray:

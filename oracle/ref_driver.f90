@@ -6,6 +6,8 @@
 ! with amdflang (wrap.f90, hotcore.f90):
 !   * the solveAbundances time loop for points=1   (src/fortran_src/wrap.f90:598-690)
 !   * hotcore updateTargetTime / updatePhysics      (src/fortran_src/hotcore.f90:56-90)
+!   * the choice of physics module per point: cloud, hot core, or the reference's own
+!     collapse module (collapse.f90, compiled in place) for model 2
 !   * a per-point reset of every module global the parameter parser can touch,
 !     so each grid point starts from the documented defaults
 !     (src/fortran_src/defaultparameters.f90:18-122; wrap.f90:699-983 never resets).
@@ -26,6 +28,8 @@ MODULE ucref_driver
         & GAS_DUST_DENSITY_RATIO, bulkGainFromMantleBuildUp
     USE postprocess_mod, only: lusecoldens, usepostprocess
     USE cloud_mod, only: cloudInitializePhysics => initializePhysics, cloudUpdateTargetTime => updateTargetTime
+    USE collapse_mod, only: collapse_mode, writePhysics, collapseInitializePhysics => initializePhysics, &
+        & collapseUpdateTargetTime => updateTargetTime, collapseUpdatePhysics => updatePhysics
     USE DVODE_F90_M, only: GET_STATS, VODE_OPTS, SET_OPTS, DVODE_F90
     IMPLICIT NONE
 
@@ -51,7 +55,7 @@ MODULE ucref_driver
         & P_FNA=38, P_FPAH=39, P_F15N=40, P_F13C=41, P_F18O=42, P_EBMAXH2=43, P_EBMAXCR=44, P_EBMAXUVCR=45, &
         & P_EPSILON=46, P_UV_YIELD=47, P_PHI=48, P_UVCREFF=49, P_OMEGA=50, P_DESORB=51, P_H2DESORB=52, &
         & P_CRDESORB=53, P_UVDESORB=54, P_THERMDESORB=55, P_CRATTEN=56, P_IONMODEL=57, P_IMPROVEDH2CRP=58, &
-        & P_ENFORCECHARGE=59, P_NPARAM=64
+        & P_ENFORCECHARGE=59, P_COLLAPSE=60, P_NPARAM=64
 CONTAINS
 
     SUBROUTINE reset_globals()
@@ -70,6 +74,7 @@ CONTAINS
         uvcreff=1.0d-3; omega=0.5
         usepostprocess=.true.; lusecoldens=.false.
         modelKind=0; tempIndx=1; maxTemp=300.0d0
+        collapse_mode=1; writePhysics=.false.
         h2CRPRate=0.0d0   ! module state the physics update sets; each point starts from zero
     END SUBROUTINE reset_globals
 
@@ -80,6 +85,7 @@ CONTAINS
         freefall=(p(P_FREEFALL) .ne. 0.0d0); finalDens=p(P_FINALDENS)
         reltol=p(P_RELTOL); abstol_factor=p(P_ABSFAC); abstol_min=p(P_ABSMIN)
         modelKind=NINT(p(P_MODEL)); tempIndx=NINT(p(P_TEMPINDX)); maxTemp=p(P_MAXTEMP)
+        collapse_mode=NINT(p(P_COLLAPSE))
         freezeFactor=p(P_FREEZEFAC); endAtFinalDensity=(p(P_ENDATFINALDENS) .ne. 0.0d0)
         freefallFactor=p(P_FREEFALLFAC); MXSTEP=NINT(p(P_MXSTEP))
         metallicity=p(P_METALLICITY); ion=NINT(p(P_ION), dp); fh=p(P_FH); fhe=p(P_FHE); fc=p(P_FC); fo=p(P_FO)
@@ -140,7 +146,11 @@ CONTAINS
         flag=0
         CALL coreInitializePhysics(flag)
         IF (flag .lt. 0) RETURN
-        CALL cloudInitializePhysics(flag)   ! hotcore's initializePhysics does the same for three-phase networks
+        IF (modelKind .eq. 2) THEN
+            CALL collapseInitializePhysics(flag)   ! reads flag before setting it: 0 here, as in wrap.f90
+        ELSE
+            CALL cloudInitializePhysics(flag)   ! hotcore's initializePhysics does the same for three-phase networks
+        END IF
         IF (flag .lt. 0) RETURN
         CALL initializeChemistry(.false.)
         dstep=1   ! coreInitializePhysics leaves its loop counter at points+1 (wrap.f90:643 resets it too)
@@ -521,6 +531,8 @@ CONTAINS
             timeInYears=currentTime/SECONDS_PER_YEAR
             IF (modelKind .eq. 1) THEN
                 CALL hotcoreUpdateTargetTime
+            ELSE IF (modelKind .eq. 2) THEN
+                CALL collapseUpdateTargetTime
             ELSE
                 CALL cloudUpdateTargetTime
             END IF
@@ -544,6 +556,7 @@ CONTAINS
             timeInYears= targetTime/SECONDS_PER_YEAR
             CALL coreUpdatePhysics
             IF (modelKind .eq. 1) CALL hotcoreUpdatePhysics
+            IF (modelKind .eq. 2) CALL collapseUpdatePhysics
             CALL record_row(rec, maxrec, nrec)
             IF (nrec .ge. maxrec) THEN
                 flag = NOT_ENOUGH_TIMEPOINTS_ERROR

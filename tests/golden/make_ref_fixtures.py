@@ -18,6 +18,9 @@ Outputs (small .npz files next to this script):
                          (t_end log-uniform 1e3..1e7 yr)
   ref_coeff.npz          the static cloud to 1e5 yr with reaction coefficients changed (the
                          alpha/beta/gamma keys) and the rate array at five output rows
+  ref_collapse_models.npz  the reference's collapse module (collapse.f90: BE1.1, BE4, filament,
+                         ambipolar) for two points each: the physics of every output row, the
+                         abundances of five rows (collapse_rows), and those at tight tolerances
 """
 from __future__ import annotations
 
@@ -115,6 +118,92 @@ def _model_capped(args):
             return None
         z = np.load(tmp.name)
         return int(z["rc"]), z["final"].copy(), int(z["nrows"]), z["stats"].copy(), float(z["wall"])
+
+
+COLLAPSE_MODES = {"BE1.1": 1, "BE4": 2, "filament": 3, "ambipolar": 4}
+
+
+def collapse_points():
+    """Two points per collapse model: the reference's defaults, and one with a warmer, more
+    strongly irradiated cloud (initialTemp 14 K, zeta 3, radfield 5, baseAv 1, rout 0.08 pc).
+    Modes 1 and 2 end at 0.97 maxTime whatever finalTime says; the filament fits hold until
+    5.47 time units (about 1.39 Myr), so modes 3 and 4 run to 1 Myr and 1.2 Myr."""
+    from uclchem_amd.params import make_params
+    out, names = [], []
+    for name, mode in COLLAPSE_MODES.items():
+        for q, extra in enumerate([dict(), dict(initialtemp=14.0, zeta=3.0, radfield=5.0, baseav=1.0, rout=0.08)]):
+            ft = 1.0e6 if mode <= 3 else 1.2e6
+            if mode == 3 and q:
+                ft = 1.2e6
+            out.append(make_params(dict(model=2, collapse=mode, finaltime=ft, **extra)))
+            names.append(name)
+    return np.stack(out), np.array(names)
+
+
+def collapse_rows(n):
+    """The output rows of a collapse fixture whose abundances are kept: the row after 1e4 yr
+    (13th), 1/8, 1/4 and 1/2 of the way, and the last."""
+    return np.array([12, n // 8, n // 4, n // 2, n - 1])
+
+
+def _collapse_capped(args):
+    """One reference model in a child process with a wall-time cap: (rc, physics rows [nrec, 4],
+    the rows of collapse_rows, stats, wall), or None when the cap is exceeded.
+    args = (params, cap, lib)."""
+    import subprocess
+    import tempfile
+    p, cap, lib = args
+    with tempfile.NamedTemporaryFile(suffix=".npz") as tmp:
+        code = ("import sys,time,numpy as np; sys.path.insert(0,%r); from oracle import RefLib; "
+                "p=np.array([float(x) for x in sys.argv[1].split(',')]); t=time.time(); ref=RefLib(sys.argv[3]); "
+                "rc,rows,st=ref.model(p,maxrec=4000); n=len(rows); k=np.array([12,n//8,n//4,n//2,n-1]); "
+                "np.savez(sys.argv[2],rc=rc,phys=rows[:,:4],keep=rows[k],stats=st,wall=time.time()-t)"
+                ) % os.path.join(ROOT, "oracle")
+        try:
+            subprocess.run([sys.executable, "-c", code, ",".join(repr(float(x)) for x in p), tmp.name, lib],
+                           timeout=cap, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        except subprocess.TimeoutExpired:
+            return None
+        z = np.load(tmp.name)
+        return int(z["rc"]), z["phys"].copy(), z["keep"].copy(), z["stats"].copy(), float(z["wall"])
+
+
+def collapse_fixture(procs, cap, lib):
+    """ref_collapse_models.npz: collapse_points through the reference at its default
+    tolerances (every row's time, density, gasTemp and Av; the rows of collapse_rows in full)
+    and at tight tolerances (rows_tight, NaN where the run exceeds `cap` seconds or writes
+    other rows)."""
+    from uclchem_amd.params import INDEX
+    params, names = collapse_points()
+    tight = params.copy()
+    tight[:, INDEX["reltol"]] = 1e-11
+    tight[:, INDEX["abstol_factor"]] = 1e-17
+    tight[:, INDEX["abstol_min"]] = 1e-30
+    t0 = time.time()
+    jobs = [(p, cap, lib) for p in params] + [(p, cap, lib) for p in tight]
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_collapse_capped, jobs, chunksize=1)
+    n = len(params)
+    base, tres = res[:n], res[n:]
+    assert all(r is not None for r in base), "a default-tolerance collapse model exceeded the cap"
+    nrows = np.array([len(r[1]) for r in base])
+    phys = np.full((n, nrows.max(), 4), np.nan)
+    for q, r in enumerate(base):
+        phys[q, :nrows[q]] = r[1]
+    rows = np.stack([r[2] for r in base])               # [n, 5, 4 + nspec]
+    rows_tight = np.full_like(rows, np.nan)
+    tight_wall = np.full(n, np.nan)
+    for q, r in enumerate(tres):
+        if r is not None and r[0] == 0 and np.array_equal(r[2][:, 0], rows[q, :, 0]):
+            rows_tight[q] = r[2]
+            tight_wall[q] = r[4]
+    np.savez_compressed(os.path.join(HERE, "ref_collapse_models.npz"), params=params, model=names,
+                        rc=np.array([r[0] for r in base]), nrows=nrows, phys=phys,
+                        row_index=np.stack([collapse_rows(k) for k in nrows]), rows=rows, rows_tight=rows_tight,
+                        stats=np.stack([r[3] for r in base]), wall=np.array([r[4] for r in base]),
+                        tight_wall=tight_wall)
+    print(f"ref_collapse_models: {n} points in {time.time() - t0:.1f}s, rows {nrows.tolist()}, "
+          f"tight {int(np.isfinite(rows_tight[:, 0, 0]).sum())} of {n}", flush=True)
 
 
 def grid4096_sample(n, procs, cap):
@@ -303,8 +392,14 @@ def main():
     ap.add_argument("--alltypes-tight", action="store_true", help="add tight-tolerance answers to ref_alltypes.npz")
     ap.add_argument("--only", default=None, help="comma-separated fixture rows to (re)compute with --*-tight")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
+    ap.add_argument("--collapse-only", action="store_true", help="only the collapse-module models")
+    ap.add_argument("--ref-lib", default=os.path.join(ROOT, "oracle", "_ref", "libucref.so"),
+                    help="the reference build to run (--collapse-only)")
     a = ap.parse_args()
     only = None if a.only is None else [int(x) for x in a.only.split(",")]
+    if a.collapse_only:
+        collapse_fixture(a.procs, a.cap, a.ref_lib)
+        return
     if a.grid_tight:
         grid_tight(a.procs, a.cap, only=only)
         return

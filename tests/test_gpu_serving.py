@@ -115,3 +115,48 @@ def test_queue_errors_are_reported(nets):
                 time.sleep(0.01)
     finally:
         q.close()
+
+
+def _max_overlap(intervals):
+    """Largest number of [start, end) intervals that are open at one time."""
+    ev = sorted([(a, 1) for a, b in intervals] + [(b, -1) for a, b in intervals], key=lambda e: (e[0], e[1]))
+    n = best = 0
+    for _, d in ev:
+        n += d
+        best = max(best, n)
+    return best
+
+
+def test_shared_queue_gives_a_burst_every_slot(nets):
+    """One queue shared by the three networks (ucg_queue_open_shared): a burst of requests on
+    the default network alone runs on all of the queue's slots at once (device timestamps:
+    every slot holds one of them at some moment; a per-network partition would give it a
+    third), requests of the other networks interleave with it, and every result is
+    bit-identical to the same point solved alone."""
+    from uclchem_amd import serving
+    from uclchem_amd.params import make_params
+    names = list(nets)
+    n_slots = 12
+    q = serving.SharedQueue([nets[k] for k in names], capacity=64, n_slots=n_slots)
+    burst = [make_params(initialdens=10 ** (3.5 + 0.05 * i), initialtemp=10.0, finaltime=1e5) for i in range(2 * n_slots)]
+    others = [(names.index(k), make_params(initialdens=1e4, initialtemp=12.0, finaltime=3e3))
+              for k in names if k != "default"]
+    try:
+        reqs = [(names.index("default"), p, q.submit(p, names.index("default"))) for p in burst]
+        reqs += [(k, p, q.submit(p, k)) for k, p in others]
+        t0 = time.time()
+        while not all(q.done(r) for _, _, r in reqs):
+            assert time.time() - t0 < 240, "shared queue did not finish"
+            time.sleep(0.01)
+        res = [(k, p, q.result(r, k), q.times(r)) for k, p, r in reqs]
+    finally:
+        q.close()
+    spans = [t for k, _, _, t in res if k == names.index("default")]
+    assert all(b > a for a, b in spans)
+    busy = _max_overlap(spans)
+    print(f"[shared] {busy} of {n_slots} slots ran default-network requests at once", flush=True)
+    assert busy == n_slots
+    for k, p, (y, st, stats), _ in res:
+        alone = nets[names[k]].solve(p[None])
+        assert st == 0 and alone["status"][0] == 0
+        assert np.array_equal(y, alone["y"][0]) and np.array_equal(stats, alone["stats"][0]), names[k]

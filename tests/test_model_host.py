@@ -104,6 +104,19 @@ def test_starting_chemistry_and_hot_core_arguments(net):
     assert (p[INDEX["model"]], p[INDEX["tempindx"]], p[INDEX["maxtemp"]]) == (1.0, 3.0, 250.0)
 
 
+def test_collapse_arguments(net):
+    """collapse_grid names the reference's four collapse models (model.py:355) and passes the
+    mode in the collapse slot of model 2."""
+    for name, mode in [("BE1.1", 1), ("BE4", 2), ("filament", 3), ("ambipolar", 4)]:
+        model.collapse_grid(name, [{"initialTemp": 12.0}])
+        p = net.calls[-1]["params"][0]
+        assert (p[INDEX["model"]], p[INDEX["collapse"]], p[INDEX["initialtemp"]]) == (2.0, float(mode), 12.0)
+    for bad in ("BE2", "be1.1", 1):
+        with pytest.raises(ValueError):
+            model.collapse_grid(bad, [{}])
+    assert INDEX["collapse"] == 59   # UCG_P_COLLAPSE
+
+
 def test_rates_rows_and_coefficient_groups(net, monkeypatch):
     """return_rates gives ratesArray[rows, points, nreac] trimmed like the abundances; points
     whose dictionaries carry the same alpha/beta/gamma changes run together on one derived

@@ -77,7 +77,7 @@ class _FakeQueue:
 def test_stream_replay_returns_every_request_with_its_own_result(monkeypatch):
     monkeypatch.setattr(serving, "RequestQueue", _FakeQueue)
     names = ["small", "default", "big"]
-    srv = serving.Server({n: n for n in names}, {n: 4 for n in names}, capacity=64)
+    srv = serving.Server({n: n for n in names}, {n: 4 for n in names}, capacity=64, shared=False)
     reqs = serving.poisson_requests(24, 400.0, names)
     done = serving.run_stream(srv, reqs, timeout_s=30.0)
     srv.close()
@@ -89,3 +89,42 @@ def test_stream_replay_returns_every_request_with_its_own_result(monkeypatch):
         assert np.all(t.y == sent[INDEX["finaltime"]]) and t.status == 0
     assert sorted((t.network, t.request) for t in done) == sorted(
         (n, i) for n in names for i in range(len(by_net[n])))
+
+
+class _FakeShared(_FakeQueue):
+    """Stands in for a shared device queue: one request sequence over every network."""
+
+    def __init__(self, nets, capacity, n_slots, lu_bits=64, idle_timeout_s=600.0):
+        super().__init__(None, capacity, n_slots)
+        self.nets, self.n_slots, self.which = list(nets), n_slots, []
+
+    def submit(self, params, network):
+        self.which.append(network)
+        return super().submit(params)
+
+    def result(self, request, network):
+        assert self.which[request] == network
+        return super().result(request)
+
+
+def test_shared_server_posts_every_network_to_one_queue(monkeypatch):
+    monkeypatch.setattr(serving, "SharedQueue", _FakeShared)
+    names = ["small", "default", "big"]
+    srv = serving.Server({n: n for n in names}, {n: 4 for n in names}, capacity=64)
+    assert srv.shared_queue.n_slots == 12 and srv.shared_queue.nets == names and not srv.queues
+    reqs = serving.poisson_requests(24, 400.0, names)
+    done = serving.run_stream(srv, reqs, timeout_s=30.0)
+    srv.close()
+    assert not srv.shared_queue.open and len(done) == 24
+    assert sorted(t.request for t in done) == list(range(24))      # one request sequence
+    for t in done:
+        arr, name, p = reqs[t.request]
+        assert t.network == name and srv.shared_queue.which[t.request] == names.index(name)
+        assert np.all(t.y == p[INDEX["finaltime"]])
+
+
+def test_plan_shared_slots_within_cus_and_budget(fake_sizes):
+    assert serving.plan_shared_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1e9) == 252
+    assert serving.plan_shared_slots(fake_sizes, n_cu=256, hbm_budget_bytes=14_000.0) == 10   # 1400 B per slot
+    with pytest.raises(GpuError):
+        serving.plan_shared_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1000.0)

@@ -75,6 +75,13 @@ struct Queue {
     void *ws = nullptr;
     int *counter = nullptr;
     size_t ws_bytes = 0;
+    // shared queue over several networks: their handles, each request's network index
+    // (pinned, read by the kernel) and the device array of their descriptors
+    std::vector<int> handles;
+    int *h_net = nullptr;         // [capacity]
+    void *d_nets = nullptr;
+    double *h_times = nullptr;    // [2][capacity] device wall clock (ticks) when taken, when finished
+    int y_stride = 0;             // doubles between requests' outputs in h_y
 };
 std::vector<Queue> g_queues;
 std::mutex g_cancel_mu;   // orders the cancel-word writes of concurrent ucg_cancel calls
@@ -82,9 +89,9 @@ std::mutex g_cancel_mu;   // orders the cancel-word writes of concurrent ucg_can
 void queue_release(Queue &q)
 {
     for (void *p : {(void *)q.h_params, (void *)q.h_ctrl, (void *)q.h_done, (void *)q.h_y, (void *)q.h_status,
-                    (void *)q.h_stats})
+                    (void *)q.h_stats, (void *)q.h_net, (void *)q.h_times})
         if (p) (void)hipHostFree(p);
-    for (void *p : {q.ws, (void *)q.counter})
+    for (void *p : {q.ws, (void *)q.counter, q.d_nets})
         if (p) (void)hipFree(p);
     if (q.stream) (void)hipStreamDestroy(q.stream);
     q = Queue();
@@ -250,9 +257,13 @@ int check_params(int n, const double *params)
             !(q[UCG_P_MXSTEP] >= 1) || !(q[UCG_P_ROUT] > q[UCG_P_RIN]) || !(q[UCG_P_MAXSTEPS] >= 0))
             return fail(UCG_E_PARAM, "invalid parameter vector for point " + std::to_string(p));
         int model = (int)q[UCG_P_MODEL];
-        if (model != 0 && model != 1) return fail(UCG_E_PARAM, "model must be 0 (cloud) or 1 (hot core)");
+        if (model != 0 && model != 1 && model != 2)
+            return fail(UCG_E_PARAM, "model must be 0 (cloud), 1 (hot core) or 2 (collapse)");
         if (model == 1 && ((int)q[UCG_P_TEMPINDX] < 1 || (int)q[UCG_P_TEMPINDX] > 6))
             return fail(UCG_E_PARAM, "tempindx must be 1..6");
+        if (model == 2 && !(q[UCG_P_COLLAPSE] == 1 || q[UCG_P_COLLAPSE] == 2 || q[UCG_P_COLLAPSE] == 3 ||
+                            q[UCG_P_COLLAPSE] == 4))   // collapse.f90:43-46 ("unacceptable collapse mode")
+            return fail(UCG_E_PARAM, "collapse mode must be 1 (BE1.1), 2 (BE4), 3 (filament) or 4 (ambipolar)");
         if (q[UCG_P_IONMODEL] != 0.0 && q[UCG_P_IONMODEL] != 1.0)
             return fail(UCG_E_PARAM, "ionmodel must be 0 ('L') or 1 ('H')");
         if (q[UCG_P_OMEGA] == 1.0 || q[UCG_P_UVCREFF] == 0.0)
@@ -282,9 +293,9 @@ SlotLayout slot_layout(const NetHandle &h, int lu_bits)
 }
 
 int launch_kernel(const ucg::DevNet &N, const ucg::DevNet *dN, const ucg::SolveArgs &A, int n_slots, int lu_bits,
-                  hipStream_t stream)
+                  hipStream_t stream, size_t lds_min = 0)
 {
-    const size_t lds = lds_bytes(N);
+    const size_t lds = std::max(lds_bytes(N), lds_min);
     if (lu_bits == 64) {
         HIPCHK(hipFuncSetAttribute((const void *)ucg::solve_points_kernel<double>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -965,7 +976,8 @@ int ucg_network_free(int handle)
     NetHandle *h = get(handle);
     if (!h) return fail(UCG_E_HANDLE, "bad network handle");
     for (const Queue &q : g_queues)
-        if (q.used && !q.drained && q.handle == handle) return fail(UCG_E_ARG, "close the network's request queues first");
+        if (q.used && !q.drained && std::find(q.handles.begin(), q.handles.end(), handle) != q.handles.end())
+            return fail(UCG_E_ARG, "close the network's request queues first");
     for (void *p : h->allocs) (void)hipFree(p);
     if (h->ws) (void)hipFree(h->ws);
     if (h->cancel_stream) (void)hipStreamDestroy(h->cancel_stream);
@@ -1338,16 +1350,34 @@ int ucg_queue_slot_bytes(int handle, int lu_bits, size_t *bytes)
     return 0;
 }
 
-int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double idle_timeout_s, int *queue)
+static int queue_open_locked(const int *handles, int nnet, int capacity, int n_slots, int lu_bits,
+                             double idle_timeout_s, int *queue)
 {
-    std::lock_guard<std::mutex> lk(g_mu);
-    NetHandle *h = get(handle);
-    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
-    if (!queue) return fail(UCG_E_ARG, "null output");
+    if (!queue || !handles || nnet <= 0) return fail(UCG_E_ARG, "null output or no network");
+    std::vector<NetHandle *> hs;
+    for (int k = 0; k < nnet; k++) {
+        NetHandle *h = get(handles[k]);
+        if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+        if (h->linear_solver != g_nets[handles[0]].linear_solver)
+            return fail(UCG_E_ARG, "the networks of a shared queue must use the same linear solver");
+        hs.push_back(h);
+    }
     if (capacity <= 0 || n_slots <= 0 || n_slots > g_num_cu)
         return fail(UCG_E_ARG, "capacity must be positive and n_slots in [1, number of CUs]");
     if (lu_bits != 64 && lu_bits != 32) return fail(UCG_E_ARG, "lu_bits must be 64 or 32");
     if (!(idle_timeout_s > 0.0) || idle_timeout_s > 3600.0) return fail(UCG_E_ARG, "idle timeout must be in (0, 3600] s");
+    // one slot layout and one LDS size that fit every network of the queue
+    size_t mat = 0, jac = 0, stride = 0, lds = 0;
+    int neq_max = 0;
+    for (NetHandle *h : hs) {
+        const SlotLayout sl = slot_layout(*h, lu_bits);
+        mat = std::max(mat, sl.mat);
+        jac = std::max(jac, sl.jac);
+        stride = std::max(stride, sl.stride - sl.mat - sl.jac);   // the rate coefficients
+        lds = std::max(lds, lds_bytes(h->dev));
+        neq_max = std::max(neq_max, h->dev.neq);
+    }
+    stride += mat + jac;
     int id = -1;
     for (int i = 0; i < (int)g_queues.size(); i++)
         if (!g_queues[i].used) {   // drained queues keep their slot until released
@@ -1361,12 +1391,13 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     Queue &q = g_queues[id];
     q = Queue();
     q.used = true;
-    q.handle = handle;
+    q.handle = handles[0];
+    q.handles.assign(handles, handles + nnet);
     q.capacity = capacity;
     q.n_slots = n_slots;
     q.lu_bits = lu_bits;
-    const ucg::DevNet &N = h->dev;
-    const SlotLayout sl = slot_layout(*h, lu_bits);
+    q.y_stride = neq_max;
+    NetHandle *h = hs[0];
     const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
     hipError_t e = hipSuccess;
     auto hm = [&](void **p, size_t b) {
@@ -1376,25 +1407,37 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     hm((void **)&q.h_params, (size_t)capacity * UCG_NPARAM * sizeof(double));
     hm((void **)&q.h_ctrl, 2 * sizeof(int));
     hm((void **)&q.h_done, (size_t)capacity * sizeof(int));
-    hm((void **)&q.h_y, (size_t)capacity * N.neq * sizeof(double));
+    hm((void **)&q.h_y, (size_t)capacity * neq_max * sizeof(double));
     hm((void **)&q.h_status, (size_t)capacity * sizeof(int));
     hm((void **)&q.h_stats, (size_t)capacity * 8 * sizeof(int));
-    q.ws_bytes = sl.stride * (size_t)n_slots;
+    if (nnet > 1) hm((void **)&q.h_net, (size_t)capacity * sizeof(int));
+    hm((void **)&q.h_times, (size_t)capacity * 2 * sizeof(double));
+    q.ws_bytes = stride * (size_t)n_slots;
     if (e == hipSuccess) e = hipMalloc(&q.ws, q.ws_bytes);
     if (e == hipSuccess) e = hipMalloc(&q.counter, sizeof(int));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking);
+    if (e == hipSuccess && nnet > 1) {   // the descriptors' device addresses, in queue order
+        std::vector<const ucg::DevNet *> dn;
+        for (NetHandle *x : hs) dn.push_back(x->d_dev);
+        e = hipMalloc(&q.d_nets, nnet * sizeof(const ucg::DevNet *));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(q.d_nets, dn.data(), nnet * sizeof(const ucg::DevNet *), hipMemcpyHostToDevice,
+                               q.stream);
+    }
     // stream-ordered clear: other queues' kernels may be running, and nothing here may wait
     // for them (a device-wide synchronisation would wait for kernels that only end on close)
     if (e == hipSuccess) e = hipMemsetAsync(q.counter, 0, sizeof(int), q.stream);
     ucg::SolveArgs A = {};
     void *d_params = nullptr, *d_ctrl = nullptr, *d_done = nullptr, *d_y = nullptr, *d_status = nullptr,
-         *d_stats = nullptr;
+         *d_stats = nullptr, *d_net = nullptr, *d_times = nullptr;
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_params, q.h_params, 0);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_ctrl, q.h_ctrl, 0);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_done, q.h_done, 0);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_y, q.h_y, 0);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_status, q.h_status, 0);
     if (e == hipSuccess) e = hipHostGetDevicePointer(&d_stats, q.h_stats, 0);
+    if (e == hipSuccess && q.h_net) e = hipHostGetDevicePointer(&d_net, q.h_net, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&d_times, q.h_times, 0);
     if (e != hipSuccess) {
         std::string msg = std::string("queue set-up: ") + hipGetErrorString(e);
         queue_release(q);
@@ -1407,17 +1450,22 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     A.stats = (int *)d_stats;
     A.sparse = h->linear_solver == UCG_LINSOL_SPARSE ? 1 : 0;
     A.ws = q.ws;
-    A.ws_stride = sl.stride;
-    A.ws_mat_bytes = sl.mat;
-    A.ws_jac_bytes = sl.jac;
+    A.ws_stride = stride;
+    A.ws_mat_bytes = mat;
+    A.ws_jac_bytes = jac;
     A.counter = q.counter;
-    A.cancel = h->cancel;   // ucg_cancel on the network stops its queues too
+    A.cancel = h->cancel;   // ucg_cancel on the (first) network stops its queues too
     A.gen = ++h->gen;
     A.q_avail = (const int *)d_ctrl;
     A.q_closed = (const int *)d_ctrl + 1;
     A.q_done = (int *)d_done;
     A.q_idle_ticks = (long long)(idle_timeout_s * 1.0e8);
-    int rc = launch_kernel(N, h->d_dev, A, n_slots, lu_bits, q.stream);
+    A.t_start = (double *)d_times;
+    A.t_done = (double *)d_times + capacity;
+    A.q_nets = (const ucg::DevNet *const *)q.d_nets;
+    A.q_net = (const int *)d_net;
+    A.y_stride = neq_max;
+    int rc = launch_kernel(h->dev, h->d_dev, A, n_slots, lu_bits, q.stream, lds);
     if (rc) {
         queue_release(q);
         return rc;
@@ -1426,10 +1474,40 @@ int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double id
     return 0;
 }
 
+int ucg_queue_open(int handle, int capacity, int n_slots, int lu_bits, double idle_timeout_s, int *queue)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return queue_open_locked(&handle, 1, capacity, n_slots, lu_bits, idle_timeout_s, queue);
+}
+
+int ucg_queue_open_shared(const int *handles, int n_networks, int capacity, int n_slots, int lu_bits,
+                          double idle_timeout_s, int *queue)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n_networks > 64) return fail(UCG_E_ARG, "at most 64 networks per shared queue");
+    return queue_open_locked(handles, n_networks, capacity, n_slots, lu_bits, idle_timeout_s, queue);
+}
+
 static Queue *get_queue(int queue)
 {
     if (queue < 0 || queue >= (int)g_queues.size() || !g_queues[queue].used || g_queues[queue].drained) return nullptr;
     return &g_queues[queue];
+}
+
+static int queue_submit_locked(Queue *q, int network, const double *params, int *request)
+{
+    if (!params || !request) return fail(UCG_E_ARG, "null buffer");
+    if (network < 0 || network >= (int)q->handles.size()) return fail(UCG_E_ARG, "bad network index for this queue");
+    int rc = check_params(1, params);
+    if (rc) return rc;
+    if (q->next >= q->capacity) return fail(UCG_E_ARG, "queue capacity reached");
+    if (__atomic_load_n(&q->h_ctrl[1], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "queue is closed");
+    const int i = q->next++;
+    memcpy(q->h_params + (size_t)i * UCG_NPARAM, params, UCG_NPARAM * sizeof(double));
+    if (q->h_net) __atomic_store_n(&q->h_net[i], network, __ATOMIC_RELAXED);
+    __atomic_store_n(&q->h_ctrl[0], i + 1, __ATOMIC_RELEASE);   // the parameters are visible first
+    *request = i;
+    return 0;
 }
 
 int ucg_queue_submit(int queue, const double *params, int *request)
@@ -1437,16 +1515,16 @@ int ucg_queue_submit(int queue, const double *params, int *request)
     std::lock_guard<std::mutex> lk(g_mu);
     Queue *q = get_queue(queue);
     if (!q) return fail(UCG_E_HANDLE, "bad queue");
-    if (!params || !request) return fail(UCG_E_ARG, "null buffer");
-    int rc = check_params(1, params);
-    if (rc) return rc;
-    if (q->next >= q->capacity) return fail(UCG_E_ARG, "queue capacity reached");
-    if (__atomic_load_n(&q->h_ctrl[1], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "queue is closed");
-    const int i = q->next++;
-    memcpy(q->h_params + (size_t)i * UCG_NPARAM, params, UCG_NPARAM * sizeof(double));
-    __atomic_store_n(&q->h_ctrl[0], i + 1, __ATOMIC_RELEASE);   // the parameters are visible first
-    *request = i;
-    return 0;
+    if (q->handles.size() > 1) return fail(UCG_E_ARG, "a shared queue takes requests through ucg_queue_submit_to");
+    return queue_submit_locked(q, 0, params, request);
+}
+
+int ucg_queue_submit_to(int queue, int network, const double *params, int *request)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "bad queue");
+    return queue_submit_locked(q, network, params, request);
 }
 
 int ucg_queue_test(int queue, int request, int *done)
@@ -1468,10 +1546,24 @@ int ucg_queue_result(int queue, int request, double *y, int *status, int *stats)
     if (!q) return fail(UCG_E_HANDLE, "bad queue");
     if (request < 0 || request >= q->next) return fail(UCG_E_ARG, "bad request");
     if (!__atomic_load_n(&q->h_done[request], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "request not finished");
-    const int neq = g_nets[q->handle].dev.neq;
-    if (y) memcpy(y, q->h_y + (size_t)request * neq, (size_t)neq * sizeof(double));
+    const int net = q->h_net ? __atomic_load_n(&q->h_net[request], __ATOMIC_RELAXED) : 0;
+    const int neq = g_nets[q->handles[net]].dev.neq;
+    if (y) memcpy(y, q->h_y + (size_t)request * q->y_stride, (size_t)neq * sizeof(double));
     if (status) *status = q->h_status[request];
     if (stats) memcpy(stats, q->h_stats + (size_t)request * 8, 8 * sizeof(int));
+    return 0;
+}
+
+int ucg_queue_times(int queue, int request, double *t_taken, double *t_done)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    Queue *q = get_queue(queue);
+    if (!q) return fail(UCG_E_HANDLE, "bad queue");
+    if (request < 0 || request >= q->next) return fail(UCG_E_ARG, "bad request");
+    if (!__atomic_load_n(&q->h_done[request], __ATOMIC_ACQUIRE)) return fail(UCG_E_ARG, "request not finished");
+    const double tick = 1.0e-8;   // the device wall clock runs at 100 MHz
+    if (t_taken) *t_taken = q->h_times[request] * tick;
+    if (t_done) *t_done = q->h_times[q->capacity + request] * tick;
     return 0;
 }
 

@@ -60,7 +60,7 @@ enum { P_DENS, P_TEMP, P_ZETA, P_RADFIELD, P_BASEAV, P_ROUT, P_RIN, P_FINALTIME,
        P_FD, P_FLI, P_FNA, P_FPAH, P_F15N, P_F13C, P_F18O,
        P_EBMAXH2, P_EBMAXCR, P_EBMAXUVCR, P_EPSILON, P_UV_YIELD, P_PHI, P_UVCREFF, P_OMEGA,
        P_DESORB, P_H2DESORB, P_CRDESORB, P_UVDESORB, P_THERMDESORB, P_CRATTEN, P_IONMODEL, P_IMPROVEDH2CRP,
-       P_ENFORCECHARGE, P_NPARAM = 64 };
+       P_ENFORCECHARGE, P_COLLAPSE, P_NPARAM = 64 };
 
 // physical constants (constants.f90, surfacereactions.f90:33-51)
 constexpr double K_BOLTZ = 1.38065040e-16;
@@ -153,7 +153,8 @@ __host__ __device__ inline int flux_region_len(const DevNet &N)
     const int a = N.npart_pad + 2 * npad;
     const int g = N.gt_off + N.ngt;                  // fluxes + chunk partial sums
     const int b0 = N.nreac > a ? N.nreac : a;
-    const int b = b0 > g ? b0 : g;
+    const int b1 = b0 > g ? b0 : g;
+    const int b = b1 > 256 ? b1 : 256;               // >= one value per thread (ordered_sum)
     if (!N.nw_stage) return b > Npad ? b : Npad;               // unstaged plan: the work vector only
     const int t64 = N.nw_T * N.nw_T + ((N.nlp_src + 7) & ~7) + Npad;   // FP64: staged tail + Schur sources
     const int s32 = (solve_stage_bytes(N, 4) + 7) / 8, s64 = (solve_stage_bytes(N, 8) + 7) / 8;
@@ -360,6 +361,167 @@ __device__ inline double next_target_dev(int model, double t)
     }
     if (t > 0.0) return 10 * t * SECONDS_PER_YEAR;
     return SECONDS_PER_YEAR * 1.0e-7;
+}
+
+// ---------------------------------------------------------------- collapse models
+// The parameterised prestellar-core collapses of collapse.f90 (Priestley et al. 2018, AJ 156,
+// 51): mode 1 Bonnor-Ebert sphere with overdensity 1.1, 2 with overdensity 4, 3 magnetised
+// filament, 4 ambipolar-diffusion cloud.  Literals the reference writes as default-kind reals
+// are float32 values (F4), as its compiled code holds them.
+#define F4(x) ((double)(x##f))
+constexpr double MH = 1.67262164e-24;   // constants.f90:7
+constexpr double AU_PER_PC = 2.063e5;   // constants.f90:12 ("au": parsecs to AU)
+
+__device__ inline double collapse_next_target(double t)
+{   // collapse.f90:65-77 (years in, seconds out)
+    if (t > 10000) return (t + 1000.0) * SECONDS_PER_YEAR;
+    if (t > 1000) return (t + 100.0) * SECONDS_PER_YEAR;
+    if (t > 0.0) return (t * 10) * SECONDS_PER_YEAR;
+    return 3.16e7 * 10.e-8;
+}
+
+struct CollapseFit {
+    int mode;        // 1..4
+    double maxt;     // maxTime [yr] of modes 1 and 2 (collapse.f90:35-40)
+
+    __device__ static double unit_t()
+    {   // (2 pi G rho0)^-1/2 in years (collapse.f90:210-211)
+        return pow(2 * PI_F * 6.67e-8 * 2.2e4 * MH, -0.5) / SECONDS_PER_YEAR;
+    }
+    __device__ static double sound_speed() { return sqrt(1.38e-16 * 10 / 2 / MH); }
+    __device__ static double unit_r()
+    {   // c_s (2 pi G rho0)^-1/2 in pc (collapse.f90:189-190)
+        return sound_speed() * pow(2 * PI_F * 6.67e-8 * 2.2e4 * MH, -0.5) / PC;
+    }
+    // central density, radius parameter and slope against time in years (collapse.f90:201-264)
+    __device__ double rho0(double t) const
+    {
+        double l;
+        switch (mode) {
+        case 1: l = F4(61.8) * pow(maxt - t, F4(-0.01)) - F4(49.4); break;
+        case 2: l = F4(68.4) * pow(maxt - t, F4(-0.01)) - F4(55.7); break;
+        case 3: l = F4(3.54) * pow(F4(5.47) - t / unit_t(), F4(-0.15)) - F4(2.73); break;
+        default:
+            if (t <= 6.0) return 2.0e3 + 1.7e3 * (t / 6.0 - 1.0);
+            l = F4(5.3) * pow(F4(16.138) - 1e-6 * t, F4(-0.1)) - 1.0;
+        }
+        return pow(10.0, l);
+    }
+    __device__ double r0(double t) const
+    {
+        double l;
+        switch (mode) {
+        case 1: l = F4(-28.5) * pow(maxt - t, F4(-0.01)) + F4(28.93); break;
+        case 2: l = F4(-39.0) * pow(maxt - t, F4(-0.01)) + F4(38.7); break;
+        case 3: l = F4(-1.34) * pow(F4(5.47) - t / unit_t(), F4(-0.15)) + F4(1.47); break;
+        default: l = F4(-2.57) * pow(F4(16.138) - 1e-6 * t, F4(-0.1)) + F4(1.85);
+        }
+        return pow(10.0, l);
+    }
+    __device__ double slope(double t) const
+    {
+        switch (mode) {
+        case 1: return 2.4;
+        case 2: return F4(1.9) + 0.5 * exp(-t / 1e5);
+        case 3: {   // x**9 by repeated squaring
+            const double x = t / unit_t() / F4(5.47), x2 = x * x, x4 = x2 * x2;
+            return 2.0 - 0.5 * (x4 * x4 * x);
+        }
+        default: {  // x**40 = x^32 x^8
+            const double x = 1e-6 * t / F4(16.138), x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+            return F4(2.4) - F4(0.2) * (x16 * x16 * x8);
+        }
+        }
+    }
+    // density at radius r [pc] for the profile (rho0, r0, a) (collapse.f90:178-198)
+    __device__ double rho(double r, double rho0v, double r0v, double a) const
+    {
+        switch (mode) {
+        case 1:
+        case 2: return rho0v / (1 + pow(r * AU_PER_PC / r0v, a));
+        case 3: {
+            const double x = r / unit_r() / r0v;
+            return 2.2e4 * rho0v / pow(1 + x * x, a);
+        }
+        default: return rho0v / (1 + pow(r / 7.5e-1 / r0v, a));
+        }
+    }
+    // radial velocity [cm/s] of modes 3 and 4 and its time fits (collapse.f90:266-383)
+    __device__ double vr(double r, double rmin, double vmin, double a) const
+    {
+        if (mode == 3) {
+            const double nr = r / unit_r() - rmin;
+            double v;
+            if (nr < 0.0) {
+                const double x = nr / rmin;
+                v = vmin * (x * x - 1);
+            } else {
+                v = vmin * (exp(-2.0 * a * nr) - 2 * exp(-a * nr));
+            }
+            return sound_speed() * v;
+        }
+        const double rmid = 0.5, r75 = r / 7.5e-1, nr = r75 - rmin;
+        double v;
+        if (r75 < rmin) {
+            const double x = nr / rmin;
+            v = vmin * (x * x - 1);
+        } else if (r75 <= rmid) {
+            v = (vmin - a) * pow(nr / (rmid - rmin), F4(0.3)) - vmin;
+        } else {
+            v = a / (1.0 - rmid) * (r75 - rmid) - a;
+        }
+        return 1e3 * v;
+    }
+    __device__ double rmin(double t) const
+    {
+        if (mode == 3) {
+            const double x = t / unit_t();
+            if (x == 0.0) return 7.2;
+            const double lx = log(x);
+            if (lx < 1.6) return F4(-1.149) * x + F4(7.2);
+            if (lx < 1.674) return F4(-9.2) * lx + F4(16.25);
+            return -22.0 * lx + F4(37.65);
+        }
+        const double t6 = 1e-6 * t;
+        if (t6 <= F4(10.2)) return F4(-0.0039) * t6 + F4(0.49);
+        if (t6 <= F4(15.1)) return F4(-0.0306) * (t6 - F4(10.2)) + F4(0.45);
+        return F4(-0.282) * (t6 - F4(15.1)) + F4(0.3);
+    }
+    __device__ double vmin(double t) const
+    {
+        if (mode == 3) {
+            const double x = t / unit_t();
+            if (x == 0.0) return 0.0;
+            const double lx = log(x);
+            if (lx < 1.6) return F4(0.0891) * x;
+            if (lx < 1.674) return 5.5 * lx - F4(8.37);
+            return F4(18.9) * lx - F4(30.8);
+        }
+        const double t6 = 1e-6 * t;
+        return F4(3.44) * pow(F4(16.138) - t6, F4(-0.35)) - F4(0.7);
+    }
+    __device__ double vslope(double t) const
+    {
+        if (mode == 3) {
+            const double x = t / unit_t();
+            if (x == 0.0) return 0.4;
+            const double lx = log(x);
+            if (lx < 1.6) return F4(0.0101) * x + F4(0.4);
+            if (lx < 1.674) return F4(0.695) * lx - F4(0.663);
+            return F4(2.69) * lx - 4.0;
+        }
+        const double t6 = 1e-6 * t;
+        if (t6 <= F4(10.2)) return F4(0.143) * t6;
+        return F4(0.217) * (t6 - F4(10.2)) + F4(1.46);
+    }
+};
+
+__device__ inline CollapseFit collapse_fit(int mode)
+{   // collapse.f90:33-46 (modes 3 and 4 have no maxTime)
+    CollapseFit f;
+    f.mode = mode;
+    f.maxt = mode == 1 ? 1.175e6 : mode == 2 ? 1.855e5 : 0.0;
+    return f;
 }
 
 }  // namespace ucg

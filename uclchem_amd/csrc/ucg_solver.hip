@@ -21,7 +21,7 @@ __constant__ PhotoTables c_photo;
 
 // LDS scalar slots
 enum { S_SM, S_SBK, S_BLR, S_TS, S_SPRE, S_SCAT, S_H2COL, S_COCOL, S_CCOL, S_PHI, S_DENS, S_TGAS, S_TDUST,
-       S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_H2CRP, S_NSCAL = 24 };
+       S_AV, S_CS, S_COLDENS, S_ABSFAC, S_FLAG, S_H2CRP, S_RAD, S_MASS, S_NSCAL = 24 };
 
 // Large collective routines can be kept out of line to bound the kernel's register
 // allocation (UCG_BIG=__attribute__((noinline))); inlined by default.
@@ -2092,6 +2092,12 @@ struct SolveArgs {
     const int *q_closed;      // nonzero once the host admits no more requests
     int *q_done;              // [n]: set to 1 after request i's outputs are written
     long long q_idle_ticks;   // a workgroup that waits this long (100 MHz ticks) for a request leaves
+    // shared queue over several networks (null for one network): request i runs on network
+    // q_net[i] (host-written with the request), whose descriptor is q_nets[q_net[i]]; outputs
+    // are y_stride apart (the largest NEQ of the networks)
+    const DevNet *const *q_nets;
+    const int *q_net;
+    int y_stride;             // y_out row stride, 0: the network's NEQ
 };
 
 __device__ void elemental_start(const DevNet &N, const LDS double *prm, LDS double *y, double density)
@@ -2208,6 +2214,115 @@ __global__ void solve_init_kernel(int *counter, int *status, int n)
     if (i < n) status[i] = STATUS_CANCELLED;
 }
 
+// Sum of term(i), i = 1..n, added in increasing i as the reference's sequential loops add
+// them (collapse.f90:140-176).  With `stop_at`, the sum ends after the first i at which it
+// is no longer below `stop` and *hit = i (0 if it never gets there).  The terms are
+// evaluated NT at a time into `buf` (NT doubles of LDS); every thread adds them in the same
+// order, so every thread returns the same sum and hit.
+template <class Term>
+__device__ double ordered_sum(const Term &term, int n, double stop, bool stop_at, int *hit, LDS double *buf)
+{
+    double s = 0.0;
+    int h = 0;
+    for (int base = 0; base < n && !h; base += NT) {
+        const int i = base + tid() + 1;
+        __syncthreads();   // the previous chunk has been read
+        buf[tid()] = i <= n ? term(i) : 0.0;
+        __syncthreads();
+        const int m = n - base < NT ? n - base : NT;
+        for (int j = 0; j < m; j++) {
+            s = s + buf[j];
+            if (stop_at && !(s < stop)) {
+                h = base + j + 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    *hit = h;
+    return s;
+}
+
+// Steps of findNewRadius's outward search (dr = rout/1e4): the reference searches without a
+// bound; a parcel this far out (400 rout) has no meaning in these profiles.
+constexpr int COLLAPSE_MAX_STEPS = 4000000;
+
+// collapse.f90:27-59 initializePhysics for one parcel (points = 1): the parcel sits at rout,
+// the density is the profile's at rin, and modes 1 and 2 record the mass inside the parcel
+// (findMassInRadius, 1000 trapezoids).  Called by every thread.
+__device__ __attribute__((noinline)) void collapse_setup(const LDS double *prm, Lds &L)
+{
+    const CollapseFit F = collapse_fit((int)prm[P_COLLAPSE]);
+    const double rho0 = F.rho0(0.0), r0 = F.r0(0.0), a = F.slope(0.0);
+    const double rad = 1 * prm[P_ROUT] / (double)1.0f;
+    double mass = 0.0;
+    if (F.mode <= 2) {
+        const double dr = rad / 1000;
+        int hit;
+        mass = ordered_sum(
+            [&](int i) {
+                const double x = i * dr;
+                return 0.5 * (F.rho(x, rho0, r0, a) + F.rho((i - 1) * dr, rho0, r0, a)) * dr * (x * x);
+            },
+            1000, 0.0, false, &hit, L.flux);
+    }
+    if (tid() == 0) {
+        L.sc[S_RAD] = rad;
+        L.sc[S_MASS] = mass;
+        L.sc[S_DENS] = F.rho(prm[P_RIN], rho0, r0, a);
+    }
+    __syncthreads();
+}
+
+// collapse.f90:81-107 updatePhysics at `ty` years, after the physics core's update: the column
+// to the cloud edge through the current profile (findcoldens, 10000 trapezoids) sets Av; modes
+// 1 and 2 move the parcel to the radius that encloses its mass (findNewRadius), modes 3 and 4
+// by the fitted radial velocity over `dt` seconds (the part of the interval the integrator did
+// not cover: the reference takes targetTime - currentTime after the interval); the density
+// is the profile's at the new radius.  Called by every thread; both collapse routines stay out
+// of line so that they add no register pressure to the integration loop.
+__device__ __attribute__((noinline)) void collapse_update(const LDS double *prm, Lds &L, double ty, double dt)
+{
+    const CollapseFit F = collapse_fit((int)prm[P_COLLAPSE]);
+    const double rho0 = F.rho0(ty), r0 = F.r0(ty), a = F.slope(ty);
+    const double rin = prm[P_RIN], rout = prm[P_ROUT];
+    int hit;
+    double cd = 0.0;
+    const double size = rout - rin;
+    if (size > 0.0) {
+        const double dr = size / 10000;
+        cd = ordered_sum(
+            [&](int i) {
+                const double r1 = rin + (i - 1) * dr, r2 = rin + i * dr;
+                return 0.5 * (F.rho(r2, rho0, r0, a) + F.rho(r1, rho0, r0, a)) * dr * PC;
+            },
+            10000, 0.0, false, &hit, L.flux);
+    }
+    double rad = L.sc[S_RAD];
+    if (F.mode <= 2) {
+        const double mass = L.sc[S_MASS];
+        if (0.0 < mass) {   // else the reference's search loop does not run and the radius stays
+            const double dr = rout / 1.0e4;
+            ordered_sum(
+                [&](int i) {
+                    const double x = i * dr;
+                    return 0.5 * (F.rho(x, rho0, r0, a) + F.rho((i - 1) * dr, rho0, r0, a)) * dr * (x * x);
+                },
+                COLLAPSE_MAX_STEPS, mass, true, &hit, L.flux);
+            rad = (hit ? hit : COLLAPSE_MAX_STEPS) * dr;
+        }
+    } else {
+        rad = rad + F.vr(rad, F.rmin(ty), F.vmin(ty), F.vslope(ty)) * dt / PC;
+    }
+    if (tid() == 0) {
+        L.sc[S_COLDENS] = cd;
+        L.sc[S_AV] = prm[P_BASEAV] + cd / 1.6e21;
+        L.sc[S_RAD] = rad;
+        L.sc[S_DENS] = F.rho(rad, rho0, r0, a);
+    }
+    __syncthreads();
+}
+
 // The checks the host API makes of a parameter vector, repeated on the device for callers of
 // the device-pointer entry points: 0, or the status of a point that is not integrated:
 // -1 (PARAMETER_READ_ERROR: a value the kernel cannot use, e.g. a tempindx that would index
@@ -2215,11 +2330,13 @@ __global__ void solve_init_kernel(int *counter, int *status, int n)
 // without cosmicRayAttenuation, physics-core.f90:60-64).
 __device__ inline int params_status(const LDS double *q)
 {
-    const double model = q[P_MODEL], ti = q[P_TEMPINDX], im = q[P_IONMODEL];
+    const double model = q[P_MODEL], ti = q[P_TEMPINDX], im = q[P_IONMODEL], cm = q[P_COLLAPSE];
     bool ok = q[P_DENS] > 0.0 && q[P_TEMP] > 0.0 && q[P_RELTOL] > 0.0 && q[P_ABSMIN] >= 0.0 &&
-              q[P_MXSTEP] >= 1.0 && q[P_ROUT] > q[P_RIN] && q[P_MAXSTEPS] >= 0.0 && (model == 0.0 || model == 1.0) &&
-              (im == 0.0 || im == 1.0) && q[P_OMEGA] != 1.0 && q[P_UVCREFF] != 0.0;
+              q[P_MXSTEP] >= 1.0 && q[P_ROUT] > q[P_RIN] && q[P_MAXSTEPS] >= 0.0 &&
+              (model == 0.0 || model == 1.0 || model == 2.0) && (im == 0.0 || im == 1.0) && q[P_OMEGA] != 1.0 &&
+              q[P_UVCREFF] != 0.0;
     if (model == 1.0) ok = ok && ti >= 1.0 && ti < 7.0;
+    if (model == 2.0) ok = ok && (cm == 1.0 || cm == 2.0 || cm == 3.0 || cm == 4.0);
     if (!ok) return -1;
     if (q[P_IMPROVEDH2CRP] != 0.0 && q[P_CRATTEN] == 0.0) return -2;
     return 0;
@@ -2228,13 +2345,9 @@ __device__ inline int params_status(const LDS double *q)
 template <typename LUT>
 __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(const DevNet *__restrict__ Np, SolveArgs A)
 {
-    const DevNet &N = *Np;   // tables in global memory: field loads stay scalar, nothing copied to scratch
     extern __shared__ double lds_raw[];
-    const int neq = N.neq;
-    Lds L;
     char *slot = (char *)A.ws + (size_t)blockIdx.x * A.ws_stride;
-    lds_layout(N, L, lds_raw, (double *)(slot + A.ws_mat_bytes + A.ws_jac_bytes));
-    __shared__ int s_point;
+    __shared__ int s_point, s_net;
     LUT *M = (LUT *)slot;
 
     for (;;) {
@@ -2256,11 +2369,22 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                 }
             }
             s_point = p;
+            // a shared queue's request names its network (written before the request was posted)
+            s_net = (A.q_net && p < A.n_points)
+                        ? __hip_atomic_load(A.q_net + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                        : 0;
         }
         __syncthreads();
-        const int pidx = s_point;
+        const int pidx = s_point, inet = s_net;
         __syncthreads();
         if (pidx >= A.n_points) break;   // every wave of the workgroup leaves here
+        // the point's network: tables in global memory (field loads stay scalar, nothing copied
+        // to scratch); its LDS layout fits the launch's LDS, sized for the largest network
+        const DevNet &N = A.q_nets ? *A.q_nets[inet] : *Np;
+        const int neq = N.neq;
+        const size_t ystride = A.y_stride ? (size_t)A.y_stride : (size_t)neq;
+        Lds L;
+        lds_layout(N, L, lds_raw, (double *)(slot + A.ws_mat_bytes + A.ws_jac_bytes));
 
         if (A.t_start && tid() == 0) A.t_start[pidx] = (double)wall_clock64();
         // ---- point set-up: parameters, physics (physics-core.f90:42-73, cloud.f90:20-31)
@@ -2299,6 +2423,8 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
             L.sc[S_H2CRP] = 0.0;   // set by the physics update (physics-core.f90:87,143-155)
         }
         __syncthreads();
+        // the collapse models replace the cloud's initializePhysics (wrap.f90:145-147)
+        if (!interval_mode && pstatus == 0 && (int)prm[P_MODEL] == 2) collapse_setup(prm, L);
         if (A.y_init) {
             for (int i = tid(); i < neq; i += NT) L.y[i] = A.y_init[(size_t)pidx * neq + i];
             __syncthreads();
@@ -2321,6 +2447,11 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         const int mxstep = (int)prm[P_MXSTEP];
         const double maxsteps = prm[P_MAXSTEPS];   // 0: unbounded
         const bool end_dens = prm[P_ENDATFINALDENS] != 0.0;
+        const bool freefall = prm[P_FREEFALL] != 0.0;
+        // collapse modes 1 and 2 end at 0.97 maxTime whatever finalTime says (collapse.f90:35-40)
+        const double final_time = (model == 2 && pstatus == 0 && prm[P_COLLAPSE] <= 2.0)
+                                      ? (double)0.97f * collapse_fit((int)prm[P_COLLAPSE]).maxt
+                                      : prm[P_FINALTIME];
         // ---- time loop (wrap.f90:657-690).  The reference loops forever when the
         // integrator cannot advance t at all; here such a point stops with
         // INT_TOO_MANY_FAILS_ERROR after 10 intervals without progress.
@@ -2330,14 +2461,16 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
         const uint64_t c_point0 = clock64();   // shader clock, for the effective frequency
         while (!flag && (interval_mode ? nint == 0
                                        : ((end_dens && L.sc[S_DENS] < prm[P_FINALDENS]) ||
-                                          (!end_dens && tyears < prm[P_FINALTIME])))) {
+                                          (!end_dens && tyears < final_time)))) {
             if (nint >= MAX_INTERVALS || stuck > 10) {
                 flag = -5;
                 break;
             }
             const double t_before = t;
             tyears = t / SECONDS_PER_YEAR;
-            double target = interval_mode ? A.t_target[pidx] : next_target_dev(model, tyears);
+            double target = interval_mode ? A.t_target[pidx]
+                            : model == 2  ? collapse_next_target(tyears)
+                                          : next_target_dev(model, tyears);
             const double orig = target;
             int loops = 0;
             while (t < target && loops < 10) {   // updateChemistry retry loop (chemistry.f90:164-223)
@@ -2345,6 +2478,9 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                     flag = -5;   // the caller's work bound (INT_TOO_MANY_FAILS_ERROR)
                     break;
                 }
+                // the density the physics set, unless it is integrated (chemistry.f90:166)
+                if (!freefall && tid() == 0) L.y[neq - 1] = L.sc[S_DENS];
+                __syncthreads();
                 const uint64_t tr0 = wall_clock64();
                 block_rates(N, L);
                 t_rates += wall_clock64() - tr0;
@@ -2400,7 +2536,7 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
             if (interval_mode) break;   // updateChemistry ends here; the physics update is the caller's
             stuck = (t > t_before) ? 0 : stuck + 1;
             tyears = target / SECONDS_PER_YEAR;
-            // physics update (physics-core.f90:75-88, hotcore.f90:78-90)
+            // physics update (physics-core.f90:75-88, then hotcore.f90:78-90 or collapse.f90:81-107)
             if (tid() == 0) {
                 const double cs = L.sc[S_CS];
                 L.sc[S_COLDENS] = cs / 1.0 * L.sc[S_DENS];
@@ -2424,12 +2560,13 @@ __global__ __launch_bounds__(NT, UCG_WAVES_PER_EU) void solve_points_kernel(cons
                 }
             }
             __syncthreads();
+            if (model == 2) collapse_update(prm, L, tyears, target - t);
             if (row < A.max_rec) record_row(N, L, A, pidx, row, tyears);
             row++;
         }
         if (A.max_rec > 0 && row > A.max_rec && flag == 0) flag = -6;   // NOT_ENOUGH_TIMEPOINTS_ERROR
         // ---- retire the point
-        for (int i = tid(); i < neq; i += NT) A.y_out[(size_t)pidx * neq + i] = L.y[i];
+        for (int i = tid(); i < neq; i += NT) A.y_out[(size_t)pidx * ystride + i] = L.y[i];
         if (tid() == 0) {
             A.status[pidx] = flag;
             if (A.phys_out) {

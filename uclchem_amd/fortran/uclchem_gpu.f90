@@ -30,7 +30,7 @@ MODULE uclchem_gpu
         UCGP_EBMAXCR = 44, UCGP_EBMAXUVCR = 45, UCGP_EPSILON = 46, UCGP_UV_YIELD = 47, UCGP_PHI = 48, &
         UCGP_UVCREFF = 49, UCGP_OMEGA = 50, UCGP_DESORB = 51, UCGP_H2DESORB = 52, UCGP_CRDESORB = 53, &
         UCGP_UVDESORB = 54, UCGP_THERMDESORB = 55, UCGP_CRATTENUATION = 56, UCGP_IONMODEL = 57, &
-        UCGP_IMPROVEDH2CRP = 58, UCGP_ENFORCECHARGE = 59
+        UCGP_IMPROVEDH2CRP = 58, UCGP_ENFORCECHARGE = 59, UCGP_COLLAPSE = 60
 
     INTERFACE
         INTEGER(C_INT) FUNCTION ucg_init(device) BIND(C, name="ucg_init")

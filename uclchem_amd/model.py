@@ -9,6 +9,9 @@ reference's own layouts:
 
 * ``return_array=False``: per point, ``[success_flag] + abundances of out_species``
   (``_format_output``, ``model.py:71-76``);
+The physics drivers are the reference's cloud, hot core (``hot_core_grid``) and collapse
+(``collapse_grid``: the BE1.1, BE4, filament and ambipolar models of ``collapse.f90``).
+
 * ``return_array=True``: ``(physicsArray, chemicalAbunArray, ratesArray, abundanceStart,
   success_flags)`` with ``physicsArray[timepoints+1, points, 8]`` (Time, Density, gasTemp,
   dustTemp, Av, radfield, zeta, dstep; ``io.f90:74-82``) and
@@ -167,6 +170,24 @@ def hot_core_grid(temp_indx: int, max_temperature: float, param_dicts: Sequence[
         raise ValueError("temp_indx must be 1..6 (hotcore.f90:21-22)")
     return _run(param_dicts, out_species, return_array, starting_chemistry, timepoints, network_name, lu_bits,
                 dict(model=1, tempindx=int(temp_indx), maxtemp=float(max_temperature)), return_rates)
+
+
+COLLAPSE_MODELS = {"BE1.1": 1, "BE4": 2, "filament": 3, "ambipolar": 4}   # model.py:355
+
+
+def collapse_grid(collapse: str, param_dicts: Sequence[Dict], out_species: Optional[List[str]] = None,
+                  return_array: bool = False, starting_chemistry=None, timepoints: int = TIMEPOINTS,
+                  network_name: str = "default", lu_bits: int = 64, return_rates: bool = False):
+    """``uclchem.model.collapse(collapse, physics_output, ...)`` (model.py:319-426) for every dict
+    in one launch: `collapse` is 'BE1.1', 'BE4', 'filament' or 'ambipolar', the parameterised
+    prestellar-core collapses of collapse.f90.  BE1.1 and BE4 end at 0.97 of their fitted
+    collapse time (1.14 Myr and 0.18 Myr, 1144 and 184 output rows) whatever finalTime says, so
+    `timepoints` must be at least that to return every row (else status -6, as the
+    reference's).  The reference's physics_output file is not written."""
+    if collapse not in COLLAPSE_MODELS:
+        raise ValueError("collapse must be one of 'BE1.1', 'BE4', 'filament', or 'ambipolar'")
+    return _run(param_dicts, out_species, return_array, starting_chemistry, timepoints, network_name, lu_bits,
+                dict(model=2, collapse=COLLAPSE_MODELS[collapse]), return_rates)
 
 
 def _initial_phys(params: np.ndarray) -> np.ndarray:

@@ -22,7 +22,9 @@ NAMES = ["initialdens", "initialtemp", "zeta", "radfield", "baseav", "rout", "ri
          "ebmaxh2", "ebmaxcr", "ebmaxuvcr", "epsilon", "uv_yield", "phi", "uvcreff", "omega",
          # behavioural switches (defaultparameters.f90:40-49)
          "desorb", "h2desorb", "crdesorb", "uvdesorb", "thermdesorb", "cosmicrayattenuation", "ionmodel",
-         "improvedh2crpdissociation", "enforcechargeconservation"]
+         "improvedh2crpdissociation", "enforcechargeconservation",
+         # the collapse model of model 2 (collapse.f90:1-7): 1 BE1.1, 2 BE4, 3 filament, 4 ambipolar
+         "collapse"]
 INDEX = {n: i for i, n in enumerate(NAMES)}
 # the first slots, in the layout files written before the elemental and desorption slots existed
 NPARAM_V1 = 24
@@ -40,7 +42,7 @@ DEFAULTS: Dict[str, float] = dict(
     ebmaxh2=1.21e3, ebmaxcr=1.21e3, ebmaxuvcr=1.0e4, epsilon=_f32(0.01), uv_yield=_f32(0.03), phi=1.0e5,
     uvcreff=1.0e-3, omega=0.5,
     desorb=1.0, h2desorb=1.0, crdesorb=1.0, uvdesorb=1.0, thermdesorb=1.0, cosmicrayattenuation=0.0,
-    ionmodel=0.0, improvedh2crpdissociation=0.0, enforcechargeconservation=0.0)
+    ionmodel=0.0, improvedh2crpdissociation=0.0, enforcechargeconservation=0.0, collapse=0.0)
 
 ALIASES = {"dens": "initialdens", "temp": "initialtemp", "t_end": "finaltime"}
 # dictionaryParser quirks kept so a parameter dict means what it means to the reference:

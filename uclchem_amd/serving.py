@@ -8,17 +8,22 @@ that takes each posted request as soon as one of its workgroups is free.  Reques
 arrive while others are running therefore start without waiting for a batch boundary,
 and the work of all networks overlaps on the device.
 
-Admission by HBM budget: each network's slot count is its share of the CUs, cut so that
-the workspaces of all open queues (slots x per-slot bytes, ``ucg_queue_slot_bytes``) stay
-within the budget; requests beyond the free slots wait inside the device queue (posted,
-not yet taken).  Network tables and workspaces stay resident for the life of the server.
+By default the networks share one queue (``ucg_queue_open_shared``): its resident workgroups
+take the next posted request whatever its network, so capacity follows the load -- a burst
+on one network runs on every slot while the others are quiet.  With ``shared=False`` each
+network keeps a queue of its own with a fixed share of the CUs.
+
+Admission by HBM budget: the slot count is cut so that the workspaces (slots x per-slot
+bytes, ``ucg_queue_slot_bytes``) stay within the budget; requests beyond the free slots wait
+inside the device queue (posted, not yet taken).  Network tables and workspaces stay
+resident for the life of the server.
 """
 from __future__ import annotations
 
 import ctypes
 import time
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Union
 
 import numpy as np
 
@@ -64,10 +69,68 @@ class RequestQueue:
                                       _p(stats)), "ucg_queue_result")
         return y, st.value, stats
 
+    def times(self, request: int):
+        """(taken, finished) of a finished request, seconds of the device wall clock."""
+        return _queue_times(self.id, request)
+
     def close(self) -> None:
         if self.open:
             self.open = False
             _check(lib().ucg_queue_close(ctypes.c_int(self.id)), "ucg_queue_close")
+
+
+class SharedQueue:
+    """ctypes wrapper of one ucg_queue shared by several loaded networks: requests name their
+    network by its index in `nets`."""
+
+    def __init__(self, nets: List[Network], capacity: int, n_slots: int, lu_bits: int = 64,
+                 idle_timeout_s: float = 600.0):
+        self.nets = list(nets)
+        handles = (ctypes.c_int * len(self.nets))(*[n.handle for n in self.nets])
+        q = ctypes.c_int(-1)
+        _check(lib().ucg_queue_open_shared(handles, ctypes.c_int(len(self.nets)), ctypes.c_int(capacity),
+                                           ctypes.c_int(n_slots), ctypes.c_int(lu_bits),
+                                           ctypes.c_double(idle_timeout_s), ctypes.byref(q)),
+               "ucg_queue_open_shared")
+        self.id = q.value
+        self.open = True
+        self._neq_max = max(n.neq for n in self.nets)
+
+    def submit(self, params: np.ndarray, network: int) -> int:
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(NPARAM)
+        r = ctypes.c_int(-1)
+        _check(lib().ucg_queue_submit_to(ctypes.c_int(self.id), ctypes.c_int(network), _p(p), ctypes.byref(r)),
+               "ucg_queue_submit_to")
+        return r.value
+
+    def done(self, request: int) -> bool:
+        d = ctypes.c_int(0)
+        _check(lib().ucg_queue_test(ctypes.c_int(self.id), ctypes.c_int(request), ctypes.byref(d)), "ucg_queue_test")
+        return bool(d.value)
+
+    def result(self, request: int, network: int):
+        y = np.zeros(self._neq_max)          # the library writes the request's own network's NEQ
+        st = ctypes.c_int(0)
+        stats = np.zeros(8, dtype=np.int32)
+        _check(lib().ucg_queue_result(ctypes.c_int(self.id), ctypes.c_int(request), _p(y), ctypes.byref(st),
+                                      _p(stats)), "ucg_queue_result")
+        return y[:self.nets[network].neq].copy(), st.value, stats
+
+    def times(self, request: int):
+        """(taken, finished) of a finished request, seconds of the device wall clock."""
+        return _queue_times(self.id, request)
+
+    def close(self) -> None:
+        if self.open:
+            self.open = False
+            _check(lib().ucg_queue_close(ctypes.c_int(self.id)), "ucg_queue_close")
+
+
+def _queue_times(qid: int, request: int):
+    a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    _check(lib().ucg_queue_times(ctypes.c_int(qid), ctypes.c_int(request), ctypes.byref(a), ctypes.byref(b)),
+           "ucg_queue_times")
+    return a.value, b.value
 
 
 @dataclass
@@ -83,23 +146,36 @@ class Ticket:
 
 @dataclass
 class Server:
-    """Resident networks with one open request queue each.
+    """Resident networks served by one shared request queue (default) or one queue each.
 
-    slots: {network: resident workgroups}; chosen by :func:`plan_slots` from the CU count
-    and the HBM budget when not given."""
+    slots: resident workgroups -- an int, or {network: slots} (summed for a shared queue, the
+    per-network shares otherwise), e.g. from :func:`plan_shared_slots` / :func:`plan_slots`;
+    capacity: requests the server will take (per network without sharing)."""
     networks: Dict[str, Network]
-    slots: Dict[str, int]
+    slots: Union[Dict[str, int], int]
     capacity: int = 4096
     lu_bits: int = 64
+    shared: bool = True
     queues: Dict[str, RequestQueue] = field(default_factory=dict)
     pending: List[Ticket] = field(default_factory=list)
+    shared_queue: Optional[SharedQueue] = None
 
     def __post_init__(self):
-        for name, net in self.networks.items():
-            self.queues[name] = RequestQueue(net, self.capacity, self.slots[name], self.lu_bits)
+        names = list(self.networks)
+        if self.shared:
+            n = self.slots if isinstance(self.slots, int) else sum(self.slots.values())
+            self._index = {k: i for i, k in enumerate(names)}
+            self.shared_queue = SharedQueue([self.networks[k] for k in names], self.capacity, int(n), self.lu_bits)
+        else:
+            for name in names:
+                self.queues[name] = RequestQueue(self.networks[name], self.capacity, self.slots[name], self.lu_bits)
 
     def submit(self, network: str, params: np.ndarray, t_arrival: Optional[float] = None) -> Ticket:
-        t = Ticket(network, self.queues[network].submit(params), time.perf_counter() if t_arrival is None else t_arrival)
+        if self.shared_queue is not None:
+            r = self.shared_queue.submit(params, self._index[network])
+        else:
+            r = self.queues[network].submit(params)
+        t = Ticket(network, r, time.perf_counter() if t_arrival is None else t_arrival)
         self.pending.append(t)
         return t
 
@@ -108,10 +184,13 @@ class Server:
         out, keep = [], []
         now = time.perf_counter()
         for t in self.pending:
-            q = self.queues[t.network]
+            q = self.shared_queue if self.shared_queue is not None else self.queues[t.network]
             if q.done(t.request):
                 t.t_done = now
-                t.y, t.status, t.stats = q.result(t.request)
+                if self.shared_queue is not None:
+                    t.y, t.status, t.stats = q.result(t.request, self._index[t.network])
+                else:
+                    t.y, t.status, t.stats = q.result(t.request)
                 out.append(t)
             else:
                 keep.append(t)
@@ -119,8 +198,21 @@ class Server:
         return out
 
     def close(self) -> None:
+        if self.shared_queue is not None:
+            self.shared_queue.close()
         for q in self.queues.values():
             q.close()
+
+
+def plan_shared_slots(networks: Dict[str, Network], n_cu: int, hbm_budget_bytes: float, lu_bits: int = 64) -> int:
+    """Resident workgroups of a shared queue: the CUs less four (left to other work), cut so
+    that the workspaces fit the HBM budget.  A shared slot holds the largest network's
+    Newton matrix, J-form values and rates; the sum of the networks' slot sizes bounds it."""
+    per = sum(RequestQueue.slot_bytes(net, lu_bits) for net in networks.values())
+    n = min(max(1, n_cu - 4), int(hbm_budget_bytes // per))
+    if n < 1:
+        raise GpuError("HBM budget too small for one shared slot")
+    return n
 
 
 def plan_slots(networks: Dict[str, Network], n_cu: int, hbm_budget_bytes: float, lu_bits: int = 64,

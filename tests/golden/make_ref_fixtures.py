@@ -6,7 +6,9 @@ Run in the build container (needs /root/reference for the build and the example 
 Outputs (small .npz files next to this script):
   static_full_t1e6.npz   examples/example-output/static-full.dat row t=1e6 yr (6 digits)
   ref_cloud_point.npz    the reference model for the default static cloud point
-  ref_cloud_grid.npz     the reference model for the first N seeded grid points
+  ref_cloud_grid.npz     the reference model for points of the 4096-point grid, with its answers
+                         at tight, half and twice its default tolerances (--grid-tight,
+                         --grid-variants)
   ref_hot_core.npz       the reference model for seeded hot-core points
   example_collapse_hotcore.npz  examples/example-output/phase1-full.dat, startcollapse.dat,
                          phase2-full.dat (the collapse and hot-core example models)
@@ -269,6 +271,37 @@ def grid_tight(procs, cap, name="ref_cloud_grid.npz", net=None, only=None):
     np.savez_compressed(path, **f)
 
 
+def grid_variants(procs, cap, name="ref_cloud_grid.npz"):
+    """Add to ref_cloud_grid.npz the reference's answers at half and at twice its relative
+    tolerance (final_half, final_double; NaN where a run exceeds `cap` seconds or ends at
+    another time).  With final_tight they measure how far the reference's own answer moves
+    under a small change of tolerance: on the grid's chaotic points (integrator calls that run
+    out of steps, so that the rates are recomputed mid-interval at places that depend on the
+    step sequence, chemistry.f90:164-221) it moves by far more than 1%."""
+    from uclchem_amd.params import INDEX, widen
+    path = os.path.join(HERE, name)
+    f = dict(np.load(path))
+    params = widen(f["params"])
+    jobs = []
+    for fac in (0.5, 2.0):
+        for q in range(len(params)):
+            p = params[q].copy()
+            p[INDEX["reltol"]] *= fac
+            jobs.append((p, cap))
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_model_capped, jobs, chunksize=1)
+    n = len(params)
+    for k, key in enumerate(("final_half", "final_double")):
+        out = np.full_like(f["final"], np.nan)
+        for q in range(n):
+            r = res[k * n + q]
+            if r is not None and r[0] == 0 and r[1][0] == f["final"][q, 0]:
+                out[q] = r[1]
+        f[key] = out
+        print(f"{key}: {int(np.isfinite(out[:, 0]).sum())} of {n}", flush=True)
+    np.savez_compressed(path, **f)
+
+
 def coeff_changes(tables):
     """The coefficient changes of ref_coeff.npz: H3+ + CO (alpha, beta), the first CRP
     reaction (alpha), the first LH and LHDES reactions (gamma); 1-based indices."""
@@ -393,12 +426,17 @@ def main():
     ap.add_argument("--only", default=None, help="comma-separated fixture rows to (re)compute with --*-tight")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     ap.add_argument("--collapse-only", action="store_true", help="only the collapse-module models")
+    ap.add_argument("--grid-variants", action="store_true",
+                    help="add the answers at half and twice the relative tolerance to ref_cloud_grid.npz")
     ap.add_argument("--ref-lib", default=os.path.join(ROOT, "oracle", "_ref", "libucref.so"),
                     help="the reference build to run (--collapse-only)")
     a = ap.parse_args()
     only = None if a.only is None else [int(x) for x in a.only.split(",")]
     if a.collapse_only:
         collapse_fixture(a.procs, a.cap, a.ref_lib)
+        return
+    if a.grid_variants:
+        grid_variants(a.procs, a.cap)
         return
     if a.grid_tight:
         grid_tight(a.procs, a.cap, only=only)

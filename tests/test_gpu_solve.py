@@ -58,33 +58,55 @@ def _ref_grid(name):
 @pytest.mark.parametrize("lu_bits", [64, 32])
 def test_grid_matches_reference(gpu_net, tables, lu_bits):
     """Points of the benchmarked 4096-point static-cloud grid (one per initial-density
-    stratum, tests/golden/ref_cloud_grid.npz) against the reference Fortran DVODE: the same
-    47 output rows to 1 Myr and every abundance above 1e-14 within 1% plus twice the
-    reference's own integration noise, |final - final_tight| (final_tight: the reference at
-    reltol 1e-11, abstol_factor 1e-17, abstol_min 1e-30).  The noise term matters on the
-    grid's stiff tail, where the reference's integrator exhausts MXSTEP and recomputes the
-    rates mid-interval at places that depend on its step count (chemistry.f90:164-221); those
-    stretches are pinned call by call in test_gpu_intervals."""
+    stratum, tests/golden/ref_cloud_grid.npz) against the reference Fortran DVODE, the same 47
+    output rows to 1 Myr.  The fixture also holds the reference's own answers at tight
+    (reltol 1e-11, abstol_factor 1e-17, abstol_min 1e-30), half and twice its default
+    tolerances; their largest distance from its default answer, per species, is its own
+    integration noise N.  Every abundance above 1e-14 agrees within 1% plus 2N.  On the grid's
+    chaotic points (the reference's own answers at these neighbouring tolerances differ from
+    its default answer by more than 1%: integrator calls run out of steps and the rates are
+    recomputed mid-interval at places that depend on the step sequence, chemistry.f90:164-221)
+    the species-wise noise from three answers is no bound, so there the GPU's answer must be
+    no farther from the reference's default answer than the reference's own answers are (the
+    1%-excess metric over all species); those stretches are pinned call by call from the
+    reference's own states in test_gpu_intervals."""
     ref = _ref_grid("ref_cloud_grid.npz")
-    if "final_tight" not in ref.files:
-        pytest.skip("ref_cloud_grid.npz has no tight-tolerance answers (make_ref_fixtures.py --grid-tight)")
+    for key in ("final_tight", "final_half", "final_double"):
+        if key not in ref.files:
+            pytest.skip(f"ref_cloud_grid.npz has no {key} (make_ref_fixtures.py --grid-tight / --grid-variants)")
     from uclchem_amd import workloads
     from uclchem_amd.params import widen
     params = widen(ref["params"])
     assert np.array_equal(params, workloads.static_cloud_grid(4096)[ref["index"]])
     out = gpu_net.solve(params, lu_bits=lu_bits, max_rec=80, order="densest")
     from helpers import excess_over_reference
-    bad, noisy = [], 0
+    bad, chaotic = [], []
     for q in range(len(params)):
         assert ref["rc"][q] == 0 and ref["nrows"][q] == 47, (q, ref["rc"][q], ref["nrows"][q])
-        assert out["status"][q] == 0 and out["n_rec"][q] == 47, (q, out["status"][q], out["n_rec"][q])
-        y_ref, tight = ref["final"][q, 4:], ref["final_tight"][q, 4:]
-        noisy += int(excess_over_reference(tight, y_ref) > 1.0) if np.all(np.isfinite(tight)) else 0
-        e = excess_over_reference(out["y"][q, :tables.nspec], y_ref, tight)
+        n = int(out["n_rec"][q])
+        assert out["status"][q] == 0 and out["rec"][q, n - 1, 0] == ref["final"][q, 0], (q, out["status"][q])
+        y_ref = ref["final"][q, 4:]
+        var = [ref[k][q, 4:] for k in ("final_tight", "final_half", "final_double") if np.all(np.isfinite(ref[k][q]))]
+        assert var, q
+        y = out["y"][q, :tables.nspec]
+        spread = max(excess_over_reference(v, y_ref) for v in var)
+        # an interval whose ten integrator calls all run out of steps writes one more row
+        # (wrap.f90:657-690), as the reference's own answers at other tolerances do here
+        assert n == 47 or spread > 1.0, (q, n)
+        if spread > 1.0:
+            e = excess_over_reference(y, y_ref)
+            chaotic.append((int(ref["index"][q]), round(e, 2), round(spread, 2)))
+            if e > spread:
+                bad.append((int(ref["index"][q]), "chaotic", e, spread))
+            continue
+        noise = np.max([np.abs(y_ref - v) for v in var], axis=0)
+        m = y_ref > 1e-14
+        e = float((np.abs(y[m] - y_ref[m]) / (0.01 * y_ref[m] + 2.0 * noise[m])).max())
         if e > 1.0:
             bad.append((int(ref["index"][q]), e))
-    print(f"[grid] {noisy} of {len(params)} points where the reference's own noise exceeds 1%", flush=True)
+    print(f"[grid] chaotic points (index, GPU excess, reference's own spread): {chaotic}", flush=True)
     assert not bad, bad
+    assert len(chaotic) <= 3, chaotic
 
 
 @pytest.mark.parametrize("lu_bits", [64, 32])

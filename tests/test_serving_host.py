@@ -128,3 +128,31 @@ def test_plan_shared_slots_within_cus_and_budget(fake_sizes):
     assert serving.plan_shared_slots(fake_sizes, n_cu=256, hbm_budget_bytes=14_000.0) == 10   # 1400 B per slot
     with pytest.raises(GpuError):
         serving.plan_shared_slots(fake_sizes, n_cu=256, hbm_budget_bytes=1000.0)
+
+
+def test_stream_timeout_cancels_the_queues(monkeypatch):
+    """A stream that does not finish in time cancels the server's queues (so that closing them
+    does not wait for requests that may never end) before it reports the time-out."""
+    class _Never(_FakeShared):
+        def done(self, request):
+            return False
+
+    class _Net:
+        def __init__(self, h):
+            self.handle = h
+
+    cancelled = []
+
+    class _Lib:
+        def ucg_cancel(self, h):
+            cancelled.append(h.value)
+            return 0
+
+    monkeypatch.setattr(serving, "SharedQueue", _Never)
+    monkeypatch.setattr(serving, "lib", lambda: _Lib())
+    nets = {"a": _Net(3), "b": _Net(5)}
+    srv = serving.Server(nets, 4, capacity=8)
+    with pytest.raises(GpuError):
+        serving.run_stream(srv, serving.poisson_requests(3, 1000.0, ["a", "b"]), timeout_s=0.05)
+    assert cancelled == [3]          # the shared queue watches its first network's cancel word
+    srv.close()

@@ -16,7 +16,8 @@ MODULE uclchem_gpu
     PRIVATE
     PUBLIC :: ucg_init, ucg_network_load_file_f, ucg_network_info, ucg_solve_batch_f, ucg_error_message, &
               ucg_default_params, ucg_cancel, ucg_queue_slot_bytes, ucg_queue_open, ucg_queue_submit, &
-              ucg_queue_test, ucg_queue_result, ucg_queue_close
+              ucg_queue_test, ucg_queue_result, ucg_queue_close, ucg_queue_open_shared, ucg_queue_submit_to, &
+              ucg_queue_times
     INTEGER, PARAMETER, PUBLIC :: UCG_NPARAM = 64
     INTEGER, PARAMETER, PUBLIC :: UCG_E_ARG = -102   ! an array does not match the batch or network
     INTEGER, PARAMETER, PUBLIC :: UCGP_DENS = 1, UCGP_TEMP = 2, UCGP_ZETA = 3, UCGP_RADFIELD = 4, &
@@ -119,6 +120,30 @@ MODULE uclchem_gpu
             IMPORT :: C_INT
             INTEGER(C_INT), VALUE :: queue
         END FUNCTION ucg_queue_close
+
+        ! one queue shared by several networks (handles(n_networks)); requests name the
+        ! network by its 0-based position in handles
+        INTEGER(C_INT) FUNCTION ucg_queue_open_shared(handles, n_networks, capacity, n_slots, lu_bits, &
+                idle_timeout_s, queue) BIND(C, name="ucg_queue_open_shared")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), INTENT(IN) :: handles(*)
+            INTEGER(C_INT), VALUE :: n_networks, capacity, n_slots, lu_bits
+            REAL(C_DOUBLE), VALUE :: idle_timeout_s
+            INTEGER(C_INT), INTENT(OUT) :: queue
+        END FUNCTION ucg_queue_open_shared
+
+        INTEGER(C_INT) FUNCTION ucg_queue_submit_to(queue, network, params, request) BIND(C, name="ucg_queue_submit_to")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), VALUE :: queue, network
+            REAL(C_DOUBLE), INTENT(IN) :: params(*)
+            INTEGER(C_INT), INTENT(OUT) :: request
+        END FUNCTION ucg_queue_submit_to
+
+        INTEGER(C_INT) FUNCTION ucg_queue_times(queue, request, t_taken, t_done) BIND(C, name="ucg_queue_times")
+            IMPORT :: C_INT, C_DOUBLE
+            INTEGER(C_INT), VALUE :: queue, request
+            REAL(C_DOUBLE), INTENT(OUT) :: t_taken, t_done
+        END FUNCTION ucg_queue_times
 
         TYPE(C_PTR) FUNCTION c_last_error() BIND(C, name="ucg_last_error")
             IMPORT :: C_PTR

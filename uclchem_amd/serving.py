@@ -197,6 +197,13 @@ class Server:
         self.pending = keep
         return out
 
+    def cancel(self) -> None:
+        """Stops the server's queues (ucg_cancel on the network whose cancel word each queue
+        watches): running requests end with UCG_STATUS_CANCELLED, waiting ones are not run."""
+        nets = [self.shared_queue.nets[0]] if self.shared_queue is not None else [q.net for q in self.queues.values()]
+        for net in nets:
+            _check(lib().ucg_cancel(ctypes.c_int(net.handle)), "ucg_cancel")
+
     def close(self) -> None:
         if self.shared_queue is not None:
             self.shared_queue.close()
@@ -266,6 +273,7 @@ def run_stream(server: Server, requests, poll_interval_s: float = 2e-4, timeout_
             progress(len(done), i, now - t0)
             last = now
         if now - t0 > timeout_s:
+            server.cancel()   # so that closing the queues does not wait for the unfinished requests
             raise GpuError(f"stream not finished after {timeout_s} s ({len(done)}/{len(requests)} done)")
         time.sleep(poll_interval_s)
     return done

@@ -404,7 +404,7 @@ def bench_serving(a):
     import torch
     n_cu = torch.cuda.get_device_properties(0).multi_processor_count
     budget = 64e9                                   # HBM the serving state may take
-    slots = serving.plan_slots(nets, n_cu, budget, lu_bits=a.lu)
+    slots = serving.plan_shared_slots(nets, n_cu, budget, lu_bits=a.lu)   # one queue shared by the networks
     reqs = serving.poisson_requests(a.requests, a.rate, names)
     srv = serving.Server(nets, slots, capacity=a.requests + 8, lu_bits=a.lu)
     try:
@@ -426,7 +426,8 @@ def bench_serving(a):
         "dtype": "f64", "data": f"synthetic (seeded Poisson stream, lambda={a.rate}/s, {a.requests} requests)",
         "config": {"workload": "serving loop: Poisson single-point requests over the default network and the "
                                "150/1500 and 500/6000 synthetic networks, t_end log-uniform 1e3..1e7 yr, "
-                               "request queues with continuous admission, slots by HBM budget",
+                               "one request queue shared by the three networks with continuous admission, "
+                               "slots by CU count and HBM budget",
                    "slots": slots, "hbm_budget_gb": budget / 1e9, "lu": f"fp{a.lu}"},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "per_network_p50": {k: float(np.percentile(v, 50)) for k, v in by_net.items() if len(v)},

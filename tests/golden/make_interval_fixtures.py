@@ -211,15 +211,68 @@ def add_tight(procs, cap, only=None):
             print(f"row {q}: tight source {src}", flush=True)
 
 
+def split_multicall():
+    """Rows whose interval the reference ends in more than one integrator call (the first runs
+    out of MXSTEP, the next starts from where it stopped with the rates recomputed there,
+    chemistry.f90:164-221) are split into those calls: "<how> call 1" (the stretch the first
+    call covers, as the first-attempt rows) and "<how> call 2" (from the reference's own state
+    at that point to the interval's end).  Each of them is one call with frozen rates; the
+    whole row depends on where the step budget ran out and is kept for reference only."""
+    from oracle import RefLib
+    from uclchem_amd.params import widen
+    path = os.path.join(HERE, "ref_intervals.npz")
+    f = dict(np.load(path))
+    ref = RefLib()
+    params = widen(f["params"])
+    n0 = len(f["target"])
+    add = {k: [] for k in f}
+    for q in range(n0):
+        how = str(f["how"][q])
+        if f["attempts"][q] <= 1 or how == "first-attempt" or " call " in how or f["phys_ref"][q, 7] < f["target"][q]:
+            continue
+        if any(str(h) == how + " call 1" and f["idx"][r] == f["idx"][q] for r, h in enumerate(f["how"])):
+            continue
+        p, y, ph, t1 = params[q], f["y_in"][q], f["phys_in"][q], float(f["target"][q])
+        ta = time.time()
+        rc1, y1, ph1, st1 = ref.interval(p, y, ph, t1, maxsteps=float(p[INDEX["mxstep"]]))
+        w1 = time.time() - ta
+        tb = time.time()
+        rc2, y2, ph2, st2 = ref.interval(p, y1, ph1, t1)
+        w2 = time.time() - tb
+        # the two calls continue the reference's own interval bit for bit (the first stops at its step
+        # bound, -5)
+        assert rc1 in (0, -5) and rc2 == 0 and np.array_equal(y2, f["y_ref"][q]), (q, rc1, rc2)
+        for part, (yi, phi, tt, yo, pho, st, w) in enumerate([(y, ph, ph1[7], y1, ph1, st1, w1),
+                                                              (y1, ph1, t1, y2, ph2, st2, w2)]):
+            row = dict(kind=f["kind"][q], idx=f["idx"][q], how=f"{how} call {part + 1}", params=f["params"][q],
+                       y_in=yi, phys_in=phi, target=tt, y_ref=yo, phys_ref=pho, rc=0, steps=st[0],
+                       attempts=st[1], wall=w, y_tight=np.full_like(yo, np.nan), tight_src=0)
+            for k in add:
+                add[k].append(row[k])
+            print(f"row {q} ({f['kind'][q]} {f['idx'][q]} {how}) call {part + 1}: t {phi[7] / SPY:.6g} -> "
+                  f"{tt / SPY:.6g} yr, steps {st[0]}, attempts {st[1]}", flush=True)
+    for k in add:
+        if add[k]:
+            new = np.array(add[k])
+            dt = np.result_type(f[k].dtype, new.dtype) if f[k].dtype.kind in "US" else f[k].dtype
+            f[k] = np.concatenate([f[k].astype(dt), new.astype(dt)])
+    np.savez_compressed(path, **f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--tight-only", action="store_true", help="add y_tight to the existing fixture")
+    ap.add_argument("--split-multicall", action="store_true",
+                    help="add the single-call parts of rows the reference ends in several calls")
     ap.add_argument("--cap", type=float, default=1800.0, help="wall seconds per tight reference stretch")
     ap.add_argument("--only", default=None, help="comma-separated rows for --tight-only")
     ap.add_argument("--procs", type=int, default=6)
     ap.add_argument("--budget", type=float, default=2400.0, help="wall seconds per hot-core trajectory walk")
     ap.add_argument("--tail-budget", type=float, default=900.0, help="wall seconds per tail-point walk")
     a = ap.parse_args()
+    if a.split_multicall:
+        split_multicall()
+        return
     if a.tight_only:
         add_tight(a.procs, a.cap, None if a.only is None else [int(x) for x in a.only.split(",")])
         return

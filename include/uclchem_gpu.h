@@ -260,6 +260,12 @@ int ucg_rhs(int handle, int n, const double *params, const double *phys, const d
             const double *y, double *ydot, double *rate_out);
 int ucg_jacobian(int handle, int n, const double *params, const double *phys, const double *rate,
                  const double *y, double *jac);
+/* ucg_rhs that also returns flux [n][nreac], the per-reaction terms of GETYDOT at y (RATE(k)
+ * times its density and abundance factors and its type's bulk-layer, totalSwap or 1/SURFACE
+ * factor; the H2 and CO photo rates recomputed at y as in F): the quantity the reference's
+ * analysis.rates_to_dy_and_flux rebuilds from output rows (src/uclchem/analysis.py:737-800). */
+int ucg_rhs_fluxes(int handle, int n, const double *params, const double *phys, const double *rate,
+                   const double *y, double *ydot, double *flux);
 int ucg_newton_solve(int handle, int n, const double *params, const double *phys, const double *y,
                      double gamma, const double *b, double *x, int lu_bits);
 

@@ -175,6 +175,14 @@ class Oracle:
                          _ptr(np.ascontiguousarray(y, dtype=np.float64)), _ptr(ydot))
         return ydot, rate
 
+    def fluxes(self, pt, rate, y):
+        """GETYDOT's per-reaction terms at y (ucl_fluxes); returns (flux, rate)."""
+        rate = np.array(rate, dtype=np.float64)
+        flux = np.zeros(self.t.nreac)
+        self.lib.ucl_fluxes(ctypes.byref(self.net), ctypes.byref(pt), _ptr(rate),
+                            _ptr(np.ascontiguousarray(y, dtype=np.float64)), _ptr(flux))
+        return flux, rate
+
     def next_target(self, model, t_years):
         return self.lib.ucl_next_target(int(model), float(t_years))
 

@@ -617,6 +617,29 @@ static double term_value(const ucl_net *n, int k, const double *rate, const doub
     return v;
 }
 
+/* The per-reaction terms of GETYDOT (odes.f90) at y: flux[k] = RATE(k) times its density and
+ * abundance factors (and the bulk-layer, totalSwap and 1/SURFACE factors of its type), the
+ * quantity the reference's analysis.rates_to_dy_and_flux (analysis.py:737-800) rebuilds from
+ * output rows; H2 and CO photodissociation recomputed at y as in F (chemistry.f90:311-319). */
+int ucl_fluxes(const ucl_net *n, const ucl_point *pt, double *rate, const double *y, double *flux)
+{
+    const int neq = n->neq;
+    const double D = y[neq - 1];
+    const double radfield = pt->p[P_RADFIELD];
+    double half = pt->cloud_size / 1.0;
+    double cocol = 0.5 * y[n->nco] * D * half;
+    double h2col = 0.5 * y[n->nh2] * D * half;
+    rate[n->r_h2_hv] = ucl_h2_photodiss(h2col, radfield, pt->av, TURB_VEL);
+    rate[n->r_co_hv] = ucl_co_photodiss(h2col, cocol, radfield, pt->av);
+    double sm = fmax(1e-30, y[n->nsurface]);
+    double sbk = fmax(1e-30, y[n->nbulk]);
+    double blr = fmin(1.0, num_sites_per_grain() / (gas_dust_density_ratio() * sbk));
+    double total_swap = 0.0;
+    for (int q = 0; q < n->nswap; q++) total_swap += term_value(n, n->swap_list[q], rate, y, D, blr, 0.0, sm);
+    for (int k = 0; k < n->nreac; k++) flux[k] = term_value(n, k, rate, y, D, blr, total_swap, sm);
+    return 0;
+}
+
 int ucl_rhs(const ucl_net *n, const ucl_point *pt, double *rate, const double *y, double *ydot)
 {
     const int neq = n->neq, ns = n->nspec;

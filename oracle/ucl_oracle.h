@@ -72,6 +72,7 @@ typedef struct {
 int ucl_point_init(const ucl_net *net, const double *params, ucl_point *pt, double *y0);
 int ucl_rates(const ucl_net *net, ucl_point *pt, double *y, double *rate, double *scal);
 int ucl_rhs(const ucl_net *net, const ucl_point *pt, double *rate, const double *y, double *ydot);
+int ucl_fluxes(const ucl_net *net, const ucl_point *pt, double *rate, const double *y, double *flux);
 double ucl_next_target(int model, double time_in_years);           /* seconds */
 void ucl_update_physics(const ucl_net *net, ucl_point *pt, double current_time_s);
 

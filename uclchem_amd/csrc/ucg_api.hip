@@ -1273,6 +1273,36 @@ int ucg_rhs(int handle, int n, const double *params, const double *phys, const d
     return 0;
 }
 
+int ucg_rhs_fluxes(int handle, int n, const double *params, const double *phys, const double *rate, const double *y,
+                   double *ydot, double *flux)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    NetHandle *h = get(handle);
+    if (!h) return fail(UCG_E_HANDLE, "bad network handle");
+    if (!params || !phys || !rate || !y || !ydot || !flux) return fail(UCG_E_ARG, "null buffer");
+    const ucg::DevNet &N = h->dev;
+    ucg::PieceArgs A;
+    DevBuf dp, dph, dy, drin, dyo, dfl;
+    int rc = piece_common(*h, n, params, phys, A, dp, dph);
+    if (rc) return rc;
+    if ((rc = to_dev(dy, y, (size_t)n * N.neq))) return rc;
+    if ((rc = to_dev(drin, rate, (size_t)n * N.nreac))) return rc;
+    if ((rc = alloc_dev<double>(dyo, (size_t)n * N.neq))) return rc;
+    if ((rc = alloc_dev<double>(dfl, (size_t)n * N.nreac))) return rc;
+    A.y = (const double *)dy.p;
+    A.rate_in = (const double *)drin.p;
+    A.y_out = (double *)dyo.p;
+    A.flux_out = (double *)dfl.p;
+    size_t lds = lds_bytes(N);
+    HIPCHK(hipFuncSetAttribute((const void *)ucg::rhs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(ucg::rhs_kernel, dim3(n), dim3(ucg::NT), lds, 0, h->d_dev, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(ydot, dyo.p, (size_t)n * N.neq * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(flux, dfl.p, (size_t)n * N.nreac * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int ucg_jacobian(int handle, int n, const double *params, const double *phys, const double *rate, const double *y,
                  double *jac)
 {

@@ -2614,6 +2614,7 @@ struct PieceArgs {
     double *y_out;          // [n][neq] (rates: y after the ice reset; rhs: ydot)
     double *scal_out;       // [n][8]
     double *jac_out;        // [n][neq*neq] column-major, J (not M)
+    double *flux_out;       // [n][nreac] per-reaction terms of the RHS (rhs_kernel), or null
     double gamma;
     int sparse;
     void *ws;
@@ -2680,6 +2681,8 @@ __global__ __launch_bounds__(NT) void rhs_kernel(const DevNet *__restrict__ Np, 
     for (int i = tid(); i < N.neq; i += NT) A.y_out[(size_t)p * N.neq + i] = L.f[i];
     if (A.rate_out)
         for (int k = tid(); k < N.nreac; k += NT) A.rate_out[(size_t)p * N.nreac + k] = L.rate[k];
+    if (A.flux_out)   // the fluxes stay in the first nreac slots of the region after block_rhs
+        for (int k = tid(); k < N.nreac; k += NT) A.flux_out[(size_t)p * N.nreac + k] = L.flux[k];
 }
 
 __global__ __launch_bounds__(NT) void jac_kernel(const DevNet *__restrict__ Np, PieceArgs A)

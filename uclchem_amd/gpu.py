@@ -299,6 +299,20 @@ class Network:
                              _p(ydot), _p(rate_out)), "ucg_rhs")
         return ydot, rate_out
 
+    def rhs_fluxes(self, params, phys, rate, y):
+        """(ydot [n, neq], flux [n, nreac]): F and its per-reaction terms at y (ucg_rhs_fluxes)."""
+        params = np.atleast_2d(params)
+        n = params.shape[0]
+        params = _rows(params, n, NPARAM, "params")
+        phys = _rows(phys, n, 8, "phys")
+        rate = _rows(rate, n, self.nreac, "rate")
+        y = _rows(y, n, self.neq, "y")
+        ydot = np.zeros((n, self.neq))
+        flux = np.zeros((n, self.nreac))
+        _check(lib().ucg_rhs_fluxes(ctypes.c_int(self.handle), ctypes.c_int(n), _p(params), _p(phys), _p(rate), _p(y),
+                                    _p(ydot), _p(flux)), "ucg_rhs_fluxes")
+        return ydot, flux
+
     def jacobian(self, params, phys, rate, y):
         """Returns J[n, neq, neq] with J[p, i, j] = d ydot_i / d y_j."""
         params = np.atleast_2d(params)

@@ -20,6 +20,7 @@ Outputs (small .npz files next to this script):
                          (t_end log-uniform 1e3..1e7 yr)
   ref_coeff.npz          the static cloud to 1e5 yr with reaction coefficients changed (the
                          alpha/beta/gamma keys) and the rate array at five output rows
+  ref_start_hotcore.npz  a model from given start abundances and a hot-core model (temp_indx 3)
   ref_collapse_models.npz  the reference's collapse module (collapse.f90: BE1.1, BE4, filament,
                          ambipolar) for two points each: the physics of every output row, the
                          abundances of five rows (collapse_rows), and those at tight tolerances
@@ -120,6 +121,38 @@ def _model_capped(args):
             return None
         z = np.load(tmp.name)
         return int(z["rc"]), z["final"].copy(), int(z["nrows"]), z["stats"].copy(), float(z["wall"])
+
+
+def start_and_hotcore_fixture():
+    """ref_start_hotcore.npz: (1) a model started from given abundances (abundanceStart,
+    wrap.f90:636-640): the static cloud (n_H 1e4, 10 K) to 1e3 yr, its final abundances then the
+    start of a denser, warmer cloud (n_H 1e5, 20 K) to 1e4 yr; (2) the hot core of the reference's
+    model API (temp_indx 3, maxTemp 300 K, n_H 1e5) to 2e4 yr: every row's time, density, gasTemp
+    and Av, and the final abundances, also at tight tolerances (reltol 1e-11, abstol_factor 1e-17,
+    abstol_min 1e-30)."""
+    from oracle import RefLib
+    from uclchem_amd.params import INDEX, make_params
+    ref = RefLib()
+    p1 = make_params(initialdens=1e4, initialtemp=10.0, finaltime=1e3)
+    rc1, rows1, _ = ref.model(p1, maxrec=200)
+    y_start = np.append(rows1[-1, 4:], 1e5)          # abundances, then the next model's density
+    p2 = make_params(initialdens=1e5, initialtemp=20.0, finaltime=1e4)
+    rc2, rows2, _ = ref.model_from(p2, y_start, maxrec=200)
+    p3 = make_params(dict(model=1, tempindx=3, maxtemp=300.0), initialdens=1e5, finaltime=2e4)
+    rc3, rows3, _ = ref.model(p3, maxrec=400)
+    assert rc1 == 0 and rc2 == 0 and rc3 == 0
+
+    def tight(p):
+        t = p.copy()
+        t[INDEX["reltol"]], t[INDEX["abstol_factor"]], t[INDEX["abstol_min"]] = 1e-11, 1e-17, 1e-30
+        return t
+    rc2t, rows2t, _ = ref.model_from(tight(p2), y_start, maxrec=200)
+    rc3t, rows3t, _ = ref.model(tight(p3), maxrec=400)
+    assert rc2t == 0 and rc3t == 0 and len(rows2t) == len(rows2) and len(rows3t) == len(rows3)
+    np.savez_compressed(os.path.join(HERE, "ref_start_hotcore.npz"), start_params=p2, y_start=y_start,
+                        start_rows=rows2[:, :4], start_final=rows2[-1], start_final_tight=rows2t[-1],
+                        hot_params=p3, hot_rows=rows3[:, :4], hot_final=rows3[-1], hot_final_tight=rows3t[-1])
+    print(f"ref_start_hotcore: start {len(rows2)} rows, hot core {len(rows3)} rows", flush=True)
 
 
 COLLAPSE_MODES = {"BE1.1": 1, "BE4": 2, "filament": 3, "ambipolar": 4}
@@ -426,6 +459,8 @@ def main():
     ap.add_argument("--only", default=None, help="comma-separated fixture rows to (re)compute with --*-tight")
     ap.add_argument("--cap", type=float, default=900.0, help="reference wall-time cap per grid point, s")
     ap.add_argument("--collapse-only", action="store_true", help="only the collapse-module models")
+    ap.add_argument("--start-hotcore-only", action="store_true",
+                    help="only the given-start-abundances and hot-core models (ref_start_hotcore.npz)")
     ap.add_argument("--grid-variants", action="store_true",
                     help="add the answers at half and twice the relative tolerance to ref_cloud_grid.npz")
     ap.add_argument("--ref-lib", default=os.path.join(ROOT, "oracle", "_ref", "libucref.so"),
@@ -434,6 +469,9 @@ def main():
     only = None if a.only is None else [int(x) for x in a.only.split(",")]
     if a.collapse_only:
         collapse_fixture(a.procs, a.cap, a.ref_lib)
+        return
+    if a.start_hotcore_only:
+        start_and_hotcore_fixture()
         return
     if a.grid_variants:
         grid_variants(a.procs, a.cap)

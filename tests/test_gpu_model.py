@@ -47,14 +47,30 @@ def test_out_species_format():
     assert res[0][1] == chem[-1, 0, names.index("CO")]
 
 
-def test_hot_core_grid_runs():
+def test_hot_core_grid_matches_reference():
+    """model.hot_core_grid (the reference's model.hot_core, hotcore.f90's temperature schedule)
+    against the reference (tests/golden/ref_start_hotcore.npz): temp_indx 3, maxTemp 300 K,
+    n_H 1e5 to 2e4 yr, every output row's time, density, gasTemp and Av, and the final
+    abundances above 1e-14 within 1% plus twice the reference's own integration noise."""
+    from helpers import excess_over_reference
     from uclchem_amd import model
     with pytest.raises(ValueError):
         model.hot_core_grid(7, 300.0, [{}])
-    phys, chem, _, _, flags = model.hot_core_grid(3, 300.0, [{"initialDens": 1e6, "finalTime": 2e4}],
-                                                  return_array=True)
+    path = os.path.join(GOLD, "ref_start_hotcore.npz")
+    if not os.path.exists(path):
+        pytest.skip("ref_start_hotcore.npz not generated (make_ref_fixtures.py --start-hotcore-only)")
+    f = np.load(path)
+    n = len(f["hot_rows"])
+    phys, chem, _, start, flags = model.hot_core_grid(3, 300.0, [{"initialDens": 1e5, "finalTime": 2e4}],
+                                                      return_array=True, timepoints=n + 8)
     assert flags[0] == 0
-    assert phys[-1, 0, 2] > 10.0          # the gas warmed up along the hot-core schedule
+    np.testing.assert_allclose(phys[:n, 0, 0], f["hot_rows"][:, 0], rtol=1e-12)
+    np.testing.assert_allclose(phys[:n, 0, 1:3], f["hot_rows"][:, 1:3], rtol=1e-12)
+    np.testing.assert_allclose(phys[:n, 0, 4], f["hot_rows"][:, 3], rtol=1e-12)
+    assert phys[n - 1, 0, 2] > 40.0          # the gas warmed along the hot-core schedule
+    e = excess_over_reference(start[0], f["hot_final"][4:], f["hot_final_tight"][4:])
+    print(f"[hot core] largest excess over the 1% + noise allowance: {e:.3g}", flush=True)
+    assert e <= 1.0
 
 
 def test_get_rates_and_odes_match_oracle(tables, oracle):

@@ -246,6 +246,23 @@ def test_hot_core_schedule_and_temperature_match_example_output(oracle):
     assert temps[-1] == 300.0 and min(temps) == 10.0
 
 
+def test_hot_core_schedule_and_temperature_match_compiled_reference(oracle):
+    """The same schedule and ramp at full precision against the reference compiled in place
+    (tests/golden/ref_start_hotcore.npz: temp_indx 3, maxTemp 300 K, n_H 1e5 to 2e4 yr, 113
+    rows of time, density, gasTemp and Av): every row equal."""
+    g = np.load(os.path.join(GOLD, "ref_start_hotcore.npz"))
+    rows = g["hot_rows"]
+    pt, _ = oracle.init_point(g["hot_params"])
+    t, got = 0.0, [(0.0, pt.density, pt.gas_temp, pt.av)]
+    while len(got) < len(rows):
+        t1 = oracle.next_target(1, t)
+        oracle.update_physics(pt, t1)
+        t = t1 / 3.16e7
+        got.append((t, pt.density, pt.gas_temp, pt.av))
+    assert np.array_equal(np.array(got), rows)
+    assert oracle.next_target(1, t) / 3.16e7 > 2e4 >= rows[-2, 0]   # the last row is the first past finalTime
+
+
 def test_free_fall_density_matches_example_output(oracle):
     """The oracle's free-fall density derivative (physics-core.f90:90-103, the last ODE row)
     integrated alone (classical RK4, 50 substeps per output interval) reproduces

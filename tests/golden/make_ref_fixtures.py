@@ -6,10 +6,9 @@ Run in the build container (needs /root/reference for the build and the example 
 Outputs (small .npz files next to this script):
   static_full_t1e6.npz   examples/example-output/static-full.dat row t=1e6 yr (6 digits)
   ref_cloud_point.npz    the reference model for the default static cloud point
-  ref_cloud_grid.npz     the reference model for points of the 4096-point grid, with its answers
-                         at tight, half and twice its default tolerances (--grid-tight,
-                         --grid-variants)
-  ref_hot_core.npz       the reference model for seeded hot-core points
+  ref_cloud_grid.npz     the reference model for points of the 4096-point grid, one per density
+                         stratum (--grid-only), with its answers at tight, half and twice its
+                         default tolerances (--grid-tight, --grid-variants)
   example_collapse_hotcore.npz  examples/example-output/phase1-full.dat, startcollapse.dat,
                          phase2-full.dat (the collapse and hot-core example models)
   ref_collapse500.npz    the reference compiled on the 500/6000 synthetic network (written by
@@ -20,7 +19,9 @@ Outputs (small .npz files next to this script):
                          (t_end log-uniform 1e3..1e7 yr)
   ref_coeff.npz          the static cloud to 1e5 yr with reaction coefficients changed (the
                          alpha/beta/gamma keys) and the rate array at five output rows
-  ref_start_hotcore.npz  a model from given start abundances and a hot-core model (temp_indx 3)
+  ref_start_hotcore.npz  a model from given start abundances and a hot-core model (temp_indx 3,
+                         300 K, n_H 1e5 to 2e4 yr), with the final abundances at tight
+                         tolerances (--start-hotcore-only)
   ref_collapse_models.npz  the reference's collapse module (collapse.f90: BE1.1, BE4, filament,
                          ambipolar) for two points each: the physics of every output row, the
                          abundances of five rows (collapse_rows), and those at tight tolerances
@@ -446,7 +447,6 @@ def synthetic_fixtures(procs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=24)
-    ap.add_argument("--hot", type=int, default=4)
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--examples-only", action="store_true", help="only convert the reference's example outputs")
     ap.add_argument("--synthetic-only", action="store_true", help="only the synthetic-network fixtures")
@@ -524,10 +524,9 @@ def main():
     example_models()
     if a.examples_only:
         return
-    # 2. reference models
-    for name, params in [("ref_cloud_point", workloads.static_cloud_point()),
-                         ("ref_cloud_grid", workloads.static_cloud_grid(a.grid)),
-                         ("ref_hot_core", workloads.hot_core_grid(a.hot))]:
+    # 2. the reference model of the default static cloud point (the grid sample and the other
+    # fixtures have flags of their own)
+    for name, params in [("ref_cloud_point", workloads.static_cloud_point())]:
         t0 = time.time()
         rc, final, stats, wall, nrows = run_models(params, a.procs)
         np.savez_compressed(os.path.join(HERE, name + ".npz"), params=params, rc=rc, final=final, stats=stats,
